@@ -1,0 +1,234 @@
+"""Benchmark of the Natrix fluid step on one MI355X.
+
+Default workload: a 2048x2048 float32 scene (velocity, pressure, dye off)
+with a circular obstacle (seeded centre in [0.3, 0.7]^2, radius 0.1*N, held
+in the persistent obstacle map), inflow 1.0 on the left wall, vorticity
+confinement 0.3, no viscosity, dt = 1/60, speed 500, 80 Jacobi sweeps per
+step from a zero guess.  One step = one FluidSimulator.update(dt) on inputs
+already resident in HBM.
+
+Prints one JSON line (steps/s, roofline of the dominant Jacobi kernel
+measured with HIP events on the simulator's stream, and the CPU oracle timed
+on a bounded sample of the same workload).
+
+    python bench.py [--steps K] [--warmup W] [--size N] [--iters I]
+                    [--mode fixed|tol] [--tol 1e-4] [--tile k,rows,waves]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("sim steps/sec at 2048² @ rel. residual ≤1e-4 "
+          "(learned+Jacobi vs plain); % HBM peak")
+HBM_PEAK_GBS = 8000.0
+JACOBI_BYTES_PER_CELL_SWEEP = 13   # read p 4 + b 4 + mask 1, write p 4
+STEP_BYTES_PER_CELL = 104          # the other stages, FP32, dye on, nu = 0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_scene(N, seed=2):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.3, 0.7, 2)
+    X = np.arange(N, dtype=np.float32)[None, :]
+    Y = np.arange(N, dtype=np.float32)[:, None]
+    sx, sy = np.float32(c[0]) * np.float32(N), np.float32(c[1]) * np.float32(N)
+    d = np.sqrt((sx - X) ** 2 + (sy - Y) ** 2)
+    solid = d <= np.float32(0.1 * N)
+    return solid, tuple(float(v) for v in c)
+
+
+def configure(sim, args):
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    sim.iterations = args.iters
+    sim.bc_mode = "inflow"
+    sim.inflow = 1.0
+    sim.jacobi_tile = tuple(args.tile)
+    if args.mode == "tol":
+        sim.tol = args.tol
+        sim.max_iters = args.max_iters
+        sim.check_every = args.check_every
+
+
+def cpu_baseline(N, iters, budget_s):
+    """The oracle (NumPy float32, one thread) on the same scene: whole steps
+    until the time budget is spent (at least one)."""
+    from oracle import fluid as F
+    solid, _ = make_scene(N)
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=iters,
+                   bc_mode=F.BC_INFLOW)
+    u = np.zeros((N, N, 2), np.float32)
+    p = np.zeros((N, N), np.float32)
+    obs = np.zeros((N, N, 2), np.float32)
+    dt = np.float32(1 / 60)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        u, p, obs, _ = F.step(u, p, obs, dt, prm, persistent=solid)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return steps / el, steps, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--iters", type=int, default=80)
+    ap.add_argument("--mode", choices=["fixed", "tol"], default="fixed")
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--max-iters", type=int, default=200000)
+    ap.add_argument("--check-every", type=int, default=1024)
+    ap.add_argument("--tile", type=lambda s: [int(v) for v in s.split(",")],
+                    default=[8, 8, 8])
+    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the copy-bandwidth probe (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the fluid "
+                          "step of one grid is not sharded (run with --gpus 1)"}))
+        sys.exit(2)
+
+    import ctypes
+
+    import torch
+    from natrix_amd import FluidSimulator, _lib
+
+    def copy_bandwidth(nbytes, iters):
+        g = ctypes.c_double()
+        _lib.call("nx_copy_bandwidth", nbytes, iters, ctypes.byref(g))
+        return g.value
+
+    if args.probe_copy:
+        gbs = copy_bandwidth(args.probe_copy, 10)
+        print(json.dumps({"probe": "copy", "bytes": args.probe_copy,
+                          "launches": 11, "GBps": round(gbs, 1)}), flush=True)
+        return
+
+    N = args.size
+    solid, centre = make_scene(N)
+    log(f"bench: {N}x{N} mode={args.mode} iters={args.iters} tile={args.tile}")
+    sim = FluidSimulator(N, N)
+    configure(sim, args)
+    sim.set_obstacle_map(solid)
+    if args.graph:
+        sim.use_graph = True
+    dt = 1.0 / 60.0
+    for _ in range(args.warmup):
+        sim.update(dt)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    log("bench: warmup done")
+
+    sim.set_timing(not args.graph)
+    sim.collect_timing()
+    sweeps_total = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sim.update(dt)
+        if args.mode == "tol":
+            sweeps_total += sim.stats()["sweeps"]
+    sim.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing = sim.collect_timing()
+    sim.set_timing(False)
+    if args.mode == "fixed":
+        sweeps_total = args.iters * args.steps
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    rel, _ = sim.residual()
+    st = sim.stats()
+    log(f"bench: {args.steps} steps in {elapsed:.3f}s, residual {rel:.3e}")
+
+    cells = N * N
+    jac_ms, jac_launches = timing["jacobi"]
+    # the first sweep of every step runs inside the fused divergence kernel
+    tb_sweeps = sweeps_total - args.steps
+    roof = None
+    if jac_launches > 0 and jac_ms > 0:
+        jac_bytes = JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps
+        achieved = jac_bytes / (jac_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_jacobi_tb",
+                "avg_launch_us": round(1e3 * jac_ms / jac_launches, 3),
+                "sweeps_per_launch": round(tb_sweeps / jac_launches, 3)}
+    step_bytes = cells * (STEP_BYTES_PER_CELL +
+                          JACOBI_BYTES_PER_CELL_SWEEP * sweeps_total / args.steps)
+    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
+
+    copy_gbs = copy_bandwidth(1 << 30, 20)
+    log(f"bench: copy probe {copy_gbs:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU oracle baseline")
+        sps, nst, el = cpu_baseline(N, args.iters, args.cpu_budget)
+        cpu = {"value": round(sps, 5), "unit": "steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"oracle/fluid.py step() on the same {N}x{N} scene, "
+                         f"{args.iters} sweeps, {nst} steps in {el:.1f} s, "
+                         f"NumPy float32 single thread "
+                         f"(host reports {os.cpu_count()} cpus)"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(args.steps / elapsed, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"fluid step {N}x{N} f32, circular obstacle r=0.1N at "
+                         f"({centre[0]:.3f},{centre[1]:.3f}), left inflow 1.0, "
+                         f"vorticity 0.3, "
+                         + (f"{args.iters} Jacobi sweeps/step" if args.mode == "fixed"
+                            else f"Jacobi to rel. residual <= {args.tol:g}")),
+            "grid": N, "solver": f"plain fused Jacobi ({args.mode})",
+            "jacobi_tile": args.tile, "graph": bool(args.graph),
+        },
+        "sweeps_per_step": round(sweeps_total / args.steps, 2),
+        "final_rel_residual": rel,
+        "step_algorithmic_GBps": round(step_gbs, 1),
+        "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "stage_ms_per_step": {k: round(v[0] / args.steps, 5)
+                              for k, v in timing.items() if v[1]},
+        "copy_probe_GBps": round(copy_gbs, 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if args.mode == "tol":
+        out["converged_last_step"] = st["converged"]
+    sim.destroy()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/* natrix.h -- C-ABI of the MI355X (gfx950) Natrix fluid-step library.
+ *
+ * The reference (fbertola/Natrix) drives its fluid step from Python through
+ * bgfx: it sets uniforms, binds buffers to slots and dispatches one compute
+ * shader per stage.  This library replaces that boundary: one opaque
+ * simulator object per grid owns its device buffers and HIP stream, and
+ * every reference operation is one exported function.  All functions return
+ * NX_OK (0) or a negative status; nx_last_error() gives the message of the
+ * calling thread's last failure.  Calls enqueue on the simulator's stream
+ * and return without waiting, except nx_download, nx_residual,
+ * nx_synchronize, nx_get_stats and the tolerance-driven nx_update.
+ * A simulator is not thread-safe (the reference drives bgfx from one thread).
+ */
+#ifndef NATRIX_H
+#define NATRIX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NX_OK 0
+#define NX_ERR_ARG (-1)   /* invalid argument (the reference raises ValueError) */
+#define NX_ERR_HIP (-2)   /* HIP runtime failure */
+#define NX_ERR_STATE (-3) /* call not valid in the simulator's current state */
+
+/* Field identifiers for nx_upload / nx_download / nx_field_ptr.  Layouts are
+ * row-major, index y*W + x (shader.AddVelocity.comp:26). */
+#define NX_FIELD_VELOCITY 0   /* float2 (x, y) per cell: the current velocity */
+#define NX_FIELD_PRESSURE 1   /* float: the current pressure iterate */
+#define NX_FIELD_DIVERGENCE 2 /* float: the Poisson right-hand side */
+#define NX_FIELD_CURL 3       /* float: vorticity scratch */
+#define NX_FIELD_MASK 4       /* uint8: obstacle bits, bit0 = float2 channel x,
+                                 bit1 = channel y, bit2 = persistent map */
+#define NX_FIELD_DYE 5        /* float on the dye grid */
+
+#define NX_BC_REFERENCE 0 /* zero ring when has_borders (InitBoundaries.comp) */
+#define NX_BC_FREE_SLIP 1 /* normal velocity 0, tangential copied inward */
+#define NX_BC_INFLOW 2    /* left inflow, right outflow, free-slip top/bottom */
+
+typedef struct nx_sim nx_sim;
+
+/* Parameters.  The first six mirror the reference's properties and class
+ * flags (natrix/core/fluid_simulator.py:28-35, validation :58-111). */
+typedef struct nx_params {
+  float speed;           /* > 0, default 500 */
+  float dissipation;     /* > 0, default 1 */
+  float vorticity;       /* >= 0, default 0 */
+  float viscosity;       /* >= 0, default 0.1; the viscosity sweep runs if > 0 */
+  int iterations;        /* > 0, default 50: Jacobi sweeps per step */
+  int has_borders;       /* default 1 */
+  int bc_mode;           /* NX_BC_*, default NX_BC_REFERENCE */
+  float inflow;          /* inflow speed for NX_BC_INFLOW, default 1 */
+  float dye_speed;       /* dye back-trace speed (SmoothParticlesArea._speed) */
+  float dye_dissipation; /* dye dissipation, default 1 */
+  int dye_in_step;       /* 1 (default): nx_update advects the dye after the
+                            step; 0: the caller runs nx_stage_advect_dye */
+  int fused;             /* 1 (default): fused kernels; 0: one kernel per
+                            reference shader in the reference order */
+  int jacobi_k;          /* sweeps per Jacobi launch (temporal blocking) */
+  int jacobi_rows;       /* register rows per wavefront of the Jacobi tile */
+  int jacobi_waves;      /* wavefronts per Jacobi block */
+  double tol;            /* > 0: solve each step until the relative residual
+                            is <= tol (then `iterations` is ignored) */
+  int max_iters;         /* sweep cap of the tolerance-driven solve */
+  int check_every;       /* sweeps between residual checks */
+} nx_params;
+
+/* Statistics of the last pressure solve. */
+typedef struct nx_stats {
+  int sweeps;          /* Jacobi sweeps applied to the final pressure */
+  int converged;       /* tolerance-driven solve: 1 if rel residual <= tol */
+  double residual;     /* last measured relative residual (FP64) */
+  int residual_sweeps; /* iterate index that residual was measured on */
+} nx_stats;
+
+/* ---- lifetime (FluidSimulator.__init__ / destroy, fluid_simulator.py:37-48,
+ *      476-515) ---- */
+int nx_create(int width, int height, nx_sim** out);
+int nx_destroy(nx_sim* sim);
+int nx_default_params(nx_params* out);
+int nx_get_params(const nx_sim* sim, nx_params* out);
+/* validates like the reference setters; returns NX_ERR_ARG on bad values */
+int nx_set_params(nx_sim* sim, const nx_params* prm);
+/* the same validation without a simulator (no GPU needed) */
+int nx_check_params(const nx_params* prm);
+/* allocate a dye field of (width x height) driven by this simulator's
+ * velocity (SmoothParticlesArea, demo/smooth_particles_area.py:26-43) */
+int nx_enable_dye(nx_sim* sim, int width, int height);
+
+/* ---- forcing and obstacles ---- */
+/* FluidSimulator.add_velocity (fluid_simulator.py:116-131,
+ * shader.AddVelocity.comp): cone splat, then every cell clamped to [-1,1] */
+int nx_add_velocity(nx_sim* sim, float px, float py, float vx, float vy,
+                    float radius);
+/* FluidSimulator.add_circle_obstacle (:135-153, AddCircleObstacle.comp) */
+int nx_add_circle_obstacle(nx_sim* sim, float px, float py, float radius,
+                           int is_static);
+/* FluidSimulator.add_triangle_obstacle (:156-172, AddTriangleObstacle.comp) */
+int nx_add_triangle_obstacle(nx_sim* sim, float x1, float y1, float x2,
+                             float y2, float x3, float y3, int is_static);
+/* SmoothParticlesArea.add_particles (smooth_particles_area.py:70-83,
+ * shader.AddParticle.comp): cone dye splat clamped to [0,255] */
+int nx_add_dye(nx_sim* sim, float px, float py, float amount, float radius);
+/* Gaussian splats (no reference counterpart): n records of 6 floats
+ * (px, py, ax, ay, radius, dye); velocity += (ax, ay) * w, dye += dye * w,
+ * w = exp(-d^2 / r^2).  Amplitudes are final (already scaled by dt). */
+int nx_add_gaussian_splats(nx_sim* sim, const float* splats, int n);
+/* Persistent obstacle map (no reference counterpart): W*H bytes, non-zero =
+ * solid.  Survives the end-of-step obstacle clear. */
+int nx_set_obstacle_map(nx_sim* sim, const uint8_t* map);
+
+/* ---- the step (FluidSimulator.update, fluid_simulator.py:174-280, then the
+ *      dye advection SmoothParticlesArea.update, :85-104, if dye is on) ---- */
+int nx_update(nx_sim* sim, float dt);
+/* Capture the fixed-iteration step into a HIP graph and replay it on later
+ * nx_update calls with the same dt and parameters (on = 1). */
+int nx_set_graph(nx_sim* sim, int on);
+
+/* ---- per-stage entry points (one reference shader each), operating on the
+ *      simulator's current buffers in the reference's slot semantics ---- */
+int nx_stage_boundaries(nx_sim* sim);               /* InitBoundaries (+modes) */
+int nx_stage_advect(nx_sim* sim, float dt);         /* AdvectVelocity + flip */
+int nx_stage_curl(nx_sim* sim);                     /* CalcVorticity */
+int nx_stage_confine(nx_sim* sim, float dt);        /* ApplyVorticity + flip */
+int nx_stage_viscosity(nx_sim* sim);                /* Viscosity + flip */
+int nx_stage_divergence(nx_sim* sim);               /* Divergence */
+int nx_stage_clear_pressure(nx_sim* sim);           /* ClearBuffer on p */
+int nx_stage_jacobi(nx_sim* sim, int sweeps);       /* Poisson + flip, xN */
+int nx_stage_subtract_gradient(nx_sim* sim);        /* SubtractGradient + flip */
+int nx_stage_clear_obstacles(nx_sim* sim);          /* ClearBuffer on obstacles */
+int nx_stage_advect_dye(nx_sim* sim, float dt);     /* AdvectParticle + flip */
+/* fused middle stage: curl, confinement, viscosity, divergence, p1 */
+int nx_stage_fused_vort_div(nx_sim* sim, float dt);
+/* `sweeps` Jacobi sweeps with the temporal-blocked kernel, k per launch */
+int nx_jacobi_blocked(nx_sim* sim, int sweeps, int k, int rows, int waves);
+
+/* ---- state access ---- */
+int nx_upload(nx_sim* sim, int field, const void* src, size_t bytes);
+int nx_download(nx_sim* sim, int field, void* dst, size_t bytes);
+/* device pointer of the current buffer of `field` (borrowed: valid until
+ * the next call that flips that field) -- get_velocity_buffer,
+ * fluid_simulator.py:113-114 */
+int nx_field_ptr(nx_sim* sim, int field, void** dev_ptr, size_t* bytes);
+/* FP64 relative residual of the current pressure against the current
+ * divergence: out[6] = {sum r, sum r^2, sum b, sum b^2, fluid cells, rel} */
+int nx_residual(nx_sim* sim, double* out);
+int nx_get_stats(nx_sim* sim, nx_stats* out);
+int nx_synchronize(nx_sim* sim);
+/* the simulator's HIP stream (hipStream_t) */
+int nx_get_stream(nx_sim* sim, void** stream);
+
+/* ---- timing: HIP events around every launch of the selected kernels ---- */
+#define NX_TIME_ADVECT 0
+#define NX_TIME_VORT_DIV 1
+#define NX_TIME_JACOBI 2
+#define NX_TIME_PROJECT 3
+#define NX_TIME_DYE 4
+#define NX_TIME_SPLAT 5
+#define NX_TIME_COUNT 6
+int nx_set_timing(nx_sim* sim, int on);
+/* synchronises; per tag: total ms and number of launches since the last
+ * reset; resets the accumulators */
+int nx_collect_timing(nx_sim* sim, double* ms, int* launches);
+
+/* attainable bandwidth: a plain device copy of `bytes` (multiple of 16),
+ * `iters` times; *gbps = (read + write bytes) / time */
+int nx_copy_bandwidth(size_t bytes, int iters, double* gbps);
+
+const char* nx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NATRIX_H */

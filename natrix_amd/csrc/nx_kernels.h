@@ -1,0 +1,98 @@
+// Host-side launchers for the fluid-step kernels.  Every launcher enqueues
+// on the given stream and returns hipGetLastError() of the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nx_common.h"
+
+namespace nx {
+
+struct Splat {  // Gaussian splat: u += (ax, ay) * w, dye += dye * w
+  float px, py, ax, ay, radius, dye;
+};
+
+// ---- per-stage kernels (one reference shader each) -------------------
+hipError_t launch_bc(float2* u, int W, int H, BcParams bc, hipStream_t s);
+hipError_t launch_advect(const float2* uin, float2* uout, const uint8_t* mask,
+                         int W, int H, float dt, float speed, float diss,
+                         BcParams bc, hipStream_t s);
+hipError_t launch_curl(const float2* u, float* w, int W, int H, hipStream_t s);
+hipError_t launch_confine(const float2* uin, const float* w, float2* uout,
+                          int W, int H, float dt, float scale, hipStream_t s);
+hipError_t launch_viscosity(const float2* uin, float2* uout, int W, int H,
+                            float alpha, float rbeta, hipStream_t s);
+hipError_t launch_divergence(const float2* u, const uint8_t* mask, float* div,
+                             int W, int H, hipStream_t s);
+hipError_t launch_jacobi_sweep(const float* pin, const float* b,
+                               const uint8_t* mask, float* pout, int W, int H,
+                               hipStream_t s);
+hipError_t launch_subtract_gradient(const float2* uin, const float* p,
+                                    const uint8_t* mask, float2* uout,
+                                    uint8_t* mask_next, int W, int H,
+                                    hipStream_t s);
+hipError_t launch_fill(float* a, float v, size_t n, hipStream_t s);
+hipError_t launch_mask_keep(uint8_t* m, uint8_t keep, size_t n, hipStream_t s);
+hipError_t launch_add_velocity(const float2* uin, float2* uout, int W, int H,
+                               float px, float py, float vx, float vy,
+                               float r, hipStream_t s);
+hipError_t launch_circle(uint8_t* mask, int W, int H, float px, float py,
+                         float r, hipStream_t s);
+hipError_t launch_triangle(uint8_t* mask, int W, int H, float x1, float y1,
+                           float x2, float y2, float x3, float y3,
+                           int is_static, hipStream_t s);
+hipError_t launch_advect_dye(const float* qin, float* qout, const float2* u,
+                             const uint8_t* mask, int Wp, int Hp, int Wv,
+                             int Hv, float dt, float speed, float diss,
+                             hipStream_t s);
+hipError_t launch_add_dye(const float* qin, float* qout, int W, int H,
+                          float px, float py, float val, float r,
+                          hipStream_t s);
+// `splats` is a host array (passed to the kernels by value)
+hipError_t launch_gaussian_splats(float2* u, float* q, int W, int H,
+                                  const Splat* splats, int n, hipStream_t s);
+
+// ---- fused fast path ---------------------------------------------------
+// curl + confinement (+ viscosity) + divergence + first Jacobi sweep from a
+// zero guess: writes the projected-step velocity input, b and p1.
+hipError_t launch_fused_vort_div(const float2* uin, const uint8_t* mask,
+                                 float2* uout, float* div, float* p1, int W,
+                                 int H, float dt, float scale, int viscous,
+                                 float alpha, float rbeta, hipStream_t s);
+
+// Jacobi sweeps: cfg.k consecutive sweeps per launch (temporal blocking),
+// bitwise equal to k single sweeps.  If `red` is non-null the launch also
+// writes FP64 residual partials of its LAST sweep's input iterate to
+// red[2*block + {0,1}] (sum r, sum r^2 over fluid cells).  `done`
+// (nullable) is a device flag: when non-zero the launch returns at once.
+struct JacobiCfg {
+  int k;      // sweeps per launch
+  int rows;   // rows per wave
+  int waves;  // waves per block
+};
+int jacobi_launch_blocks(int W, int H, JacobiCfg cfg);
+hipError_t launch_jacobi_tb(const float* pin, const float* b,
+                            const uint8_t* mask, float* pout, int W, int H,
+                            JacobiCfg cfg, double* red, const int* done,
+                            hipStream_t s);
+bool jacobi_cfg_supported(JacobiCfg cfg);
+
+// Exact FP64 residual sums of `p` (of a zero guess when p == nullptr) per
+// 64x4 block: red[5*blk + {sum r, sum r^2, sum b, sum b^2, fluid count}].
+int residual_blocks(int W, int H);
+hipError_t launch_sums(const float* p, const float* b, const uint8_t* mask,
+                       int W, int H, double* red, hipStream_t s);
+// Deterministic reduction of `nblocks` records of `width` doubles into
+// sums[off..off+width), then sums[5] = relative residual from sums[0..4].
+// With `state` (tolerance-driven solve): the first time rel <= tol it sets
+// state[0]=1, state[1]=sweeps_now, state[2]=final_idx; state[3]=sweeps_now.
+hipError_t launch_reduce(const double* part, int nblocks, int width, int off,
+                         double* sums, double tol, int* state, int sweeps_now,
+                         int final_idx, hipStream_t s);
+
+// b -= fluid mean (sums[2]/sums[4]) and, if p != nullptr, p = first sweep
+// of the centred system, on fluid cells only.
+hipError_t launch_center_rhs(float* b, float* p, const uint8_t* mask, int n,
+                             const double* sums, hipStream_t s);
+
+}  // namespace nx

@@ -1,0 +1,241 @@
+"""Pressure solve and full-step parity on the GPU.
+
+* temporal-blocked Jacobi (k sweeps per launch, every tile shape) is bitwise
+  equal to k single sweeps of the oracle;
+* the FP64 residual matches the oracle's two-pass evaluation;
+* the tolerance-driven solve stops at a residual <= tol, verified on the
+  downloaded pressure by the oracle;
+* fused and reference-order steps match the oracle step bitwise over
+  10-step rollouts; a HIP-graph replay matches eager execution.
+"""
+import numpy as np
+import pytest
+
+from oracle import fluid as F
+from oracle import poisson as P
+from tests.helpers import (assert_bitwise, assert_close, blob_mask, f32,
+                           make_sim, obs_from_mask, random_velocity)
+
+pytestmark = pytest.mark.gpu
+
+TILES = [(4, 8), (4, 16), (8, 4), (8, 8), (8, 16), (12, 4), (12, 8), (16, 4), (16, 8)]
+
+
+@pytest.mark.parametrize("rows,waves", TILES)
+def test_jacobi_blocked_bitwise(gpu, rows, waves):
+    for (H, W), k in [((100, 77), 1), ((100, 77), 5), ((257, 130), 8),
+                      ((33, 200), 3)]:
+        if 64 - 2 * k < 8 or rows * waves - 2 * k < 8:
+            continue
+        rng = np.random.default_rng(H + W + k)
+        m = blob_mask(rng, H, W)
+        b = rng.standard_normal((H, W)).astype(f32)
+        p = rng.standard_normal((H, W)).astype(f32)
+        sim = make_sim(H, W)
+        sim.upload("mask", m)
+        sim.upload("divergence", b)
+        sim.upload("pressure", p)
+        sweeps = 3 * k + 2  # includes a shortened final launch
+        sim.jacobi_blocked(sweeps, k, rows, waves)
+        ref = F.jacobi(p, b, obs_from_mask(m), sweeps)
+        assert_bitwise(sim.download("pressure"), ref,
+                       f"tb {rows}x{waves} k={k} {H}x{W}")
+        sim.destroy()
+
+
+def test_residual_matches_oracle(gpu):
+    H, W = 120, 90
+    rng = np.random.default_rng(5)
+    m = blob_mask(rng, H, W)
+    obs = obs_from_mask(m)
+    b = rng.standard_normal((H, W)).astype(f32)
+    p = F.jacobi(np.zeros((H, W), f32), b, obs, 50)
+    sim = make_sim(H, W)
+    sim.upload("mask", m)
+    sim.upload("divergence", b)
+    sim.upload("pressure", p)
+    rel, sums = sim.residual()
+    ref = F.relative_residual(p, b, obs)
+    assert abs(rel - ref) <= 1e-9 * ref
+    one = F.residual_sums(p, b, obs)
+    np.testing.assert_allclose(sums[:4], one[:4], rtol=1e-10)
+    assert sums[4] == one[4]
+    sim.destroy()
+
+
+def test_residual_of_fp64_solution_is_small(gpu):
+    H, W = 40, 48
+    rng = np.random.default_rng(9)
+    solid = np.zeros((H, W), bool)
+    solid[10:20, 15:22] = True
+    m = solid.astype(np.uint8)
+    b = rng.standard_normal((H, W))
+    b[solid] = 0
+    b[~solid] -= b[~solid].mean()
+    p = P.solve_fp64(b, solid)
+    sim = make_sim(H, W)
+    sim.upload("mask", m)
+    sim.upload("divergence", b.astype(f32))
+    sim.upload("pressure", p.astype(f32))
+    rel, _ = sim.residual()
+    assert rel < 1e-5  # float32 rounding of p and b only
+    sim.destroy()
+
+
+def _scene(H, W, seed, bc_mode=2):
+    rng = np.random.default_rng(seed)
+    u = random_velocity(rng, H, W, 0.5)
+    m = np.zeros((H, W), np.uint8)
+    obs = F.add_circle_obstacle(np.zeros((H, W, 2), f32), (0.4, 0.5), 0.1 * W)
+    m[obs[..., 0] > 0] = 4  # persistent map
+    return u, m
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("bc_mode,nu", [(0, 0.1), (1, 0.0), (2, 0.0)])
+def test_rollout_10_steps_bitwise(gpu, fused, bc_mode, nu):
+    H, W = 96, 128
+    u, m = _scene(H, W, 21)
+    sim = make_sim(H, W, dye=True, bc_mode=bc_mode, fused=fused)
+    sim.vorticity = 0.3
+    sim.viscosity = nu
+    sim.iterations = 20
+    sim.upload("velocity", u)
+    sim.set_obstacle_map(m)
+    q = np.zeros((H, W), f32)
+    prm = F.Params(vorticity=0.3, viscosity=nu, iterations=20, bc_mode=bc_mode)
+    dt = np.float32(1 / 60)
+    pers = m != 0
+    obs = np.zeros((H, W, 2), f32)
+    p = np.zeros((H, W), f32)
+    for step in range(10):
+        splat = (0.25 + 0.05 * step, 0.5, 1.5, 0.3, 4.0, 1.0)
+        sim.add_splats([splat])
+        u, q = F.gaussian_splat(u, q, [tuple(np.float32(v) for v in splat)])
+        # a dynamic obstacle re-added every step, as the demo does
+        sim.add_circle_obstacle((0.75, 0.3), 5.0)
+        obs = F.add_circle_obstacle(obs, (0.75, 0.3), 5.0)
+        sim.update(dt)
+        u, p, obs, q = F.step(u, p, obs, dt, prm, dye=q, persistent=pers)
+    assert_bitwise(sim.download("velocity"), u, "rollout velocity")
+    assert_bitwise(sim.download("pressure"), p, "rollout pressure")
+    assert_bitwise(sim.download("dye"), q, "rollout dye")
+    assert_bitwise(sim.download("mask"), m, "mask after clear")
+    sim.destroy()
+
+
+def test_graph_replay_matches_eager(gpu):
+    H, W = 80, 112
+    u, m = _scene(H, W, 33)
+    sims = []
+    for graph in (False, True):
+        sim = make_sim(H, W, dye=True, bc_mode=2)
+        sim.vorticity = 0.3
+        sim.viscosity = 0.0
+        sim.iterations = 17
+        sim.upload("velocity", u)
+        sim.set_obstacle_map(m)
+        sim.use_graph = graph
+        sims.append(sim)
+    for step in range(6):
+        for sim in sims:
+            sim.add_splats([(0.3, 0.5, 1.0, 0.5, 3.0, 1.0)])
+            sim.update(1 / 60)
+        # every replay checked against the eager result
+        a, b = (s.download("velocity") for s in sims)
+        assert_bitwise(b, a, f"graph replay step {step}")
+        assert_bitwise(sims[1].download("dye"), sims[0].download("dye"), "graph dye")
+    for s in sims:
+        s.destroy()
+
+
+def test_tolerance_solve_reaches_tol(gpu):
+    H, W = 64, 64
+    u, m = _scene(H, W, 44)
+    sim = make_sim(H, W, bc_mode=2)
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    sim.upload("velocity", u)
+    sim.set_obstacle_map(m)
+    sim.tol = 1e-4
+    sim.check_every = 64
+    sim.jacobi_tile = (8, 8, 8)
+    sim.update(1 / 60)
+    st = sim.stats()
+    assert st["converged"] and st["residual"] <= 1e-4
+    p = sim.download("pressure")
+    b = sim.download("divergence")
+    obs = obs_from_mask(m)
+    rel = F.relative_residual(p, b, obs)
+    assert rel <= 1e-4, rel
+    assert abs(rel - st["residual"]) <= 1e-9
+    # the right-hand side is the oracle's divergence centred on its fluid
+    # mean, and p is Jacobi on that system from its first sweep
+    dt = np.float32(1 / 60)
+    v = F.enforce_boundaries(u, F.BC_INFLOW)
+    v = F.advect_velocity(v, obs, dt, 500.0, 1.0)
+    v = F.apply_vorticity(v, F.calc_vorticity(v), dt, 0.3)
+    b0 = F.divergence(v, obs)
+    fluid = m == 0
+    mean = np.float32(b0[fluid].astype(np.float64).sum() / fluid.sum())
+    bc = b0.copy()
+    bc[fluid] = b0[fluid] - mean
+    assert_bitwise(b, bc, "centred rhs")
+    # solid cells keep b0, so (0 - bc) * 0.25 is the first sweep everywhere
+    p1 = (np.float32(0) - bc) * np.float32(0.25)
+    q = F.jacobi(p1, bc, obs, st["sweeps"] - 1)
+    assert_bitwise(p, q, "tolerance solve vs oracle")
+    # it stopped within two check intervals of the first iterate meeting tol
+    r = p1
+    k_first = None
+    for k in range(1, st["sweeps"] + 1):
+        if F.relative_residual(r, bc, obs) <= 1e-4:
+            k_first = k
+            break
+        r = F.jacobi(r, bc, obs, 1)
+    assert k_first is not None and st["sweeps"] - k_first <= 2 * 64 + 8
+    sim.destroy()
+
+
+def test_tolerance_solve_respects_cap(gpu):
+    H, W = 64, 64
+    u, m = _scene(H, W, 45)
+    sim = make_sim(H, W, bc_mode=2)
+    sim.upload("velocity", u)
+    sim.set_obstacle_map(m)
+    sim.viscosity = 0.0
+    sim.tol = 1e-12
+    sim.max_iters = 300
+    sim.check_every = 64
+    sim.update(1 / 60)
+    st = sim.stats()
+    assert not st["converged"] and st["sweeps"] == 300
+    sim.destroy()
+
+
+def test_velocity_buffer_interop_with_torch(gpu):
+    import torch
+    H, W = 32, 48
+    rng = np.random.default_rng(1)
+    u = random_velocity(rng, H, W)
+    sim = make_sim(H, W)
+    sim.upload("velocity", u)
+    buf = sim.get_velocity_buffer()
+    t = torch.as_tensor(buf, device="cuda")
+    assert tuple(t.shape) == (H, W, 2)
+    assert np.array_equal(t.cpu().numpy(), u)
+    sim.destroy()
+
+
+def test_simulate_false_freezes(gpu):
+    H, W = 16, 16
+    sim = make_sim(H, W)
+    u = random_velocity(np.random.default_rng(2), H, W)
+    sim.upload("velocity", u)
+    sim.simulate = False
+    sim.update(1 / 60)
+    sim.add_velocity((0.5, 0.5), (1, 1), 4)
+    sim.add_circle_obstacle((0.5, 0.5), 3)
+    assert_bitwise(sim.download("velocity"), u, "frozen")
+    assert np.all(sim.download("mask") == 0)
+    sim.destroy()
