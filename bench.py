@@ -55,7 +55,8 @@ def configure(sim, args):
     sim.iterations = args.iters
     sim.bc_mode = "inflow"
     sim.inflow = 1.0
-    sim.jacobi_tile = tuple(args.tile)
+    if args.tile:
+        sim.jacobi_tile = tuple(args.tile)
     if args.mode == "tol":
         sim.tol = args.tol
         sim.max_iters = args.max_iters
@@ -96,7 +97,7 @@ def main():
     ap.add_argument("--max-iters", type=int, default=200000)
     ap.add_argument("--check-every", type=int, default=1024)
     ap.add_argument("--tile", type=lambda s: [int(v) for v in s.split(",")],
-                    default=[8, 8, 8])
+                    default=None)
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -212,7 +213,7 @@ def main():
                          + (f"{args.iters} Jacobi sweeps/step" if args.mode == "fixed"
                             else f"Jacobi to rel. residual <= {args.tol:g}")),
             "grid": N, "solver": f"plain fused Jacobi ({args.mode})",
-            "jacobi_tile": args.tile, "graph": bool(args.graph),
+            "jacobi_tile": list(sim.jacobi_tile), "graph": bool(args.graph),
         },
         "sweeps_per_step": round(sweeps_total / args.steps, 2),
         "final_rel_residual": rel,

@@ -62,6 +62,7 @@ typedef struct nx_params {
   int jacobi_k;          /* sweeps per Jacobi launch (temporal blocking) */
   int jacobi_rows;       /* register rows per wavefront of the Jacobi tile */
   int jacobi_waves;      /* wavefronts per Jacobi block */
+  int jacobi_cols;       /* grid columns per lane of the Jacobi tile */
   double tol;            /* > 0: solve each step until the relative residual
                             is <= tol (then `iterations` is ignored) */
   int max_iters;         /* sweep cap of the tolerance-driven solve */
@@ -135,7 +136,8 @@ int nx_stage_advect_dye(nx_sim* sim, float dt);     /* AdvectParticle + flip */
 /* fused middle stage: curl, confinement, viscosity, divergence, p1 */
 int nx_stage_fused_vort_div(nx_sim* sim, float dt);
 /* `sweeps` Jacobi sweeps with the temporal-blocked kernel, k per launch */
-int nx_jacobi_blocked(nx_sim* sim, int sweeps, int k, int rows, int waves);
+int nx_jacobi_blocked(nx_sim* sim, int sweeps, int k, int rows, int waves,
+                      int cols);
 
 /* ---- state access ---- */
 int nx_upload(nx_sim* sim, int field, const void* src, size_t bytes);

@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("jacobi_k", C.c_int),
         ("jacobi_rows", C.c_int),
         ("jacobi_waves", C.c_int),
+        ("jacobi_cols", C.c_int),
         ("tol", C.c_double),
         ("max_iters", C.c_int),
         ("check_every", C.c_int),
@@ -98,7 +99,7 @@ SIGNATURES = {
     "nx_stage_clear_obstacles": [_P],
     "nx_stage_advect_dye": [_P, _F],
     "nx_stage_fused_vort_div": [_P, _F],
-    "nx_jacobi_blocked": [_P, _I, _I, _I, _I],
+    "nx_jacobi_blocked": [_P, _I, _I, _I, _I, _I],
     "nx_upload": [_P, _I, _P, _SZ],
     "nx_download": [_P, _I, _P, _SZ],
     "nx_field_ptr": [_P, _I, C.POINTER(_P), C.POINTER(_SZ)],

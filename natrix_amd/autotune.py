@@ -1,0 +1,118 @@
+"""Autotuner for the temporal-blocked Jacobi kernel's tile shape.
+
+The search space is every (k, rows, waves, cols) the library instantiates
+(k sweeps per launch on a (64*cols) x (rows*waves) register tile).  Each
+candidate runs a fixed number of sweeps on a synthetic scene of the target
+size and is timed with HIP events on the simulator's stream; the fastest
+per sweep wins.  Results are cached per (architecture, dtype, grid) in
+``tuned.json`` next to this file, which ``FluidSimulator`` consults when it
+is created.
+
+    python -m natrix_amd.autotune 2048 1024 512   # tunes and updates tuned.json
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuned.json")
+ARCH = "gfx950"
+
+# (rows, waves, cols) instantiated in csrc/nx_jacobi.hip (NX_TB_SHAPES)
+SHAPES = [(4, 8, 1), (8, 8, 1), (8, 16, 1), (16, 8, 1), (2, 8, 2), (2, 16, 2),
+          (4, 4, 2), (4, 8, 2), (4, 16, 2), (8, 4, 2), (8, 8, 2)]
+KS = (2, 4, 6, 8, 10, 12, 16)
+DEFAULT = (6, 4, 8, 2)
+
+
+def supported(k, rows, waves, cols):
+    return (rows, waves, cols) in SHAPES and k >= 1 and \
+        64 * cols - 2 * k >= 8 and rows * waves - 2 * k >= 8
+
+
+def candidates():
+    return [(k, r, w, c) for (r, w, c) in SHAPES for k in KS if supported(k, r, w, c)]
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def lookup(W, H, dtype="f32"):
+    """Tuned tile for a W x H grid: the exact entry, else the entry with the
+    nearest cell count, else the built-in default."""
+    table = _load().get(ARCH, {}).get(dtype, {})
+    key = f"{W}x{H}"
+    if key in table:
+        return tuple(table[key]["tile"])
+    best, dist = None, None
+    for k, v in table.items():
+        w, h = (int(t) for t in k.split("x"))
+        d = abs((w * h) - W * H)
+        if dist is None or d < dist:
+            best, dist = tuple(v["tile"]), d
+    return best if best is not None else DEFAULT
+
+
+def tune(W, H, dtype="f32", sweeps_per_candidate=None, log=print):
+    """Time every candidate on a W x H scene; returns (best tile, results)."""
+    import numpy as np
+
+    from .core.fluid_simulator import FluidSimulator
+    sim = FluidSimulator(W, H)
+    X = np.arange(W)[None, :]
+    Y = np.arange(H)[:, None]
+    solid = (X - 0.4 * W) ** 2 + (Y - 0.42 * H) ** 2 <= (0.1 * min(W, H)) ** 2
+    sim.set_obstacle_map(solid)
+    rng = np.random.default_rng(0)
+    sim.upload("divergence", (0.01 * rng.standard_normal((H, W))).astype(np.float32))
+    if sweeps_per_candidate is None:
+        # about 1e9 cell-sweeps per candidate, at least 48 sweeps
+        sweeps_per_candidate = max(48, int(1e9 // (W * H)))
+    results = []
+    try:
+        for (k, r, w, c) in candidates():
+            n = max(k, (sweeps_per_candidate // k) * k)
+            sim.jacobi_blocked(2 * k, k, r, w, c)  # warm-up
+            sim.synchronize()
+            sim.set_timing(True)
+            sim.collect_timing()
+            sim.jacobi_blocked(n, k, r, w, c)
+            ms, launches = sim.collect_timing()["jacobi"]
+            sim.set_timing(False)
+            us = 1e3 * ms / n
+            results.append((us, (k, r, w, c)))
+            log(f"  {W}x{H} k={k:2d} rows={r:2d} waves={w:2d} cols={c}: "
+                f"{us:8.3f} us/sweep")
+    finally:
+        sim.destroy()
+    results.sort()
+    return results[0][1], results
+
+
+def store(W, H, tile, us_per_sweep, dtype="f32"):
+    data = _load()
+    data.setdefault(ARCH, {}).setdefault(dtype, {})[f"{W}x{H}"] = {
+        "tile": list(tile), "us_per_sweep": round(us_per_sweep, 4)}
+    tmp = CACHE + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE)
+
+
+def main(argv):
+    sizes = [int(a) for a in argv] or [2048]
+    for n in sizes:
+        best, res = tune(n, n, log=lambda m: print(m, flush=True))
+        store(n, n, best, res[0][0])
+        print(f"best {n}x{n}: tile {best} {res[0][0]:.3f} us/sweep", flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

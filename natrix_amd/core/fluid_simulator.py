@@ -103,8 +103,8 @@ class FluidSimulator:
         self.tol = 0.0
         self.max_iters = 1_000_000
         self.check_every = 256
-        self.jacobi_tile = (self._prm.jacobi_k, self._prm.jacobi_rows,
-                            self._prm.jacobi_waves)
+        from ..autotune import lookup
+        self.jacobi_tile = lookup(self._width, self._height)
         self._dye_speed = 500.0
         self._dye_dissipation = 1.0
         self._dye_in_step = True
@@ -216,7 +216,8 @@ class FluidSimulator:
         p.dye_dissipation = self._dye_dissipation
         p.dye_in_step = 1 if self._dye_in_step else 0
         p.fused = 1 if self.fused else 0
-        p.jacobi_k, p.jacobi_rows, p.jacobi_waves = (int(v) for v in self.jacobi_tile)
+        p.jacobi_k, p.jacobi_rows, p.jacobi_waves, p.jacobi_cols = (
+            int(v) for v in self.jacobi_tile)
         p.tol = float(self.tol)
         p.max_iters = int(self.max_iters)
         p.check_every = int(self.check_every)
@@ -396,9 +397,11 @@ class FluidSimulator:
         else:
             raise ValueError(f"unknown stage {name!r}")
 
-    def jacobi_blocked(self, sweeps, k, rows, waves):
+    def jacobi_blocked(self, sweeps, k, rows, waves, cols=1):
+        """``sweeps`` Jacobi sweeps with the temporal-blocked kernel, k per
+        launch, on a (64*cols) x (rows*waves) register tile."""
         _lib.call("nx_jacobi_blocked", self._alive(), int(sweeps), int(k),
-                  int(rows), int(waves))
+                  int(rows), int(waves), int(cols))
 
     # ------------------------------------------------------------ timing
     def set_timing(self, on=True):

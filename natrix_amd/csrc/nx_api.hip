@@ -148,9 +148,9 @@ static int validate(const nx_sim* s, const nx_params& p) {
   if (p.bc_mode < 0 || p.bc_mode > 2) return fail(NX_ERR_ARG, "unknown bc_mode");
   if (p.bc_mode != BC_REFERENCE && s && (s->W < 3 || s->H < 3))
     return fail(NX_ERR_ARG, "free-slip and inflow walls need a grid of at least 3x3");
-  JacobiCfg c{p.jacobi_k, p.jacobi_rows, p.jacobi_waves};
+  JacobiCfg c{p.jacobi_k, p.jacobi_rows, p.jacobi_waves, p.jacobi_cols};
   if (!jacobi_cfg_supported(c))
-    return fail(NX_ERR_ARG, "unsupported Jacobi tile (k, rows, waves)");
+    return fail(NX_ERR_ARG, "unsupported Jacobi tile (k, rows, waves, cols)");
   if (!(p.tol >= 0)) return fail(NX_ERR_ARG, "tol must be >= 0");
   if (p.tol > 0 && !p.fused)
     return fail(NX_ERR_ARG, "the tolerance-driven solve needs the fused path");
@@ -284,7 +284,7 @@ static int exact_residual(nx_sim* s, double* rel) {
 // can move the last digits); iteration resumes if it is above tol.
 static int solve_tolerance(nx_sim* s, int sweeps_done, bool from_zero) {
   const nx_params& P = s->prm;
-  JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves};
+  JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols};
   NX_TRY(ensure_red2(s, c));
   const int n = s->W * s->H;
   NX_HIP(launch_sums(nullptr, s->div, s->mask[s->mcur], s->W, s->H, s->red5,
@@ -391,7 +391,7 @@ static int step_fused_enqueue(nx_sim* s, float dt, bool tolerance) {
   if (tolerance) {
     NX_TRY(solve_tolerance(s, 1, true));
   } else {
-    JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves};
+    JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols};
     NX_TRY(run_jacobi_tb(s, P.iterations - 1, c));
     s->stats.sweeps = P.iterations;
     s->stats.converged = 0;
@@ -435,9 +435,10 @@ int nx_default_params(nx_params* out) {
   p.dye_dissipation = 1.0f;
   p.dye_in_step = 1;
   p.fused = 1;
-  p.jacobi_k = 8;
-  p.jacobi_rows = 8;
+  p.jacobi_k = 6;
+  p.jacobi_rows = 4;
   p.jacobi_waves = 8;
+  p.jacobi_cols = 2;
   p.tol = 0.0;
   p.max_iters = 1000000;
   p.check_every = 256;
@@ -814,12 +815,13 @@ int nx_stage_fused_vort_div(nx_sim* s, float dt) {
   return NX_OK;
 }
 
-int nx_jacobi_blocked(nx_sim* s, int sweeps, int k, int rows, int waves) {
+int nx_jacobi_blocked(nx_sim* s, int sweeps, int k, int rows, int waves,
+                      int cols) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
   if (sweeps < 0) return fail(NX_ERR_ARG, "sweeps must be >= 0");
-  JacobiCfg c{k, rows, waves};
+  JacobiCfg c{k, rows, waves, cols};
   if (!jacobi_cfg_supported(c))
-    return fail(NX_ERR_ARG, "unsupported Jacobi tile (k, rows, waves)");
+    return fail(NX_ERR_ARG, "unsupported Jacobi tile (k, rows, waves, cols)");
   return run_jacobi_tb(s, sweeps, c);
 }
 

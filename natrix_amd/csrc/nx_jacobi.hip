@@ -44,73 +44,150 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int ROWS, int WAVES>
+__device__ __forceinline__ int ilane_from_left(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ int ilane_from_right(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);
+}
+
+// ROWS x COLS cells per lane: lane `l` of wave `w` owns tile columns
+// [COLS*l, COLS*l + COLS) and tile rows [ROWS*w, ROWS*w + ROWS).
+template <int ROWS, int WAVES, int COLS>
 __global__ void __launch_bounds__(WAVES * 64)
     k_jacobi_tb(const float* __restrict__ pin, const float* __restrict__ b,
                 const uint8_t* __restrict__ mask, float* __restrict__ pout,
                 int W, int H, int k, double* __restrict__ red,
                 const int* __restrict__ done) {
   if (done != nullptr && *done != 0) return;  // block-uniform early exit
-  constexpr int EY = ROWS * WAVES;
+  constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int VX = 64 - 2 * k, VY = EY - 2 * k;
-  const int gx = (int)blockIdx.x * VX - k + lane;
+  const int VX = EX - 2 * k, VY = EY - 2 * k;
+  const int gx0 = (int)blockIdx.x * VX - k + COLS * lane;
   const int gy0 = (int)blockIdx.y * VY - k + wave * ROWS;
-  const bool col_in = gx >= 0 && gx < W;
+  // float2 loads/stores when every lane's column pair is aligned and either
+  // wholly inside or wholly outside the grid and the valid band
+  const bool vec = COLS == 2 && (W % 2 == 0) && (k % 2 == 0);
 
-  float p[ROWS], bb[ROWS];
-  unsigned cd[ROWS];
-  bool any_nbr = false;
+  float p[ROWS][COLS], bb[ROWS][COLS];
+  int sol[ROWS][COLS];  // cell is solid or outside the grid
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     const int gy = gy0 + r;
-    unsigned c = 0x1F;
-    p[r] = 0.0f;
-    bb[r] = 0.0f;
-    if (col_in && gy >= 0 && gy < H) {
-      const int idx = gy * W + gx;
-      p[r] = pin[idx];
-      bb[r] = b[idx];
-      c = (unsigned)(gx == 0 || mask[idx - 1] != 0) |
-          ((unsigned)(gx == W - 1 || mask[idx + 1] != 0) << 1) |
-          ((unsigned)(gy == 0 || mask[idx - W] != 0) << 2) |
-          ((unsigned)(gy == H - 1 || mask[idx + W] != 0) << 3) |
-          ((unsigned)(mask[idx] != 0) << 4);
+    const bool row_in = gy >= 0 && gy < H;
+    if (vec) {
+      const bool in = row_in && gx0 >= 0 && gx0 < W;
+      float2 pv = make_float2(0.0f, 0.0f), bv = pv;
+      int m0 = 1, m1 = 1;
+      if (in) {
+        const int idx = gy * W + gx0;
+        pv = *reinterpret_cast<const float2*>(pin + idx);
+        bv = *reinterpret_cast<const float2*>(b + idx);
+        const uint16_t mm = *reinterpret_cast<const uint16_t*>(mask + idx);
+        m0 = (mm & 0xFF) != 0;
+        m1 = (mm >> 8) != 0;
+      }
+      p[r][0] = pv.x;
+      p[r][COLS - 1] = pv.y;
+      bb[r][0] = bv.x;
+      bb[r][COLS - 1] = bv.y;
+      sol[r][0] = m0;
+      sol[r][COLS - 1] = m1;
+    } else {
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const int gx = gx0 + c;
+        p[r][c] = 0.0f;
+        bb[r][c] = 0.0f;
+        sol[r][c] = 1;
+        if (row_in && gx >= 0 && gx < W) {
+          const int idx = gy * W + gx;
+          p[r][c] = pin[idx];
+          bb[r][c] = b[idx];
+          sol[r][c] = mask[idx] != 0;
+        }
+      }
     }
-    cd[r] = c;
-    any_nbr |= (c & 0xF) != 0;
+  }
+
+  __shared__ float xch[2][2][WAVES][EX];
+  __shared__ int sxch[2][WAVES][EX];
+  // neighbour codes: bit0 L, bit1 R, bit2 B, bit3 T replaced by the centre
+  // (solid or outside the grid: common.sh clamp == Poisson.comp obstacle
+  // rule); bit4 the cell itself is not fluid.
+#pragma unroll
+  for (int c = 0; c < COLS; ++c) {
+    sxch[0][wave][COLS * lane + c] = sol[0][c];
+    sxch[1][wave][COLS * lane + c] = sol[ROWS - 1][c];
+  }
+  __syncthreads();
+  int cd[ROWS][COLS];
+  bool any_nbr = false;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    const int sl = ilane_from_left(sol[r][COLS - 1]);
+    const int sr = ilane_from_right(sol[r][0]);
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      const int L = c > 0 ? sol[r][c - 1] : sl;
+      const int R = c < COLS - 1 ? sol[r][c + 1] : sr;
+      const int B = r > 0 ? sol[r - 1][c]
+                          : (wave > 0 ? sxch[1][wave - 1][COLS * lane + c] : 1);
+      const int T = r < ROWS - 1 ? sol[r + 1][c]
+                                 : (wave < WAVES - 1 ? sxch[0][wave + 1][COLS * lane + c] : 1);
+      const int code = L | (R << 1) | (B << 2) | (T << 3) | (sol[r][c] << 4);
+      cd[r][c] = code;
+      any_nbr |= (code & 0xF) != 0;
+    }
   }
   const bool special = __any(any_nbr);
 
-  __shared__ float xch[2][2][WAVES][64];
   double sr = 0.0, srr = 0.0;
-  const int trow0 = wave * ROWS;  // tile row of p[0]
+  const int trow0 = wave * ROWS;  // tile row of p[0][*]
+  const int tcol0 = COLS * lane;  // tile column of p[*][0]
 
   for (int s = 0; s < k; ++s) {
     const int buf = s & 1;
-    xch[buf][0][wave][lane] = p[0];
-    xch[buf][1][wave][lane] = p[ROWS - 1];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      xch[buf][0][wave][tcol0 + c] = p[0][c];
+      xch[buf][1][wave][tcol0 + c] = p[ROWS - 1][c];
+    }
     __syncthreads();
-    const float below = wave > 0 ? xch[buf][1][wave - 1][lane] : p[0];
-    const float above = wave < WAVES - 1 ? xch[buf][0][wave + 1][lane] : p[ROWS - 1];
+    float prev[COLS], above[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      prev[c] = wave > 0 ? xch[buf][1][wave - 1][tcol0 + c] : p[0][c];
+      above[c] = wave < WAVES - 1 ? xch[buf][0][wave + 1][tcol0 + c] : p[ROWS - 1][c];
+    }
     const bool last = (s == k - 1) && red != nullptr;
-    float prev = below;
-    if (!special) {
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        const float pc = p[r];
-        const float up = r < ROWS - 1 ? p[r + 1] : above;
-        const float x1 = lane_from_left(pc);
-        const float x2 = lane_from_right(pc);
-        const float y1 = prev;
-        const float y2 = up;
-        const float np = ((((x1 + x2) + y1) + y2) - bb[r]) * 0.25f;
+    for (int r = 0; r < ROWS; ++r) {
+      const float lft = lane_from_left(p[r][COLS - 1]);
+      const float rgt = lane_from_right(p[r][0]);
+      float np[COLS];
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const float pc = p[r][c];
+        float x1 = c > 0 ? p[r][c - 1] : lft;
+        float x2 = c < COLS - 1 ? p[r][c + 1] : rgt;
+        float y1 = prev[c];
+        float y2 = r < ROWS - 1 ? p[r + 1][c] : above[c];
+        if (special) {
+          const int code = cd[r][c];
+          x1 = (code & 1) ? pc : x1;
+          x2 = (code & 2) ? pc : x2;
+          y1 = (code & 4) ? pc : y1;
+          y2 = (code & 8) ? pc : y2;
+        }
+        np[c] = ((((x1 + x2) + y1) + y2) - bb[r][c]) * 0.25f;
         if (last) {
-          const int trow = trow0 + r;
-          if (lane >= k && lane < 64 - k && trow >= k && trow < EY - k &&
-              (cd[r] & 16) == 0) {
-            const double rr = (double)bb[r] -
+          const int trow = trow0 + r, tcol = tcol0 + c;
+          if (tcol >= k && tcol < EX - k && trow >= k && trow < EY - k &&
+              (cd[r][c] & 16) == 0) {
+            const double rr = (double)bb[r][c] -
                               (((((double)x1 + (double)x2) + (double)y1) +
                                 (double)y2) -
                                4.0 * (double)pc);
@@ -118,47 +195,31 @@ __global__ void __launch_bounds__(WAVES * 64)
             srr += rr * rr;
           }
         }
-        prev = pc;
-        p[r] = np;
       }
-    } else {
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        const float pc = p[r];
-        const float up = r < ROWS - 1 ? p[r + 1] : above;
-        const float l = lane_from_left(pc);
-        const float rt = lane_from_right(pc);
-        const unsigned c = cd[r];
-        const float x1 = (c & 1) ? pc : l;
-        const float x2 = (c & 2) ? pc : rt;
-        const float y1 = (c & 4) ? pc : prev;
-        const float y2 = (c & 8) ? pc : up;
-        const float np = ((((x1 + x2) + y1) + y2) - bb[r]) * 0.25f;
-        if (last) {
-          const int trow = trow0 + r;
-          if (lane >= k && lane < 64 - k && trow >= k && trow < EY - k &&
-              (c & 16) == 0) {
-            const double rr = (double)bb[r] -
-                              (((((double)x1 + (double)x2) + (double)y1) +
-                                (double)y2) -
-                               4.0 * (double)pc);
-            sr += rr;
-            srr += rr * rr;
-          }
-        }
-        prev = pc;
-        p[r] = np;
+      for (int c = 0; c < COLS; ++c) {
+        prev[c] = p[r][c];
+        p[r][c] = np[c];
       }
     }
   }
 
-  if (col_in && lane >= k && lane < 64 - k) {
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-      const int trow = trow0 + r;
-      const int gy = gy0 + r;
-      if (trow >= k && trow < EY - k && gy >= 0 && gy < H)
-        pout[gy * W + gx] = p[r];
+  for (int r = 0; r < ROWS; ++r) {
+    const int trow = trow0 + r;
+    const int gy = gy0 + r;
+    if (trow < k || trow >= EY - k || gy < 0 || gy >= H) continue;
+    if (vec) {
+      if (tcol0 >= k && tcol0 < EX - k && gx0 >= 0 && gx0 < W)
+        *reinterpret_cast<float2*>(pout + gy * W + gx0) =
+            make_float2(p[r][0], p[r][COLS - 1]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const int gx = gx0 + c, tcol = tcol0 + c;
+        if (tcol >= k && tcol < EX - k && gx >= 0 && gx < W)
+          pout[gy * W + gx] = p[r][c];
+      }
     }
   }
 
@@ -296,32 +357,48 @@ __global__ void __launch_bounds__(256)
   if (p != nullptr) p[i] = (0.0f - bc) * 0.25f;
 }
 
-template <int ROWS, int WAVES>
+template <int ROWS, int WAVES, int COLS>
 hipError_t launch_tb(const float* pin, const float* b, const uint8_t* mask,
                      float* pout, int W, int H, int k, double* red,
                      const int* done, hipStream_t s) {
-  constexpr int EY = ROWS * WAVES;
-  const int VX = 64 - 2 * k, VY = EY - 2 * k;
+  constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
+  const int VX = EX - 2 * k, VY = EY - 2 * k;
   dim3 grid((W + VX - 1) / VX, (H + VY - 1) / VY, 1);
-  k_jacobi_tb<ROWS, WAVES><<<grid, WAVES * 64, 0, s>>>(pin, b, mask, pout, W,
-                                                       H, k, red, done);
+  k_jacobi_tb<ROWS, WAVES, COLS><<<grid, WAVES * 64, 0, s>>>(
+      pin, b, mask, pout, W, H, k, red, done);
   return hipGetLastError();
 }
 
 }  // namespace
 
+// Instantiated tile shapes (rows per wave, waves per block, columns per
+// lane).  The table is the autotuner's search space.
+#define NX_TB_SHAPES(X) \
+  X(4, 8, 1)            \
+  X(8, 8, 1)            \
+  X(8, 16, 1)           \
+  X(16, 8, 1)           \
+  X(2, 8, 2)            \
+  X(2, 16, 2)           \
+  X(4, 4, 2)            \
+  X(4, 8, 2)            \
+  X(4, 16, 2)           \
+  X(8, 4, 2)            \
+  X(8, 8, 2)
+
 bool jacobi_cfg_supported(JacobiCfg c) {
-  const bool shape = (c.rows == 4 && (c.waves == 8 || c.waves == 16)) ||
-                     (c.rows == 8 && (c.waves == 4 || c.waves == 8 || c.waves == 16)) ||
-                     (c.rows == 12 && (c.waves == 4 || c.waves == 8)) ||
-                     (c.rows == 16 && (c.waves == 4 || c.waves == 8));
+  bool shape = false;
+#define NX_TB_CHECK(R, WV, CL) \
+  shape |= (c.rows == R && c.waves == WV && c.cols == CL);
+  NX_TB_SHAPES(NX_TB_CHECK)
+#undef NX_TB_CHECK
   if (!shape) return false;
-  const int EY = c.rows * c.waves;
-  return c.k >= 1 && 64 - 2 * c.k >= 8 && EY - 2 * c.k >= 8;
+  const int EX = 64 * c.cols, EY = c.rows * c.waves;
+  return c.k >= 1 && EX - 2 * c.k >= 8 && EY - 2 * c.k >= 8;
 }
 
 int jacobi_launch_blocks(int W, int H, JacobiCfg c) {
-  const int VX = 64 - 2 * c.k, VY = c.rows * c.waves - 2 * c.k;
+  const int VX = 64 * c.cols - 2 * c.k, VY = c.rows * c.waves - 2 * c.k;
   return ((W + VX - 1) / VX) * ((H + VY - 1) / VY);
 }
 
@@ -330,19 +407,11 @@ hipError_t launch_jacobi_tb(const float* pin, const float* b,
                             JacobiCfg c, double* red, const int* done,
                             hipStream_t s) {
   if (!jacobi_cfg_supported(c)) return hipErrorInvalidValue;
-#define NX_TB(R, WV)                                                        \
-  if (c.rows == R && c.waves == WV)                                         \
-    return launch_tb<R, WV>(pin, b, mask, pout, W, H, c.k, red, done, s);
-  NX_TB(4, 8)
-  NX_TB(4, 16)
-  NX_TB(8, 4)
-  NX_TB(8, 8)
-  NX_TB(8, 16)
-  NX_TB(12, 4)
-  NX_TB(12, 8)
-  NX_TB(16, 4)
-  NX_TB(16, 8)
-#undef NX_TB
+#define NX_TB_LAUNCH(R, WV, CL)                                              \
+  if (c.rows == R && c.waves == WV && c.cols == CL)                          \
+    return launch_tb<R, WV, CL>(pin, b, mask, pout, W, H, c.k, red, done, s);
+  NX_TB_SHAPES(NX_TB_LAUNCH)
+#undef NX_TB_LAUNCH
   return hipErrorInvalidValue;
 }
 

@@ -69,6 +69,7 @@ struct JacobiCfg {
   int k;      // sweeps per launch
   int rows;   // rows per wave
   int waves;  // waves per block
+  int cols;   // columns per lane
 };
 int jacobi_launch_blocks(int W, int H, JacobiCfg cfg);
 hipError_t launch_jacobi_tb(const float* pin, const float* b,

@@ -94,7 +94,7 @@ def test_c_validation_messages(field, value, msg):
 
 def test_c_validation_jacobi_tile_and_solver():
     p = _defaults()
-    p.jacobi_k = 40
+    p.jacobi_k = 70
     with pytest.raises(ValueError):
         _lib.check(_lib.lib().nx_check_params(C.byref(p)))
     p = _defaults()

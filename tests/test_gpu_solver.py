@@ -18,14 +18,16 @@ from tests.helpers import (assert_bitwise, assert_close, blob_mask, f32,
 
 pytestmark = pytest.mark.gpu
 
-TILES = [(4, 8), (4, 16), (8, 4), (8, 8), (8, 16), (12, 4), (12, 8), (16, 4), (16, 8)]
+TILES = [(4, 8, 1), (8, 8, 1), (8, 16, 1), (16, 8, 1), (2, 8, 2), (2, 16, 2),
+         (4, 4, 2), (4, 8, 2), (4, 16, 2), (8, 4, 2), (8, 8, 2)]
 
 
-@pytest.mark.parametrize("rows,waves", TILES)
-def test_jacobi_blocked_bitwise(gpu, rows, waves):
-    for (H, W), k in [((100, 77), 1), ((100, 77), 5), ((257, 130), 8),
-                      ((33, 200), 3)]:
-        if 64 - 2 * k < 8 or rows * waves - 2 * k < 8:
+@pytest.mark.parametrize("rows,waves,cols", TILES)
+def test_jacobi_blocked_bitwise(gpu, rows, waves, cols):
+    # odd and even widths and k: the float2 path needs both even
+    for (H, W), k in [((100, 77), 1), ((100, 78), 4), ((100, 77), 5),
+                      ((257, 130), 8), ((33, 200), 3), ((64, 256), 6)]:
+        if 64 * cols - 2 * k < 8 or rows * waves - 2 * k < 8:
             continue
         rng = np.random.default_rng(H + W + k)
         m = blob_mask(rng, H, W)
@@ -36,10 +38,10 @@ def test_jacobi_blocked_bitwise(gpu, rows, waves):
         sim.upload("divergence", b)
         sim.upload("pressure", p)
         sweeps = 3 * k + 2  # includes a shortened final launch
-        sim.jacobi_blocked(sweeps, k, rows, waves)
+        sim.jacobi_blocked(sweeps, k, rows, waves, cols)
         ref = F.jacobi(p, b, obs_from_mask(m), sweeps)
         assert_bitwise(sim.download("pressure"), ref,
-                       f"tb {rows}x{waves} k={k} {H}x{W}")
+                       f"tb {rows}x{waves}x{cols} k={k} {H}x{W}")
         sim.destroy()
 
 
@@ -159,7 +161,7 @@ def test_tolerance_solve_reaches_tol(gpu):
     sim.set_obstacle_map(m)
     sim.tol = 1e-4
     sim.check_every = 64
-    sim.jacobi_tile = (8, 8, 8)
+    sim.jacobi_tile = (8, 8, 8, 1)
     sim.update(1 / 60)
     st = sim.stats()
     assert st["converged"] and st["residual"] <= 1e-4
