@@ -167,6 +167,37 @@ int nx_set_timing(nx_sim* sim, int on);
  * reset; resets the accumulators */
 int nx_collect_timing(nx_sim* sim, double* ms, int* launches);
 
+/* ---- batched scenes: many independent grids of uneven sizes advanced by
+ *      one persistent launch per stage (no reference counterpart; each
+ *      scene behaves as its own nx_sim with reference-mode semantics) ---- */
+typedef struct nx_batch nx_batch;
+/* n scenes of widths[i] x heights[i] (each side >= 3); dye on the velocity
+ * grid when with_dye */
+int nx_batch_create(int n, const int* widths, const int* heights, int with_dye,
+                    nx_batch** out);
+int nx_batch_destroy(nx_batch* b);
+/* solver settings (iterations, Jacobi tile) and every scene's parameters */
+int nx_batch_set_params(nx_batch* b, const nx_params* prm);
+/* one scene's parameter block (speed, dissipation, vorticity, viscosity,
+ * walls, inflow, dye speed/dissipation) */
+int nx_batch_set_scene_params(nx_batch* b, int scene, const nx_params* prm);
+int nx_batch_set_obstacle_map(nx_batch* b, int scene, const uint8_t* map);
+/* Gaussian splats applied at the start of every nx_batch_update: n records
+ * of 7 floats (scene, px, py, ax, ay, radius, dye) */
+int nx_batch_set_step_splats(nx_batch* b, const float* rec, int n);
+int nx_batch_update(nx_batch* b, float dt);
+int nx_batch_set_graph(nx_batch* b, int on);
+int nx_batch_jacobi_blocked(nx_batch* b, int sweeps, int k, int rows, int waves,
+                            int cols);
+int nx_batch_upload(nx_batch* b, int scene, int field, const void* src,
+                    size_t bytes);
+int nx_batch_download(nx_batch* b, int scene, int field, void* dst,
+                      size_t bytes);
+int nx_batch_cells(nx_batch* b, long long* cells);
+int nx_batch_synchronize(nx_batch* b);
+int nx_batch_set_timing(nx_batch* b, int on);
+int nx_batch_collect_timing(nx_batch* b, double* ms, int* launches);
+
 /* attainable bandwidth: a plain device copy of `bytes` (multiple of 16),
  * `iters` times; *gbps = (read + write bytes) / time */
 int nx_copy_bandwidth(size_t bytes, int iters, double* gbps);

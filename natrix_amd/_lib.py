@@ -109,6 +109,21 @@ SIGNATURES = {
     "nx_get_stream": [_P, C.POINTER(_P)],
     "nx_set_timing": [_P, _I],
     "nx_collect_timing": [_P, C.POINTER(_D), C.POINTER(_I)],
+    "nx_batch_create": [_I, C.POINTER(_I), C.POINTER(_I), _I, C.POINTER(_P)],
+    "nx_batch_destroy": [_P],
+    "nx_batch_set_params": [_P, C.POINTER(Params)],
+    "nx_batch_set_scene_params": [_P, _I, C.POINTER(Params)],
+    "nx_batch_set_obstacle_map": [_P, _I, C.POINTER(C.c_uint8)],
+    "nx_batch_set_step_splats": [_P, C.POINTER(_F), _I],
+    "nx_batch_update": [_P, _F],
+    "nx_batch_set_graph": [_P, _I],
+    "nx_batch_jacobi_blocked": [_P, _I, _I, _I, _I, _I],
+    "nx_batch_upload": [_P, _I, _I, _P, _SZ],
+    "nx_batch_download": [_P, _I, _I, _P, _SZ],
+    "nx_batch_cells": [_P, C.POINTER(C.c_longlong)],
+    "nx_batch_synchronize": [_P],
+    "nx_batch_set_timing": [_P, _I],
+    "nx_batch_collect_timing": [_P, C.POINTER(_D), C.POINTER(_I)],
     "nx_copy_bandwidth": [_SZ, _I, C.POINTER(_D)],
     "nx_last_error": [],
 }

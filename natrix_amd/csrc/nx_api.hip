@@ -25,6 +25,8 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
+int nx::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
 #define NX_HIP(call)                                                     \
   do {                                                                   \
     hipError_t e_ = (call);                                              \

@@ -129,4 +129,8 @@ __device__ __forceinline__ float2 load_bc(const float2* __restrict__ u, int x,
   return make_float2(u[(H - 2) * W + x].x, 0.0f);
 }
 
+struct Splat {  // Gaussian splat: u += (ax, ay) * w, dye += dye * w
+  float px, py, ax, ay, radius, dye;
+};
+
 }  // namespace nx

@@ -1,0 +1,230 @@
+// Temporal-blocked Jacobi tile (shared by the single-grid and batched
+// kernels); see nx_jacobi.hip for the scheme.
+#pragma once
+#include "nx_common.h"
+
+namespace nx {
+namespace jt {
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ float lane_from_left(float v) {
+  // lane i receives lane i-1 (wave_shr:1); lane 0 keeps its own value
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138,
+                                  0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float lane_from_right(float v) {
+  // lane i receives lane i+1 (wave_shl:1); lane 63 keeps its own value
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130,
+                                  0xf, 0xf, false));
+}
+
+__device__ __forceinline__ int ilane_from_left(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ int ilane_from_right(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);
+}
+
+// ROWS x COLS cells per lane: lane `l` of wave `w` owns tile columns
+// [COLS*l, COLS*l + COLS) and tile rows [ROWS*w, ROWS*w + ROWS).
+// k sweeps on tile (bx, by) of the (VX x VY)-interior tiling; `rid` is the
+// partial-sum record this tile writes when red != nullptr.  Uses
+// __syncthreads(): every thread of the block must call it.
+template <int ROWS, int WAVES, int COLS>
+__device__ __forceinline__ void jacobi_tb_tile(
+    int bx, int by, const float* __restrict__ pin, const float* __restrict__ b,
+    const uint8_t* __restrict__ mask, float* __restrict__ pout, int W, int H,
+    int k, double* __restrict__ red, int rid) {
+  constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int VX = EX - 2 * k, VY = EY - 2 * k;
+  const int gx0 = bx * VX - k + COLS * lane;
+  const int gy0 = by * VY - k + wave * ROWS;
+  // float2 loads/stores when every lane's column pair is aligned and either
+  // wholly inside or wholly outside the grid and the valid band
+  const bool vec = COLS == 2 && (W % 2 == 0) && (k % 2 == 0);
+
+  float p[ROWS][COLS], bb[ROWS][COLS];
+  int sol[ROWS][COLS];  // cell is solid or outside the grid
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    const int gy = gy0 + r;
+    const bool row_in = gy >= 0 && gy < H;
+    if (vec) {
+      const bool in = row_in && gx0 >= 0 && gx0 < W;
+      float2 pv = make_float2(0.0f, 0.0f), bv = pv;
+      int m0 = 1, m1 = 1;
+      if (in) {
+        const int idx = gy * W + gx0;
+        pv = *reinterpret_cast<const float2*>(pin + idx);
+        bv = *reinterpret_cast<const float2*>(b + idx);
+        const uint16_t mm = *reinterpret_cast<const uint16_t*>(mask + idx);
+        m0 = (mm & 0xFF) != 0;
+        m1 = (mm >> 8) != 0;
+      }
+      p[r][0] = pv.x;
+      p[r][COLS - 1] = pv.y;
+      bb[r][0] = bv.x;
+      bb[r][COLS - 1] = bv.y;
+      sol[r][0] = m0;
+      sol[r][COLS - 1] = m1;
+    } else {
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const int gx = gx0 + c;
+        p[r][c] = 0.0f;
+        bb[r][c] = 0.0f;
+        sol[r][c] = 1;
+        if (row_in && gx >= 0 && gx < W) {
+          const int idx = gy * W + gx;
+          p[r][c] = pin[idx];
+          bb[r][c] = b[idx];
+          sol[r][c] = mask[idx] != 0;
+        }
+      }
+    }
+  }
+
+  __shared__ float xch[2][2][WAVES][EX];
+  __shared__ int sxch[2][WAVES][EX];
+  // neighbour codes: bit0 L, bit1 R, bit2 B, bit3 T replaced by the centre
+  // (solid or outside the grid: common.sh clamp == Poisson.comp obstacle
+  // rule); bit4 the cell itself is not fluid.
+#pragma unroll
+  for (int c = 0; c < COLS; ++c) {
+    sxch[0][wave][COLS * lane + c] = sol[0][c];
+    sxch[1][wave][COLS * lane + c] = sol[ROWS - 1][c];
+  }
+  __syncthreads();
+  int cd[ROWS][COLS];
+  bool any_nbr = false;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    const int sl = ilane_from_left(sol[r][COLS - 1]);
+    const int sr = ilane_from_right(sol[r][0]);
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      const int L = c > 0 ? sol[r][c - 1] : sl;
+      const int R = c < COLS - 1 ? sol[r][c + 1] : sr;
+      const int B = r > 0 ? sol[r - 1][c]
+                          : (wave > 0 ? sxch[1][wave - 1][COLS * lane + c] : 1);
+      const int T = r < ROWS - 1 ? sol[r + 1][c]
+                                 : (wave < WAVES - 1 ? sxch[0][wave + 1][COLS * lane + c] : 1);
+      const int code = L | (R << 1) | (B << 2) | (T << 3) | (sol[r][c] << 4);
+      cd[r][c] = code;
+      any_nbr |= (code & 0xF) != 0;
+    }
+  }
+  const bool special = __any(any_nbr);
+
+  double sr = 0.0, srr = 0.0;
+  const int trow0 = wave * ROWS;  // tile row of p[0][*]
+  const int tcol0 = COLS * lane;  // tile column of p[*][0]
+
+  for (int s = 0; s < k; ++s) {
+    const int buf = s & 1;
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      xch[buf][0][wave][tcol0 + c] = p[0][c];
+      xch[buf][1][wave][tcol0 + c] = p[ROWS - 1][c];
+    }
+    __syncthreads();
+    float prev[COLS], above[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      prev[c] = wave > 0 ? xch[buf][1][wave - 1][tcol0 + c] : p[0][c];
+      above[c] = wave < WAVES - 1 ? xch[buf][0][wave + 1][tcol0 + c] : p[ROWS - 1][c];
+    }
+    const bool last = (s == k - 1) && red != nullptr;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const float lft = lane_from_left(p[r][COLS - 1]);
+      const float rgt = lane_from_right(p[r][0]);
+      float np[COLS];
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const float pc = p[r][c];
+        float x1 = c > 0 ? p[r][c - 1] : lft;
+        float x2 = c < COLS - 1 ? p[r][c + 1] : rgt;
+        float y1 = prev[c];
+        float y2 = r < ROWS - 1 ? p[r + 1][c] : above[c];
+        if (special) {
+          const int code = cd[r][c];
+          x1 = (code & 1) ? pc : x1;
+          x2 = (code & 2) ? pc : x2;
+          y1 = (code & 4) ? pc : y1;
+          y2 = (code & 8) ? pc : y2;
+        }
+        np[c] = ((((x1 + x2) + y1) + y2) - bb[r][c]) * 0.25f;
+        if (last) {
+          const int trow = trow0 + r, tcol = tcol0 + c;
+          if (tcol >= k && tcol < EX - k && trow >= k && trow < EY - k &&
+              (cd[r][c] & 16) == 0) {
+            const double rr = (double)bb[r][c] -
+                              (((((double)x1 + (double)x2) + (double)y1) +
+                                (double)y2) -
+                               4.0 * (double)pc);
+            sr += rr;
+            srr += rr * rr;
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        prev[c] = p[r][c];
+        p[r][c] = np[c];
+      }
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    const int trow = trow0 + r;
+    const int gy = gy0 + r;
+    if (trow < k || trow >= EY - k || gy < 0 || gy >= H) continue;
+    if (vec) {
+      if (tcol0 >= k && tcol0 < EX - k && gx0 >= 0 && gx0 < W)
+        *reinterpret_cast<float2*>(pout + gy * W + gx0) =
+            make_float2(p[r][0], p[r][COLS - 1]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const int gx = gx0 + c, tcol = tcol0 + c;
+        if (tcol >= k && tcol < EX - k && gx >= 0 && gx < W)
+          pout[gy * W + gx] = p[r][c];
+      }
+    }
+  }
+
+  if (red != nullptr) {
+    __shared__ double part[WAVES][2];
+    sr = wave_sum(sr);
+    srr = wave_sum(srr);
+    if (lane == 0) {
+      part[wave][0] = sr;
+      part[wave][1] = srr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = 0.0, c2 = 0.0;
+      for (int w = 0; w < WAVES; ++w) {
+        a += part[w][0];
+        c2 += part[w][1];
+      }
+      red[2 * rid + 0] = a;
+      red[2 * rid + 1] = c2;
+    }
+  }
+}
+
+}  // namespace jt
+}  // namespace nx

@@ -4,13 +4,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "nx_common.h"
+#include <string>
+
+#include "nx_cells.h"
 
 namespace nx {
 
-struct Splat {  // Gaussian splat: u += (ax, ay) * w, dye += dye * w
-  float px, py, ax, ay, radius, dye;
-};
+// record `msg` as the calling thread's nx_last_error() and return `code`
+int set_error(int code, const std::string& msg);
 
 // ---- per-stage kernels (one reference shader each) -------------------
 hipError_t launch_bc(float2* u, int W, int H, BcParams bc, hipStream_t s);

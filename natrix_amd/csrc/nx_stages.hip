@@ -37,29 +37,13 @@ __global__ void __launch_bounds__(256) k_bc(float2* u, int W, int H,
 }
 
 // ---------------------------------------------------------------- S2
-// shader.AdvectVelocity.comp:27-49, velocity read through the wall
-// treatment of S1.
 __global__ void __launch_bounds__(256)
     k_advect(const float2* __restrict__ uin, float2* __restrict__ uout,
              const uint8_t* __restrict__ mask, int W, int H, float dt,
              float speed, float diss, BcParams bc) {
   int x, y;
   if (!cell(x, y, W, H)) return;
-  const int pos = y * W + x;
-  if (mask[pos] != 0) {
-    uout[pos] = make_float2(0.0f, 0.0f);
-    return;
-  }
-  const float2 vel = load_bc(uin, x, y, W, H, bc);
-  const float fx = (float)x - (vel.x * dt) * speed;
-  const float fy = (float)y - (vel.y * dt) * speed;
-  const Bilin c = bilinear_corners(fx, fy, W, H);
-  const float2 v = bilinear_mix<float2>(
-      load_bc(uin, c.bx, c.ty, W, H, bc), load_bc(uin, c.tx, c.ty, W, H, bc),
-      load_bc(uin, c.bx, c.by, W, H, bc), load_bc(uin, c.tx, c.by, W, H, bc),
-      c.dx, c.dy);
-  uout[pos] = make_float2(clampf(v.x * diss, -1.0f, 1.0f),
-                          clampf(v.y * diss, -1.0f, 1.0f));
+  uout[y * W + x] = advect_cell(uin, mask, x, y, W, H, dt, speed, diss, bc);
 }
 
 // ---------------------------------------------------------------- S3
@@ -167,17 +151,7 @@ __global__ void __launch_bounds__(256)
   int x, y;
   if (!cell(x, y, W, H)) return;
   const int pos = y * W + x;
-  const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
-  const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
-  const float pc = p[pos];
-  const float x1 = mask[iL] ? pc : p[iL];
-  const float x2 = mask[iR] ? pc : p[iR];
-  const float y1 = mask[iB] ? pc : p[iB];
-  const float y2 = mask[iT] ? pc : p[iT];
-  float2 v = uin[pos];
-  v.x -= 0.5f * (x2 - x1);
-  v.y -= 0.5f * (y2 - y1);
-  uout[pos] = v;
+  uout[pos] = subgrad_cell(uin, p, mask, x, y, W, H);
   if (mask_next) mask_next[pos] = mask[pos] & MASK_PERSIST;
 }
 
@@ -260,27 +234,8 @@ __global__ void __launch_bounds__(256)
                  int Hv, float dt, float speed, float diss) {
   int x, y;
   if (!cell(x, y, Wp, Hp)) return;
-  const int pos = y * Wp + x;
-  const float nx = ((float)x / (float)Wp) * (float)Wv;
-  const float ny = ((float)y / (float)Hp) * (float)Hv;
-  // uint() truncation (n >= 0); x/Wp < 1 keeps it below Wv, the min() only
-  // guards the memory access
-  const int opos = min((int)ny, Hv - 1) * Wv + min((int)nx, Wv - 1);
-  if (mask[opos] != 0) {
-    qout[pos] = 0.0f;
-    return;
-  }
-  const Bilin cv = bilinear_corners(nx, ny, Wv, Hv);
-  const float2 vel = bilinear_mix<float2>(u[cv.lt], u[cv.rt], u[cv.lb],
-                                          u[cv.rb], cv.dx, cv.dy);
-  const float sx = (float)Wp / (float)Wv, sy = (float)Hp / (float)Hv;
-  const float vx = vel.x * sx, vy = vel.y * sy;
-  const float fx = (float)x - (vx * dt) * speed;
-  const float fy = (float)y - (vy * dt) * speed;
-  const Bilin c = bilinear_corners(fx, fy, Wp, Hp);
-  qout[pos] = bilinear_mix<float>(qin[c.lt], qin[c.rt], qin[c.lb], qin[c.rb],
-                                  c.dx, c.dy) *
-              diss;
+  qout[y * Wp + x] =
+      advect_dye_cell(qin, u, mask, x, y, Wp, Hp, Wv, Hv, dt, speed, diss);
 }
 
 // ---------------------------------------------------------------- F4
@@ -318,10 +273,7 @@ __global__ void __launch_bounds__(256)
   float d = q ? q[pos] : 0.0f;
   for (int i = 0; i < sb.n; ++i) {
     const Splat s = sb.s[i];
-    const float sx = s.px * (float)W, sy = s.py * (float)H;
-    const float dx = sx - (float)x, dy = sy - (float)y;
-    const float e = -(dx * dx + dy * dy) / (s.radius * s.radius);
-    const float w = (float)exp((double)e);
+    const float w = gaussian_weight(s, x, y, W, H);
     v.x = v.x + s.ax * w;
     v.y = v.y + s.ay * w;
     d = d + s.dye * w;

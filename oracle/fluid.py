@@ -478,6 +478,7 @@ class Params:
         self.has_borders = True
         self.bc_mode = BC_REFERENCE
         self.inflow = 1.0
+        self.dye_speed = 500.0  # SmoothParticlesArea._speed, smooth_particles_area.py:23
         self.dye_dissipation = 1.0
         for k, v in kw.items():
             if not hasattr(self, k):
@@ -522,7 +523,7 @@ def step(u, p, obs, dt, prm, dye=None, persistent=None, return_stages=False):
     if persistent is not None:
         o_after[persistent, 0] = 1
     if dye is not None:
-        dye = advect_dye(dye, u, o_after, dt, prm.speed, prm.dye_dissipation)
+        dye = advect_dye(dye, u, o_after, dt, prm.dye_speed, prm.dye_dissipation)
         stages["dye"] = dye
     out = (u, p, np.zeros_like(obs), dye)
     if return_stages:
