@@ -35,6 +35,14 @@ extern "C" {
 #define NX_FIELD_MASK 4       /* uint8: obstacle bits, bit0 = float2 channel x,
                                  bit1 = channel y, bit2 = persistent map */
 #define NX_FIELD_DYE 5        /* float on the dye grid */
+#define NX_FIELD_PRESSURE32 6 /* float32 pressure the projection used
+                                 (reduced storage: p + correction) */
+
+/* Storage formats of velocity, pressure, divergence and dye (arithmetic is
+ * float32 throughout; stores round to nearest even). */
+#define NX_STORAGE_F32 0
+#define NX_STORAGE_BF16 1
+#define NX_STORAGE_F16 2
 
 #define NX_BC_REFERENCE 0 /* zero ring when has_borders (InitBoundaries.comp) */
 #define NX_BC_FREE_SLIP 1 /* normal velocity 0, tangential copied inward */
@@ -67,6 +75,8 @@ typedef struct nx_params {
                             is <= tol (then `iterations` is ignored) */
   int max_iters;         /* sweep cap of the tolerance-driven solve */
   int check_every;       /* sweeps between residual checks */
+  int correction_sweeps; /* reduced storage: float32 residual-correction
+                            sweeps after the solve (0..16, default 8) */
 } nx_params;
 
 /* Statistics of the last pressure solve. */
@@ -80,6 +90,9 @@ typedef struct nx_stats {
 /* ---- lifetime (FluidSimulator.__init__ / destroy, fluid_simulator.py:37-48,
  *      476-515) ---- */
 int nx_create(int width, int height, nx_sim** out);
+/* as nx_create with a storage format (NX_STORAGE_*); reduced storage runs
+ * the fused step only (no per-stage entry points, no tolerance solve) */
+int nx_create_ex(int width, int height, int storage, nx_sim** out);
 int nx_destroy(nx_sim* sim);
 int nx_default_params(nx_params* out);
 int nx_get_params(const nx_sim* sim, nx_params* out);

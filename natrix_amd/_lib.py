@@ -23,6 +23,11 @@ FIELD_DIVERGENCE = 2
 FIELD_CURL = 3
 FIELD_MASK = 4
 FIELD_DYE = 5
+FIELD_PRESSURE32 = 6
+
+STORAGE_F32 = 0
+STORAGE_BF16 = 1
+STORAGE_F16 = 2
 
 BC_REFERENCE = 0
 BC_FREE_SLIP = 1
@@ -52,6 +57,7 @@ class Params(C.Structure):
         ("tol", C.c_double),
         ("max_iters", C.c_int),
         ("check_every", C.c_int),
+        ("correction_sweeps", C.c_int),
     ]
 
 
@@ -73,6 +79,7 @@ _SZ = C.c_size_t
 # name -> argtypes (every function returns int unless listed in _RESTYPE)
 SIGNATURES = {
     "nx_create": [_I, _I, C.POINTER(_P)],
+    "nx_create_ex": [_I, _I, _I, C.POINTER(_P)],
     "nx_destroy": [_P],
     "nx_default_params": [C.POINTER(Params)],
     "nx_get_params": [_P, C.POINTER(Params)],

@@ -21,7 +21,7 @@ import numpy as np
 from .. import _lib
 from .._lib import (BC_FREE_SLIP, BC_INFLOW, BC_REFERENCE, FIELD_CURL,
                     FIELD_DIVERGENCE, FIELD_DYE, FIELD_MASK, FIELD_PRESSURE,
-                    FIELD_VELOCITY, Params, Stats)
+                    FIELD_PRESSURE32, FIELD_VELOCITY, Params, Stats)
 
 _FIELD_NAMES = {
     "velocity": FIELD_VELOCITY,
@@ -32,7 +32,20 @@ _FIELD_NAMES = {
     "mask": FIELD_MASK,
     "obstacles": FIELD_MASK,
     "dye": FIELD_DYE,
+    "pressure32": FIELD_PRESSURE32,
 }
+
+_STORAGE = {"f32": 0, "bf16": 1, "f16": 2}
+
+
+def bf16_bits_to_f32(h):
+    return (np.asarray(h, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def f32_to_bf16_bits(x):
+    """Round to nearest even (finite values)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
 
 _BC_NAMES = {"reference": BC_REFERENCE, "free_slip": BC_FREE_SLIP,
              "inflow": BC_INFLOW}
@@ -84,16 +97,22 @@ class FluidSimulator:
 
     def __init__(self, width: int, height: int, vertex_layout=None, *,
                  dye: bool | tuple = False, bc_mode="reference",
-                 inflow: float = 1.0, fused: bool = True):
+                 inflow: float = 1.0, fused: bool = True, storage: str = "f32"):
         """``vertex_layout`` is accepted for signature compatibility with the
         reference (a bgfx vertex layout) and ignored.  ``dye=True`` adds a
-        dye field on the velocity grid, ``dye=(w, h)`` on its own grid."""
+        dye field on the velocity grid, ``dye=(w, h)`` on its own grid.
+        ``storage`` is "f32", "bf16" or "f16" for velocity, pressure,
+        divergence and dye (arithmetic is float32 either way)."""
         self._width = int(width)
         self._height = int(height)
         self.vertex_layout = vertex_layout
         self._h = None
+        if storage not in _STORAGE:
+            raise ValueError(f"unknown storage {storage!r}")
+        self.storage = storage
         h = C.c_void_p()
-        _lib.call("nx_create", self._width, self._height, C.byref(h))
+        _lib.call("nx_create_ex", self._width, self._height, _STORAGE[storage],
+                  C.byref(h))
         self._h = h
         self._prm = Params()
         _lib.call("nx_default_params", C.byref(self._prm))
@@ -103,6 +122,7 @@ class FluidSimulator:
         self.tol = 0.0
         self.max_iters = 1_000_000
         self.check_every = 256
+        self.correction_sweeps = 8
         from ..autotune import lookup
         self.jacobi_tile = lookup(self._width, self._height)
         self._dye_speed = 500.0
@@ -221,6 +241,7 @@ class FluidSimulator:
         p.tol = float(self.tol)
         p.max_iters = int(self.max_iters)
         p.check_every = int(self.check_every)
+        p.correction_sweeps = int(self.correction_sweeps)
         _lib.call("nx_set_params", self._h, C.byref(p))
 
     def _alive(self):
@@ -302,13 +323,29 @@ class FluidSimulator:
                   m.ctypes.data_as(C.POINTER(C.c_uint8)))
 
     def _field_shape(self, fid):
+        """(shape, stored dtype) of a field."""
+        st = {"f32": np.float32, "bf16": np.uint16, "f16": np.float16}[self.storage]
         if fid == FIELD_VELOCITY:
-            return (self._height, self._width, 2), np.float32
+            return (self._height, self._width, 2), st
         if fid == FIELD_MASK:
             return (self._height, self._width), np.uint8
         if fid == FIELD_DYE:
-            return self.dye_shape, np.float32
-        return (self._height, self._width), np.float32
+            return self.dye_shape, st
+        if fid in (FIELD_CURL, FIELD_PRESSURE32):
+            return (self._height, self._width), np.float32
+        return (self._height, self._width), st
+
+    def _widen(self, a):
+        if a.dtype == np.uint16:
+            return bf16_bits_to_f32(a)
+        if a.dtype == np.float16:
+            return a.astype(np.float32)
+        return a
+
+    def _narrow(self, a, dt):
+        if dt == np.uint16:
+            return f32_to_bf16_bits(a)
+        return np.asarray(a, dtype=dt)
 
     def field_buffer_raw(self, fid):
         ptr = C.c_void_p()
@@ -337,13 +374,14 @@ class FluidSimulator:
         assert out.nbytes == nbytes
         _lib.call("nx_download", self._alive(), fid, out.ctypes.data_as(C.c_void_p),
                   out.nbytes)
-        return out
+        return out if fid == FIELD_MASK else self._widen(out)
 
     def upload(self, name, array):
         fid = _FIELD_NAMES[name] if isinstance(name, str) else int(name)
         ptr, nbytes, _ = self.field_buffer_raw(fid)
         shape, dt = self._field_shape(fid)
-        a = np.ascontiguousarray(np.asarray(array, dtype=dt))
+        a = np.ascontiguousarray(self._narrow(array, dt) if fid != FIELD_MASK
+                                 else np.asarray(array, dtype=dt))
         if a.shape != tuple(shape):
             raise ValueError(f"{name}: expected shape {shape}, got {a.shape}")
         _lib.call("nx_upload", self._alive(), fid, a.ctypes.data_as(C.c_void_p),

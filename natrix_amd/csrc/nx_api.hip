@@ -50,6 +50,9 @@ struct TimedPair {
 struct nx_sim {
   int W = 0, H = 0;
   int device = 0;
+  int storage = ST_F32;  // format of velocity, pressure, divergence, dye
+  size_t es = 4;         // bytes per stored scalar
+  float* p32 = nullptr;  // float32 corrected pressure (reduced storage)
   hipStream_t stream = nullptr;
   float2* vel[2] = {nullptr, nullptr};
   float* p[2] = {nullptr, nullptr};
@@ -157,6 +160,16 @@ static int validate(const nx_sim* s, const nx_params& p) {
   if (p.tol > 0 && !p.fused)
     return fail(NX_ERR_ARG, "the tolerance-driven solve needs the fused path");
   if (p.max_iters < 1) return fail(NX_ERR_ARG, "max_iters must be >= 1");
+  if (p.correction_sweeps < 0 || p.correction_sweeps > 16)
+    return fail(NX_ERR_ARG, "correction_sweeps must be in [0, 16]");
+  if (s && s->storage != ST_F32) {
+    if (!p.fused) return fail(NX_ERR_ARG, "reduced-precision storage runs the fused path only");
+    if (p.tol > 0)
+      return fail(NX_ERR_ARG, "the tolerance-driven solve needs float32 storage");
+    JacobiCfg cr{1 + p.correction_sweeps, p.jacobi_rows, p.jacobi_waves, p.jacobi_cols};
+    if (!jacobi_cfg_supported(cr))
+      return fail(NX_ERR_ARG, "correction_sweeps too large for the Jacobi tile");
+  }
   if (p.check_every < 1) return fail(NX_ERR_ARG, "check_every must be >= 1");
   return NX_OK;
 }
@@ -369,7 +382,68 @@ static int step_reference_order(nx_sim* s, float dt) {
 
 // Fused step: advect (walls folded into its reads), fused curl..p1,
 // temporal-blocked Jacobi, projection + obstacle clear, dye advection.
+// Reduced-precision step: the fused sequence with every field stored in
+// bfloat16/float16, then the float32 residual-correction launch.
+static int step_mixed_enqueue(nx_sim* s, float dt) {
+  const nx_params& P = s->prm;
+  const int W = s->W, H = s->H, st = s->storage;
+  const int a = s->vcur;
+  NX_LAUNCH_TIMED(s, NX_TIME_ADVECT,
+                  launch_m_advect(st, s->vel[a], s->vel[1 - a], s->mask[s->mcur],
+                                  W, H, dt, P.speed, P.dissipation, bc_of(P),
+                                  s->stream));
+  float alpha = 0.0f, rbeta = 0.0f;
+  const int viscous = P.viscosity > 0.0f;
+  if (viscous) {
+    const double av = 1.0 / (double)P.viscosity;
+    alpha = (float)av;
+    rbeta = (float)(1.0 / (4.0 + av));
+  }
+  NX_LAUNCH_TIMED(s, NX_TIME_VORT_DIV,
+                  launch_m_fused(st, s->vel[1 - a], s->mask[s->mcur], s->vel[a],
+                                 s->div, s->p[0], W, H, dt, P.vorticity, viscous,
+                                 alpha, rbeta, s->stream));
+  s->pcur = 0;
+  JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols};
+  int sweeps = P.iterations - 1;
+  while (sweeps > 0) {
+    JacobiCfg cc = c;
+    if (sweeps < cc.k) cc.k = sweeps;
+    NX_LAUNCH_TIMED(s, NX_TIME_JACOBI,
+                    launch_m_jacobi(st, s->p[s->pcur], s->div, s->mask[s->mcur],
+                                    s->p[1 - s->pcur], nullptr, W, H, cc, false,
+                                    s->stream));
+    s->pcur ^= 1;
+    sweeps -= cc.k;
+  }
+  // float32 residual correction: one residual pass + correction_sweeps
+  // sweeps of A e = r, written as p + e (with 0 sweeps: the widened p)
+  JacobiCfg cr = c;
+  cr.k = 1 + P.correction_sweeps;
+  NX_LAUNCH_TIMED(s, NX_TIME_JACOBI,
+                  launch_m_jacobi(st, s->p[s->pcur], s->div, s->mask[s->mcur],
+                                  nullptr, s->p32, W, H, cr, true, s->stream));
+  s->stats.sweeps = P.iterations;
+  s->stats.converged = 0;
+  NX_LAUNCH_TIMED(s, NX_TIME_PROJECT,
+                  launch_m_subgrad(st, s->vel[a], s->p32, s->mask[s->mcur],
+                                   s->vel[1 - a], s->mask[1 - s->mcur], W, H,
+                                   s->stream));
+  s->vcur = 1 - a;
+  s->mcur ^= 1;
+  if (s->dye[0] && P.dye_in_step) {
+    NX_LAUNCH_TIMED(s, NX_TIME_DYE,
+                    launch_m_advect_dye(st, s->dye[s->dcur], s->dye[1 - s->dcur],
+                                        s->vel[s->vcur], s->mask[s->mcur],
+                                        s->dyeW, s->dyeH, W, H, dt, P.dye_speed,
+                                        P.dye_dissipation, s->stream));
+    s->dcur ^= 1;
+  }
+  return NX_OK;
+}
+
 static int step_fused_enqueue(nx_sim* s, float dt, bool tolerance) {
+  if (s->storage != ST_F32) return step_mixed_enqueue(s, dt);
   const nx_params& P = s->prm;
   const int W = s->W, H = s->H;
   const int a = s->vcur;
@@ -444,18 +518,28 @@ int nx_default_params(nx_params* out) {
   p.tol = 0.0;
   p.max_iters = 1000000;
   p.check_every = 256;
+  p.correction_sweeps = 8;
   *out = p;
   return NX_OK;
 }
 
 int nx_create(int width, int height, nx_sim** out) {
+  return nx_create_ex(width, height, NX_STORAGE_F32, out);
+}
+
+int nx_create_ex(int width, int height, int storage, nx_sim** out) {
   if (!out) return fail(NX_ERR_ARG, "null output pointer");
   *out = nullptr;
   if (width < 1 || height < 1 || (long long)width * height > (1ll << 28))
     return fail(NX_ERR_ARG, "grid size out of range");
+  if (storage != NX_STORAGE_F32 && storage != NX_STORAGE_BF16 &&
+      storage != NX_STORAGE_F16)
+    return fail(NX_ERR_ARG, "unknown storage format");
   nx_sim* s = new nx_sim();
   s->W = width;
   s->H = height;
+  s->storage = storage;
+  s->es = storage == NX_STORAGE_F32 ? 4 : 2;
   nx_default_params(&s->prm);
   memset(&s->stats, 0, sizeof(s->stats));
   const size_t n = (size_t)width * height;
@@ -468,13 +552,15 @@ int nx_create(int width, int height, nx_sim** out) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(NX_ERR_HIP, "hipStreamCreate failed"));
   bool ok = true;
+  const size_t es = s->es;
   for (int i = 0; i < 2; ++i) {
-    ok &= hipMalloc(&s->vel[i], n * sizeof(float2)) == hipSuccess;
-    ok &= hipMalloc(&s->p[i], n * sizeof(float)) == hipSuccess;
+    ok &= hipMalloc(&s->vel[i], n * 2 * es) == hipSuccess;
+    ok &= hipMalloc(&s->p[i], n * es) == hipSuccess;
     ok &= hipMalloc(&s->mask[i], n) == hipSuccess;
   }
-  ok &= hipMalloc(&s->div, n * sizeof(float)) == hipSuccess;
+  ok &= hipMalloc(&s->div, n * es) == hipSuccess;
   ok &= hipMalloc(&s->curl, n * sizeof(float)) == hipSuccess;
+  if (storage != NX_STORAGE_F32) ok &= hipMalloc(&s->p32, n * sizeof(float)) == hipSuccess;
   ok &= hipMalloc(&s->red5, (size_t)residual_blocks(width, height) * 5 *
                                 sizeof(double)) == hipSuccess;
   ok &= hipMalloc(&s->sums, 8 * sizeof(double)) == hipSuccess;
@@ -485,11 +571,12 @@ int nx_create(int width, int height, nx_sim** out) {
   if (!ok) return cleanup(fail(NX_ERR_HIP, "device allocation failed"));
   // zero state: fields start at rest, as freshly created bgfx buffers
   for (int i = 0; i < 2; ++i) {
-    ok &= hipMemsetAsync(s->vel[i], 0, n * sizeof(float2), s->stream) == hipSuccess;
-    ok &= hipMemsetAsync(s->p[i], 0, n * sizeof(float), s->stream) == hipSuccess;
+    ok &= hipMemsetAsync(s->vel[i], 0, n * 2 * es, s->stream) == hipSuccess;
+    ok &= hipMemsetAsync(s->p[i], 0, n * es, s->stream) == hipSuccess;
     ok &= hipMemsetAsync(s->mask[i], 0, n, s->stream) == hipSuccess;
   }
-  ok &= hipMemsetAsync(s->div, 0, n * sizeof(float), s->stream) == hipSuccess;
+  ok &= hipMemsetAsync(s->div, 0, n * es, s->stream) == hipSuccess;
+  if (s->p32) ok &= hipMemsetAsync(s->p32, 0, n * sizeof(float), s->stream) == hipSuccess;
   ok &= hipMemsetAsync(s->curl, 0, n * sizeof(float), s->stream) == hipSuccess;
   ok &= hipMemsetAsync(s->sums, 0, 8 * sizeof(double), s->stream) == hipSuccess;
   ok &= hipMemsetAsync(s->state, 0, 4 * sizeof(int), s->stream) == hipSuccess;
@@ -516,6 +603,7 @@ int nx_destroy(nx_sim* s) {
     if (s->poll_ev[i]) (void)hipEventDestroy(s->poll_ev[i]);
   }
   (void)hipFree(s->div);
+  (void)hipFree(s->p32);
   (void)hipFree(s->curl);
   (void)hipFree(s->red5);
   (void)hipFree(s->red2);
@@ -558,8 +646,8 @@ int nx_enable_dye(nx_sim* s, int width, int height) {
     s->dye[i] = nullptr;
   }
   for (int i = 0; i < 2; ++i) {
-    NX_HIP(hipMalloc(&s->dye[i], n * sizeof(float)));
-    NX_HIP(hipMemsetAsync(s->dye[i], 0, n * sizeof(float), s->stream));
+    NX_HIP(hipMalloc(&s->dye[i], n * s->es));
+    NX_HIP(hipMemsetAsync(s->dye[i], 0, n * s->es, s->stream));
   }
   NX_HIP(hipStreamSynchronize(s->stream));
   s->dyeW = width;
@@ -571,6 +659,8 @@ int nx_enable_dye(nx_sim* s, int width, int height) {
 int nx_add_velocity(nx_sim* s, float px, float py, float vx, float vy,
                     float radius) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_LAUNCH_TIMED(s, NX_TIME_SPLAT,
                   launch_add_velocity(s->vel[s->vcur], s->vel[1 - s->vcur],
                                       s->W, s->H, px, py, vx, vy, radius,
@@ -598,6 +688,8 @@ int nx_add_triangle_obstacle(nx_sim* s, float x1, float y1, float x2,
 
 int nx_add_dye(nx_sim* s, float px, float py, float amount, float radius) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   if (!s->dye[0]) return fail(NX_ERR_STATE, "dye is not enabled");
   NX_LAUNCH_TIMED(s, NX_TIME_SPLAT,
                   launch_add_dye(s->dye[s->dcur], s->dye[1 - s->dcur], s->dyeW,
@@ -622,6 +714,13 @@ int nx_add_gaussian_splats(nx_sim* s, const float* splats, int n) {
     const float* f = splats + 6 * i;
     if (!(f[4] > 0)) return fail(NX_ERR_ARG, "splat radius must be > 0");
     v[i] = Splat{f[0], f[1], f[2], f[3], f[4], f[5]};
+  }
+  if (s->storage != ST_F32) {
+    NX_LAUNCH_TIMED(s, NX_TIME_SPLAT,
+                    launch_m_splats(s->storage, s->vel[s->vcur],
+                                    s->dye[0] ? (void*)s->dye[s->dcur] : nullptr,
+                                    s->W, s->H, v.data(), n, s->stream));
+    return NX_OK;
   }
   NX_LAUNCH_TIMED(s, NX_TIME_SPLAT,
                   launch_gaussian_splats(s->vel[s->vcur],
@@ -706,6 +805,8 @@ int nx_set_graph(nx_sim* s, int on) {
 // ---- per-stage entry points
 int nx_stage_boundaries(nx_sim* s) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_TRY(validate(s, s->prm));
   NX_HIP(launch_bc(s->vel[s->vcur], s->W, s->H, bc_of(s->prm), s->stream));
   return NX_OK;
@@ -713,6 +814,8 @@ int nx_stage_boundaries(nx_sim* s) {
 
 int nx_stage_advect(nx_sim* s, float dt) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_HIP(launch_advect(s->vel[s->vcur], s->vel[1 - s->vcur], s->mask[s->mcur],
                        s->W, s->H, dt, s->prm.speed, s->prm.dissipation,
                        bc_identity(), s->stream));
@@ -722,12 +825,16 @@ int nx_stage_advect(nx_sim* s, float dt) {
 
 int nx_stage_curl(nx_sim* s) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_HIP(launch_curl(s->vel[s->vcur], s->curl, s->W, s->H, s->stream));
   return NX_OK;
 }
 
 int nx_stage_confine(nx_sim* s, float dt) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_HIP(launch_confine(s->vel[s->vcur], s->curl, s->vel[1 - s->vcur], s->W,
                         s->H, dt, s->prm.vorticity, s->stream));
   s->vcur ^= 1;
@@ -736,6 +843,8 @@ int nx_stage_confine(nx_sim* s, float dt) {
 
 int nx_stage_viscosity(nx_sim* s) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   if (!(s->prm.viscosity > 0.0f))
     return fail(NX_ERR_STATE, "viscosity is 0: the reference skips this pass");
   const double a = 1.0 / (double)s->prm.viscosity;
@@ -748,6 +857,8 @@ int nx_stage_viscosity(nx_sim* s) {
 
 int nx_stage_divergence(nx_sim* s) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_HIP(launch_divergence(s->vel[s->vcur], s->mask[s->mcur], s->div, s->W,
                            s->H, s->stream));
   return NX_OK;
@@ -755,12 +866,16 @@ int nx_stage_divergence(nx_sim* s) {
 
 int nx_stage_clear_pressure(nx_sim* s) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_HIP(launch_fill(s->p[s->pcur], 0.0f, (size_t)s->W * s->H, s->stream));
   return NX_OK;
 }
 
 int nx_stage_jacobi(nx_sim* s, int sweeps) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   if (sweeps < 0) return fail(NX_ERR_ARG, "sweeps must be >= 0");
   for (int i = 0; i < sweeps; ++i) {
     NX_LAUNCH_TIMED(s, NX_TIME_JACOBI,
@@ -774,6 +889,8 @@ int nx_stage_jacobi(nx_sim* s, int sweeps) {
 
 int nx_stage_subtract_gradient(nx_sim* s) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_HIP(launch_subtract_gradient(s->vel[s->vcur], s->p[s->pcur],
                                   s->mask[s->mcur], s->vel[1 - s->vcur],
                                   nullptr, s->W, s->H, s->stream));
@@ -790,6 +907,8 @@ int nx_stage_clear_obstacles(nx_sim* s) {
 
 int nx_stage_advect_dye(nx_sim* s, float dt) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   if (!s->dye[0]) return fail(NX_ERR_STATE, "dye is not enabled");
   NX_HIP(launch_advect_dye(s->dye[s->dcur], s->dye[1 - s->dcur],
                            s->vel[s->vcur], s->mask[s->mcur], s->dyeW,
@@ -801,6 +920,8 @@ int nx_stage_advect_dye(nx_sim* s, float dt) {
 
 int nx_stage_fused_vort_div(nx_sim* s, float dt) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   float alpha = 0.0f, rbeta = 0.0f;
   const int viscous = s->prm.viscosity > 0.0f;
   if (viscous) {
@@ -820,6 +941,8 @@ int nx_stage_fused_vort_div(nx_sim* s, float dt) {
 int nx_jacobi_blocked(nx_sim* s, int sweeps, int k, int rows, int waves,
                       int cols) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   if (sweeps < 0) return fail(NX_ERR_ARG, "sweeps must be >= 0");
   JacobiCfg c{k, rows, waves, cols};
   if (!jacobi_cfg_supported(c))
@@ -833,15 +956,19 @@ static int field_info(nx_sim* s, int field, void** ptr, size_t* bytes) {
   switch (field) {
     case NX_FIELD_VELOCITY:
       *ptr = s->vel[s->vcur];
-      *bytes = n * sizeof(float2);
+      *bytes = n * 2 * s->es;
       return NX_OK;
     case NX_FIELD_PRESSURE:
       *ptr = s->p[s->pcur];
+      *bytes = n * s->es;
+      return NX_OK;
+    case NX_FIELD_PRESSURE32:
+      *ptr = s->p32 ? s->p32 : s->p[s->pcur];
       *bytes = n * sizeof(float);
       return NX_OK;
     case NX_FIELD_DIVERGENCE:
       *ptr = s->div;
-      *bytes = n * sizeof(float);
+      *bytes = n * s->es;
       return NX_OK;
     case NX_FIELD_CURL:
       *ptr = s->curl;
@@ -854,7 +981,7 @@ static int field_info(nx_sim* s, int field, void** ptr, size_t* bytes) {
     case NX_FIELD_DYE:
       if (!s->dye[0]) return fail(NX_ERR_STATE, "dye is not enabled");
       *ptr = s->dye[s->dcur];
-      *bytes = (size_t)s->dyeW * s->dyeH * sizeof(float);
+      *bytes = (size_t)s->dyeW * s->dyeH * s->es;
       return NX_OK;
   }
   return fail(NX_ERR_ARG, "unknown field id");
@@ -891,6 +1018,8 @@ int nx_download(nx_sim* s, int field, void* dst, size_t bytes) {
 
 int nx_residual(nx_sim* s, double* out) {
   if (!s || !out) return fail(NX_ERR_ARG, "null argument");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   NX_HIP(launch_sums(s->p[s->pcur], s->div, s->mask[s->mcur], s->W, s->H,
                      s->red5, s->stream));
   NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums,

@@ -4,6 +4,7 @@
 // library is compiled with -ffp-contract=off so every + - * / rounds once,
 // as a float32 shader evaluation without FMA contraction does.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -103,10 +104,94 @@ struct BcParams {
   float inflow;
 };
 
-__device__ __forceinline__ float2 load_bc(const float2* __restrict__ u, int x,
-                                          int y, int W, int H,
+// ---- storage formats --------------------------------------------------
+// Velocity, pressure, divergence and dye may be stored as float32, bfloat16
+// or float16; every kernel widens on load and computes in float32, and
+// rounds to nearest even on store.  Velocity pairs are stored interleaved
+// (x, y) in one 8-, 4- or 4-byte word.
+enum Storage { ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2 };
+
+struct F32S {
+  using s1 = float;
+  using s2 = float2;
+  static constexpr int id = ST_F32;
+  __device__ static __forceinline__ float ld(const s1* p, long long i) { return p[i]; }
+  __device__ static __forceinline__ void st(s1* p, long long i, float v) { p[i] = v; }
+  __device__ static __forceinline__ float2 ld2(const s2* p, long long i) { return p[i]; }
+  __device__ static __forceinline__ void st2(s2* p, long long i, float2 v) { p[i] = v; }
+  // two adjacent scalars p[i], p[i+1] (i even)
+  __device__ static __forceinline__ float2 ldp(const s1* p, long long i) {
+    return *reinterpret_cast<const float2*>(p + i);
+  }
+  __device__ static __forceinline__ void stp(s1* p, long long i, float2 v) {
+    *reinterpret_cast<float2*>(p + i) = v;
+  }
+};
+
+__device__ __forceinline__ float bf16_to_f32(uint32_t h) {
+  return __uint_as_float(h << 16);
+}
+
+// round to nearest, ties to even (finite inputs)
+__device__ __forceinline__ uint32_t f32_to_bf16(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+struct BF16S {
+  using s1 = uint16_t;  // bfloat16 bits
+  using s2 = uint32_t;  // x in bits 0..15, y in bits 16..31
+  static constexpr int id = ST_BF16;
+  __device__ static __forceinline__ float ld(const s1* p, long long i) {
+    return bf16_to_f32(p[i]);
+  }
+  __device__ static __forceinline__ void st(s1* p, long long i, float v) {
+    p[i] = (uint16_t)f32_to_bf16(v);
+  }
+  __device__ static __forceinline__ float2 ld2(const s2* p, long long i) {
+    const uint32_t v = p[i];
+    return make_float2(bf16_to_f32(v & 0xFFFFu), __uint_as_float(v & 0xFFFF0000u));
+  }
+  __device__ static __forceinline__ void st2(s2* p, long long i, float2 v) {
+    p[i] = f32_to_bf16(v.x) | (f32_to_bf16(v.y) << 16);
+  }
+  __device__ static __forceinline__ float2 ldp(const s1* p, long long i) {
+    return ld2(reinterpret_cast<const s2*>(p + i), 0);
+  }
+  __device__ static __forceinline__ void stp(s1* p, long long i, float2 v) {
+    st2(reinterpret_cast<s2*>(p + i), 0, v);
+  }
+};
+
+struct F16S {
+  using s1 = __half;
+  using s2 = __half2;
+  static constexpr int id = ST_F16;
+  __device__ static __forceinline__ float ld(const s1* p, long long i) {
+    return __half2float(p[i]);
+  }
+  __device__ static __forceinline__ void st(s1* p, long long i, float v) {
+    p[i] = __float2half_rn(v);
+  }
+  __device__ static __forceinline__ float2 ld2(const s2* p, long long i) {
+    return __half22float2(p[i]);
+  }
+  __device__ static __forceinline__ void st2(s2* p, long long i, float2 v) {
+    p[i] = __floats2half2_rn(v.x, v.y);
+  }
+  __device__ static __forceinline__ float2 ldp(const s1* p, long long i) {
+    return ld2(reinterpret_cast<const s2*>(p + i), 0);
+  }
+  __device__ static __forceinline__ void stp(s1* p, long long i, float2 v) {
+    st2(reinterpret_cast<s2*>(p + i), 0, v);
+  }
+};
+
+template <class S>
+__device__ __forceinline__ float2 load_bc(const typename S::s2* __restrict__ u,
+                                          int x, int y, int W, int H,
                                           const BcParams& bc) {
-  const float2 v = u[y * W + x];
+  const float2 v = S::ld2(u, y * W + x);
   const bool left = x == 0, right = x == W - 1, bot = y == 0, top = y == H - 1;
   if (!(left || right || bot || top)) return v;
   if (bc.mode == BC_REFERENCE) {
@@ -114,19 +199,25 @@ __device__ __forceinline__ float2 load_bc(const float2* __restrict__ u, int x,
   }
   if (bc.mode == BC_FREE_SLIP) {
     if ((left || right) && (bot || top)) return make_float2(0.0f, 0.0f);
-    if (left) return make_float2(0.0f, u[y * W + 1].y);
-    if (right) return make_float2(0.0f, u[y * W + W - 2].y);
-    if (bot) return make_float2(u[W + x].x, 0.0f);
-    return make_float2(u[(H - 2) * W + x].x, 0.0f);
+    if (left) return make_float2(0.0f, S::ld2(u, y * W + 1).y);
+    if (right) return make_float2(0.0f, S::ld2(u, y * W + W - 2).y);
+    if (bot) return make_float2(S::ld2(u, W + x).x, 0.0f);
+    return make_float2(S::ld2(u, (H - 2) * W + x).x, 0.0f);
   }
   // BC_INFLOW
   if (left) return make_float2(bc.inflow, 0.0f);
   if (right) {
     if (bot || top) return make_float2(0.0f, 0.0f);
-    return u[y * W + W - 2];
+    return S::ld2(u, y * W + W - 2);
   }
-  if (bot) return make_float2(u[W + x].x, 0.0f);
-  return make_float2(u[(H - 2) * W + x].x, 0.0f);
+  if (bot) return make_float2(S::ld2(u, W + x).x, 0.0f);
+  return make_float2(S::ld2(u, (H - 2) * W + x).x, 0.0f);
+}
+
+__device__ __forceinline__ float2 load_bc(const float2* __restrict__ u, int x,
+                                          int y, int W, int H,
+                                          const BcParams& bc) {
+  return load_bc<F32S>(u, x, y, W, H, bc);
 }
 
 struct Splat {  // Gaussian splat: u += (ax, ay) * w, dye += dye * w

@@ -37,11 +37,13 @@ __device__ __forceinline__ float2 confine_cell(float2 u, float wc, float wl,
 }
 
 // One 64x16 output tile with origin (ox, oy); 256 threads as (64, 4).
+// Velocity, divergence and p1 are stored in format S; the chain is float32.
+template <class S = F32S>
 __device__ __forceinline__ void fused_vort_div_tile(
-    int ox, int oy, const float2* __restrict__ uin,
-    const uint8_t* __restrict__ mask, float2* __restrict__ uout,
-    float* __restrict__ div, float* __restrict__ p1, int W, int H, float dt,
-    float scale, int viscous, float alpha, float rbeta) {
+    int ox, int oy, const typename S::s2* __restrict__ uin,
+    const uint8_t* __restrict__ mask, typename S::s2* __restrict__ uout,
+    typename S::s1* __restrict__ div, typename S::s1* __restrict__ p1, int W,
+    int H, float dt, float scale, int viscous, float alpha, float rbeta) {
   __shared__ float2 A[(TX + 2 * HMAX) * (TY + 2 * HMAX)];
   __shared__ float Wc[(TX + 2 * HMAX - 2) * (TY + 2 * HMAX - 2)];
   __shared__ float2 V2[(TX + 2 * HMAX - 4) * (TY + 2 * HMAX - 4)];
@@ -56,7 +58,7 @@ __device__ __forceinline__ void fused_vort_div_tile(
     const int gy = t.cy(t.oy - hA + j);
     for (int i = tx; i < TX + 2 * hA; i += 64) {
       const int gx = t.cx(t.ox - hA + i);
-      A[j * (TX + 2 * hA) + i] = uin[gy * W + gx];
+      A[j * (TX + 2 * hA) + i] = S::ld2(uin, gy * W + gx);
     }
   }
   __syncthreads();
@@ -133,9 +135,9 @@ __device__ __forceinline__ void fused_vort_div_tile(
       const float y2 = mask[yt * W + gx] ? 0.0f : Vf[sidx(t, hf, gx, yt)].y;
       const float d = 0.5f * ((x2 - x1) + (y2 - y1));
       const int pos = gy * W + gx;
-      uout[pos] = Vf[sidx(t, hf, gx, gy)];
-      div[pos] = d;
-      if (p1) p1[pos] = (0.0f - d) * 0.25f;
+      S::st2(uout, pos, Vf[sidx(t, hf, gx, gy)]);
+      S::st(div, pos, d);
+      if (p1) S::st(p1, pos, (0.0f - d) * 0.25f);
     }
   }
 }

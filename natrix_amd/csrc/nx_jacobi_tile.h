@@ -38,11 +38,18 @@ __device__ __forceinline__ int ilane_from_right(int v) {
 // k sweeps on tile (bx, by) of the (VX x VY)-interior tiling; `rid` is the
 // partial-sum record this tile writes when red != nullptr.  Uses
 // __syncthreads(): every thread of the block must call it.
-template <int ROWS, int WAVES, int COLS>
+// p and b are stored in format S and held in float32 registers across the k
+// sweeps; p is rounded to S once, at the store.
+// CORR (residual correction): the first of the k passes computes the float32
+// residual r = b - A p of the stored iterate instead of a sweep, the other
+// k-1 passes are Jacobi sweeps of A e = r from e = 0, and the tile writes
+// the float32 pressure p + e to pout32 (pout is unused).
+template <int ROWS, int WAVES, int COLS, class S = F32S, bool CORR = false>
 __device__ __forceinline__ void jacobi_tb_tile(
-    int bx, int by, const float* __restrict__ pin, const float* __restrict__ b,
-    const uint8_t* __restrict__ mask, float* __restrict__ pout, int W, int H,
-    int k, double* __restrict__ red, int rid) {
+    int bx, int by, const typename S::s1* __restrict__ pin,
+    const typename S::s1* __restrict__ b, const uint8_t* __restrict__ mask,
+    typename S::s1* __restrict__ pout, int W, int H, int k,
+    double* __restrict__ red, int rid, float* __restrict__ pout32 = nullptr) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -65,8 +72,8 @@ __device__ __forceinline__ void jacobi_tb_tile(
       int m0 = 1, m1 = 1;
       if (in) {
         const int idx = gy * W + gx0;
-        pv = *reinterpret_cast<const float2*>(pin + idx);
-        bv = *reinterpret_cast<const float2*>(b + idx);
+        pv = S::ldp(pin, idx);
+        bv = S::ldp(b, idx);
         const uint16_t mm = *reinterpret_cast<const uint16_t*>(mask + idx);
         m0 = (mm & 0xFF) != 0;
         m1 = (mm >> 8) != 0;
@@ -86,8 +93,8 @@ __device__ __forceinline__ void jacobi_tb_tile(
         sol[r][c] = 1;
         if (row_in && gx >= 0 && gx < W) {
           const int idx = gy * W + gx;
-          p[r][c] = pin[idx];
-          bb[r][c] = b[idx];
+          p[r][c] = S::ld(pin, idx);
+          bb[r][c] = S::ld(b, idx);
           sol[r][c] = mask[idx] != 0;
         }
       }
@@ -130,7 +137,47 @@ __device__ __forceinline__ void jacobi_tb_tile(
   const int trow0 = wave * ROWS;  // tile row of p[0][*]
   const int tcol0 = COLS * lane;  // tile column of p[*][0]
 
-  for (int s = 0; s < k; ++s) {
+  float pst[CORR ? ROWS : 1][CORR ? COLS : 1];  // stored iterate (CORR)
+  int s0 = 0;
+  if (CORR) {
+    // pass 0: r = b - ((((x1 + x2) + y1) + y2) - 4 p), float32
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      xch[0][0][wave][tcol0 + c] = p[0][c];
+      xch[0][1][wave][tcol0 + c] = p[ROWS - 1][c];
+    }
+    __syncthreads();
+    float prev[COLS], above[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      prev[c] = wave > 0 ? xch[0][1][wave - 1][tcol0 + c] : p[0][c];
+      above[c] = wave < WAVES - 1 ? xch[0][0][wave + 1][tcol0 + c] : p[ROWS - 1][c];
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const float lft = lane_from_left(p[r][COLS - 1]);
+      const float rgt = lane_from_right(p[r][0]);
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const float pc = p[r][c];
+        const int code = cd[r][c];
+        const float x1 = (code & 1) ? pc : (c > 0 ? p[r][c - 1] : lft);
+        const float x2 = (code & 2) ? pc : (c < COLS - 1 ? p[r][c + 1] : rgt);
+        const float y1 = (code & 4) ? pc : prev[c];
+        const float y2 = (code & 8) ? pc : (r < ROWS - 1 ? p[r + 1][c] : above[c]);
+        bb[r][c] = bb[r][c] - ((((x1 + x2) + y1) + y2) - 4.0f * pc);
+      }
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        prev[c] = p[r][c];
+        pst[CORR ? r : 0][CORR ? c : 0] = p[r][c];
+        p[r][c] = 0.0f;
+      }
+    }
+    s0 = 1;
+  }
+
+  for (int s = s0; s < k; ++s) {
     const int buf = s & 1;
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
@@ -191,16 +238,22 @@ __device__ __forceinline__ void jacobi_tb_tile(
     const int trow = trow0 + r;
     const int gy = gy0 + r;
     if (trow < k || trow >= EY - k || gy < 0 || gy >= H) continue;
-    if (vec) {
+    if (CORR) {
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const int gx = gx0 + c, tcol = tcol0 + c;
+        if (tcol >= k && tcol < EX - k && gx >= 0 && gx < W)
+          pout32[gy * W + gx] = pst[CORR ? r : 0][CORR ? c : 0] + p[r][c];
+      }
+    } else if (vec) {
       if (tcol0 >= k && tcol0 < EX - k && gx0 >= 0 && gx0 < W)
-        *reinterpret_cast<float2*>(pout + gy * W + gx0) =
-            make_float2(p[r][0], p[r][COLS - 1]);
+        S::stp(pout, gy * W + gx0, make_float2(p[r][0], p[r][COLS - 1]));
     } else {
 #pragma unroll
       for (int c = 0; c < COLS; ++c) {
         const int gx = gx0 + c, tcol = tcol0 + c;
         if (tcol >= k && tcol < EX - k && gx >= 0 && gx < W)
-          pout[gy * W + gx] = p[r][c];
+          S::st(pout, gy * W + gx, p[r][c]);
       }
     }
   }

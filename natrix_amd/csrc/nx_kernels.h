@@ -97,4 +97,25 @@ hipError_t launch_reduce(const double* part, int nblocks, int width, int off,
 hipError_t launch_center_rhs(float* b, float* p, const uint8_t* mask, int n,
                              const double* sums, hipStream_t s);
 
+// ---- reduced-precision storage (st = ST_BF16 or ST_F16) ------------------
+hipError_t launch_m_advect(int st, const void* uin, void* uout, const uint8_t* mask,
+                           int W, int H, float dt, float speed, float diss,
+                           BcParams bc, hipStream_t s);
+hipError_t launch_m_fused(int st, const void* uin, const uint8_t* mask, void* uout,
+                          void* div, void* p1, int W, int H, float dt, float scale,
+                          int viscous, float alpha, float rbeta, hipStream_t s);
+// corr = false: k sweeps, p stored in `st`; corr = true: float32 residual
+// correction (1 residual pass + k-1 sweeps), float32 p + e to pout32
+hipError_t launch_m_jacobi(int st, const void* pin, const void* b, const uint8_t* mask,
+                           void* pout, float* pout32, int W, int H, JacobiCfg c,
+                           bool corr, hipStream_t s);
+hipError_t launch_m_subgrad(int st, const void* uin, const float* p,
+                            const uint8_t* mask, void* uout, uint8_t* mask_next,
+                            int W, int H, hipStream_t s);
+hipError_t launch_m_advect_dye(int st, const void* qin, void* qout, const void* u,
+                               const uint8_t* mask, int Wp, int Hp, int Wv, int Hv,
+                               float dt, float speed, float diss, hipStream_t s);
+hipError_t launch_m_splats(int st, void* u, void* q, int W, int H, const Splat* sp,
+                           int n, hipStream_t s);
+
 }  // namespace nx
