@@ -160,6 +160,111 @@ def bench_batch(args, copy_bandwidth):
     print(json.dumps(out), flush=True)
 
 
+def mixed_scene(N, storage, iters, dye=True):
+    """4096^2-class scene: 8 seeded Gaussian splats (seed 4) every step,
+    free-slip walls, vorticity 0.3, no viscosity, dye on."""
+    from natrix_amd import FluidSimulator
+    rng = np.random.default_rng(4)
+    splats = []
+    for _ in range(8):
+        f = rng.uniform(50, 200) / 60.0
+        ang = rng.uniform(0, 2 * np.pi)
+        splats.append((rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9),
+                       f * np.cos(ang), f * np.sin(ang), 0.02 * N, 1.0))
+    sim = FluidSimulator(N, N, dye=dye, bc_mode="free_slip", storage=storage)
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    sim.iterations = iters
+    return sim, splats
+
+
+def mixed_error_vs_fp64(storage, N=256, steps=20, iters=120):
+    """Max-norm velocity error of the reduced-storage GPU rollout and of the
+    float32 GPU rollout against the float64 oracle rollout (CPU)."""
+    from oracle import fluid as F
+    errs = {}
+    ref = None
+    for st in (storage, "f32"):
+        sim, splats = mixed_scene(N, st, iters, dye=False)
+        for _ in range(steps):
+            sim.add_splats(splats)
+            sim.update(1 / 60)
+        got = sim.download("velocity").astype(np.float64)
+        sim.destroy()
+        if ref is None:
+            prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=iters,
+                           bc_mode=F.BC_FREE_SLIP)
+            u = np.zeros((N, N, 2))
+            p = np.zeros((N, N))
+            o = np.zeros((N, N, 2))
+            for _ in range(steps):
+                u, _ = F.gaussian_splat(u, None, [tuple(float(np.float32(v)) for v in s)
+                                                  for s in splats])
+                u, p, o, _ = F.step(u, p, o, 1 / 60, prm)
+            ref = u
+        errs[st] = float(np.abs(got - ref).max() / np.abs(ref).max())
+    return errs
+
+
+def bench_mixed(args):
+    import torch
+    N = 4096 if args.size == 2048 else args.size
+    iters = 120 if args.iters == 80 else args.iters
+    sim, splats = mixed_scene(N, args.storage, iters)
+    log(f"bench mixed: {N}^2 {args.storage} I={iters} tile {sim.jacobi_tile}")
+    if args.graph:
+        sim.use_graph = True
+    for _ in range(args.warmup):
+        sim.add_splats(splats)
+        sim.update(1 / 60)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    sim.set_timing(not args.graph)
+    sim.collect_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.add_splats(splats)
+        sim.update(1 / 60)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    timing = sim.collect_timing()
+    u_mixed = sim.download("velocity").astype(np.float64)
+    sim.destroy()
+    # the same rollout in float32 on the GPU, for the full-size error
+    ref, _ = mixed_scene(N, "f32", iters)
+    for _ in range(args.warmup + args.steps):
+        ref.add_splats(splats)
+        ref.update(1 / 60)
+    u32 = ref.download("velocity").astype(np.float64)
+    ref.destroy()
+    err32 = float(np.abs(u_mixed - u32).max() / np.abs(u32).max())
+    errs = mixed_error_vs_fp64(args.storage)
+    ms = 1e3 * el / args.steps
+    cells = N * N
+    # bf16/f16 bytes per cell: 54 + 7*I (BASELINE.md) + splats 12 + the
+    # correction launch (p 2 + b 2 + mask 1 in, p32 4 out) + projection's
+    # float32 p (2 extra)
+    step_bytes = cells * (54 + 7 * (iters - 1) + 12 + 9 + 2)
+    jac_ms, jac_n = timing["jacobi"]
+    out = {"metric": f"sim steps/sec at {N}^2, {args.storage} storage",
+           "value": round(args.steps / el, 3), "unit": "steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": f"{args.storage} storage, f32 arithmetic", "data": "synthetic",
+           "config": {"workload": f"{N}x{N}, 8 Gaussian splats/step, I={iters} + "
+                                  f"8 float32 correction sweeps, free-slip walls, dye",
+                      "jacobi_tile": None, "graph": bool(args.graph)},
+           "step_algorithmic_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
+           "step_roofline_frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "stage_ms_per_step": {k: round(v[0] / args.steps, 5)
+                                 for k, v in timing.items() if v[1]},
+           "velocity_rel_err_vs_f32_gpu": err32,
+           "velocity_rel_err_vs_fp64_oracle_256sq_20steps": errs,
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,8 +281,11 @@ def main():
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single",
-                    help="batch: 64 uneven scenes in one launch per stage")
+    ap.add_argument("--workload", choices=["single", "batch", "mixed"],
+                    default="single",
+                    help="batch: 64 uneven scenes in one launch per stage; "
+                         "mixed: 4096^2 with bf16/f16 storage")
+    ap.add_argument("--storage", choices=["bf16", "f16"], default="bf16")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the copy-bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -204,6 +312,8 @@ def main():
 
     if args.workload == "batch":
         return bench_batch(args, copy_bandwidth)
+    if args.workload == "mixed":
+        return bench_mixed(args)
 
     N = args.size
     solid, centre = make_scene(N)

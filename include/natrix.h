@@ -129,6 +129,29 @@ int nx_set_obstacle_map(nx_sim* sim, const uint8_t* map);
 /* ---- the step (FluidSimulator.update, fluid_simulator.py:174-280, then the
  *      dye advection SmoothParticlesArea.update, :85-104, if dye is on) ---- */
 int nx_update(nx_sim* sim, float dt);
+/* ---- the step in phases, for a caller-provided pressure guess (the
+ *      learned solve).  begin: walls + advection + the fused curl ..
+ *      divergence stage; the divergence is centred on its fluid mean and the
+ *      pressure set to the first Jacobi sweep from zero.  The caller may then
+ *      overwrite or correct the pressure (NX_FIELD_PRESSURE, or the coarse
+ *      transfers below) and call nx_solve any number of times; finish:
+ *      projection, obstacle clear, dye advection. ---- */
+int nx_step_begin(nx_sim* sim, float dt);
+/* Jacobi from the current pressure until the FP64 relative residual is
+ * <= tol or max_sweeps sweeps were applied; `out` (nullable) gets the stats
+ * of the whole step so far */
+int nx_solve(nx_sim* sim, double tol, int max_sweeps, nx_stats* out);
+/* coarse correction problem: coarse_rhs[Hc*Wc] (device, float32) = factor^2
+ * x the fluid mean of the fine residual per factor x factor block,
+ * coarse_mask[Hc*Wc] (device, uint8) = 1 for blocks at least half solid;
+ * Wc = ceil(W/factor), Hc = ceil(H/factor) */
+int nx_restrict_residual(nx_sim* sim, int factor, float* coarse_rhs,
+                         uint8_t* coarse_mask);
+/* pressure += scale * bilinear interpolation of coarse[Hc*Wc] (device) on
+ * fluid cells */
+int nx_prolong_add(nx_sim* sim, int factor, const float* coarse, float scale);
+int nx_step_finish(nx_sim* sim);
+
 /* Capture the fixed-iteration step into a HIP graph and replay it on later
  * nx_update calls with the same dt and parameters (on = 1). */
 int nx_set_graph(nx_sim* sim, int on);
@@ -207,9 +230,46 @@ int nx_batch_upload(nx_batch* b, int scene, int field, const void* src,
 int nx_batch_download(nx_batch* b, int scene, int field, void* dst,
                       size_t bytes);
 int nx_batch_cells(nx_batch* b, long long* cells);
+/* FP64 relative residual of every scene's current pressure (synchronises):
+ * out[6*s + {sum r, sum r^2, sum b, sum b^2, fluid cells, rel}] */
+int nx_batch_residuals(nx_batch* b, double* out);
 int nx_batch_synchronize(nx_batch* b);
 int nx_batch_set_timing(nx_batch* b, int on);
 int nx_batch_collect_timing(nx_batch* b, double* ms, int* launches);
+
+/* ---- U-Net building blocks (learned pressure guess; no reference
+ *      counterpart).  NHWC bfloat16 activations, KRSC bfloat16 3x3 weights
+ *      (w[cout][ky*3+kx][cin]), float32 accumulation on the matrix cores.
+ *      All pointers are device pointers; `stream` is a hipStream_t. ---- */
+/* y = [relu](conv3x3(x, w) + bias), zero padding; cin % 8 == 0, cout % 32 == 0 */
+int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int n,
+                   int h, int wd, int cin, int cout, int relu, void* stream);
+/* dx = conv3x3 input gradient of dy; wt: scratch of 9*cin*cout bfloat16 */
+int nx_conv3x3_dgrad(const void* dy, const void* w, void* dx, void* wt, int n, int h,
+                     int wd, int cin, int cout, void* stream);
+/* dw (float32 [cout][9][cin]) = weight gradient; work: splits*cout*9*cin float32 */
+int nx_conv3x3_wgrad(const void* x, const void* dy, float* dw, float* work, int splits,
+                     int n, int h, int wd, int cin, int cout, void* stream);
+/* dz = dy where y > 0 else 0 (bfloat16, `count` elements) */
+int nx_relu_bwd(const void* y, const void* dy, void* dz, long long count, void* stream);
+/* out[c] = sum over rows of a bfloat16 (rows x c) matrix; work: 1024*c float32 */
+int nx_colsum(const void* a, float* out, float* work, long long rows, int c,
+              void* stream);
+/* 2x2 average pool (backward: spread dy/4); h, wd = the full-resolution size */
+int nx_avgpool2(const void* x, void* y, int n, int h, int wd, int c, int backward,
+                void* stream);
+/* 2x nearest upsample (backward: sum of 4); h, wd = the full-resolution size */
+int nx_upsample2(const void* x, void* y, int n, int h, int wd, int c, int backward,
+                 void* stream);
+/* channel concat y = [a, b] (split = 1: a, b = y's parts) */
+int nx_concat(void* y, void* a, void* b, long long pixels, int ca, int cb, int split,
+              void* stream);
+/* 1x1 head to one float32 channel, and its backward (dwb[0..c-1] = dw,
+ * dwb[c] = dbias; work: 1024*(c+1) float32) */
+int nx_head_fwd(const void* x, const float* w, const float* bias, float* out,
+                long long pixels, int c, void* stream);
+int nx_head_bwd(const void* x, const float* w, const float* dout, void* dx, float* dwb,
+                float* work, long long pixels, int c, void* stream);
 
 /* attainable bandwidth: a plain device copy of `bytes` (multiple of 16),
  * `iters` times; *gbps = (read + write bytes) / time */

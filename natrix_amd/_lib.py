@@ -128,9 +128,25 @@ SIGNATURES = {
     "nx_batch_upload": [_P, _I, _I, _P, _SZ],
     "nx_batch_download": [_P, _I, _I, _P, _SZ],
     "nx_batch_cells": [_P, C.POINTER(C.c_longlong)],
+    "nx_batch_residuals": [_P, C.POINTER(_D)],
     "nx_batch_synchronize": [_P],
     "nx_batch_set_timing": [_P, _I],
     "nx_batch_collect_timing": [_P, C.POINTER(_D), C.POINTER(_I)],
+    "nx_step_begin": [_P, _F],
+    "nx_solve": [_P, _D, _I, C.POINTER(Stats)],
+    "nx_restrict_residual": [_P, _I, _P, _P],
+    "nx_prolong_add": [_P, _I, _P, _F],
+    "nx_step_finish": [_P],
+    "nx_conv3x3_fwd": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
+    "nx_conv3x3_dgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
+    "nx_conv3x3_wgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
+    "nx_relu_bwd": [_P, _P, _P, C.c_longlong, _P],
+    "nx_colsum": [_P, _P, _P, C.c_longlong, _I, _P],
+    "nx_avgpool2": [_P, _P, _I, _I, _I, _I, _I, _P],
+    "nx_upsample2": [_P, _P, _I, _I, _I, _I, _I, _P],
+    "nx_concat": [_P, _P, _P, C.c_longlong, _I, _I, _I, _P],
+    "nx_head_fwd": [_P, _P, _P, _P, C.c_longlong, _I, _P],
+    "nx_head_bwd": [_P, _P, _P, _P, _P, _P, C.c_longlong, _I, _P],
     "nx_copy_bandwidth": [_SZ, _I, C.POINTER(_D)],
     "nx_last_error": [],
 }

@@ -144,6 +144,13 @@ class BatchedScenes:
         _lib.call("nx_batch_upload", self._alive(), int(scene), fid,
                   a.ctypes.data_as(C.c_void_p), a.nbytes)
 
+    def residuals(self):
+        """Relative FP64 residual of every scene's current pressure."""
+        n = len(self.sizes)
+        out = (C.c_double * (6 * n))()
+        _lib.call("nx_batch_residuals", self._alive(), out)
+        return np.array([out[6 * s + 5] for s in range(n)])
+
     def synchronize(self):
         _lib.call("nx_batch_synchronize", self._alive())
 

@@ -79,6 +79,10 @@ struct nx_sim {
   std::vector<TimedPair> timed;
   std::vector<hipEvent_t> free_events;
 
+  // step phases (nx_step_begin .. nx_step_finish)
+  bool in_step = false;
+  float phase_dt = 0.0f;
+  int step_sweeps = 0;
   // graphs keyed by the parities of the velocity, mask and dye buffers
   bool graph_on = false;
   hipGraphExec_t graph[8] = {};
@@ -490,6 +494,66 @@ static int step_fused_enqueue(nx_sim* s, float dt, bool tolerance) {
   return NX_OK;
 }
 
+// ---- coarse-grid transfer for the learned guess --------------------------
+// Restriction: coarse cell (I, J) of factor f covers fine cells
+// [fI, fI+f) x [fJ, fJ+f).  rhs_c = f^2 * mean over its fluid fine cells of
+// the float32 residual r = b - A p (the correction equation in coarse grid
+// units); mask_c = 1 when at least half of its fine cells are solid.
+__global__ void __launch_bounds__(256)
+    k_restrict(const float* __restrict__ p, const float* __restrict__ b,
+               const uint8_t* __restrict__ mask, float* __restrict__ rc,
+               uint8_t* __restrict__ mc, int W, int H, int f, int Wc, int Hc) {
+  const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y * 4 + threadIdx.y;
+  if (X >= Wc || Y >= Hc) return;
+  float sum = 0.0f;
+  int fluid = 0, total = 0;
+  for (int j = 0; j < f; ++j) {
+    const int y = Y * f + j;
+    if (y >= H) break;
+    for (int i = 0; i < f; ++i) {
+      const int x = X * f + i;
+      if (x >= W) break;
+      ++total;
+      const int pos = y * W + x;
+      if (mask[pos]) continue;
+      const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
+      const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
+      const float pc = p[pos];
+      const float x1 = mask[iL] ? pc : p[iL];
+      const float x2 = mask[iR] ? pc : p[iR];
+      const float y1 = mask[iB] ? pc : p[iB];
+      const float y2 = mask[iT] ? pc : p[iT];
+      sum += b[pos] - ((((x1 + x2) + y1) + y2) - 4.0f * pc);
+      ++fluid;
+    }
+  }
+  const bool solid = 2 * (total - fluid) >= total;
+  rc[Y * Wc + X] = (solid || fluid == 0) ? 0.0f : (float)(f * f) * (sum / (float)fluid);
+  mc[Y * Wc + X] = solid ? 1 : 0;
+}
+
+// Prolongation: p += scale * bilinear(e_c) on fluid cells, coarse cell
+// centres at fine coordinate f*I + (f-1)/2, clamped at the edges.
+__global__ void __launch_bounds__(256)
+    k_prolong_add(float* __restrict__ p, const uint8_t* __restrict__ mask,
+                  const float* __restrict__ ec, int W, int H, int f, int Wc, int Hc,
+                  float scale) {
+  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  if (x >= W || y >= H) return;
+  const int pos = y * W + x;
+  if (mask[pos]) return;
+  const float cx = ((float)x - 0.5f * (float)(f - 1)) / (float)f;
+  const float cy = ((float)y - 0.5f * (float)(f - 1)) / (float)f;
+  const float fx = fminf(fmaxf(cx, 0.0f), (float)(Wc - 1));
+  const float fy = fminf(fmaxf(cy, 0.0f), (float)(Hc - 1));
+  const int x0 = (int)fx, y0 = (int)fy;
+  const int x1 = min(x0 + 1, Wc - 1), y1 = min(y0 + 1, Hc - 1);
+  const float tx = fx - (float)x0, ty = fy - (float)y0;
+  const float a = ec[y0 * Wc + x0] * (1.0f - tx) + ec[y0 * Wc + x1] * tx;
+  const float c = ec[y1 * Wc + x0] * (1.0f - tx) + ec[y1 * Wc + x1] * tx;
+  p[pos] = p[pos] + scale * (a * (1.0f - ty) + c * ty);
+}
+
 // ------------------------------------------------------------- C-ABI
 extern "C" {
 
@@ -791,6 +855,125 @@ int nx_update(nx_sim* s, float dt) {
   if (dye_step) s->dcur ^= 1;
   s->stats.sweeps = P.iterations;
   s->stats.converged = 0;
+  return NX_OK;
+}
+
+int nx_step_begin(nx_sim* s, float dt) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (s->storage != ST_F32)
+    return fail(NX_ERR_STATE, "this entry point needs float32 storage");
+  NX_TRY(validate(s, s->prm));
+  const nx_params& P = s->prm;
+  const int W = s->W, H = s->H;
+  const int a = s->vcur;
+  NX_LAUNCH_TIMED(s, NX_TIME_ADVECT,
+                  launch_advect(s->vel[a], s->vel[1 - a], s->mask[s->mcur], W,
+                                H, dt, P.speed, P.dissipation, bc_of(P),
+                                s->stream));
+  float alpha = 0.0f, rbeta = 0.0f;
+  const int viscous = P.viscosity > 0.0f;
+  if (viscous) {
+    const double av = 1.0 / (double)P.viscosity;
+    alpha = (float)av;
+    rbeta = (float)(1.0 / (4.0 + av));
+  }
+  NX_LAUNCH_TIMED(s, NX_TIME_VORT_DIV,
+                  launch_fused_vort_div(s->vel[1 - a], s->mask[s->mcur],
+                                        s->vel[a], s->div, s->p[0], W, H, dt,
+                                        P.vorticity, viscous, alpha, rbeta,
+                                        s->stream));
+  s->pcur = 0;
+  NX_HIP(launch_sums(nullptr, s->div, s->mask[s->mcur], W, H, s->red5, s->stream));
+  NX_HIP(launch_reduce(s->red5, residual_blocks(W, H), 5, 0, s->sums, -1.0,
+                       nullptr, 0, 0, s->stream));
+  NX_HIP(launch_center_rhs(s->div, s->p[0], s->mask[s->mcur], W * H, s->sums,
+                           s->stream));
+  s->phase_dt = dt;
+  s->in_step = true;
+  s->step_sweeps = 1;
+  return NX_OK;
+}
+
+int nx_solve(nx_sim* s, double tol, int max_sweeps, nx_stats* out) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (!s->in_step) return fail(NX_ERR_STATE, "nx_solve outside nx_step_begin/finish");
+  if (!(tol > 0) || max_sweeps < 1) return fail(NX_ERR_ARG, "bad tol / max_sweeps");
+  nx_params saved = s->prm;
+  s->prm.tol = tol;
+  s->prm.max_iters = max_sweeps;
+  JacobiCfg c{s->prm.jacobi_k, s->prm.jacobi_rows, s->prm.jacobi_waves, s->prm.jacobi_cols};
+  int rc = ensure_red2(s, c);
+  int sweeps = 0;
+  bool converged = false;
+  double rel = 0.0;
+  while (rc == NX_OK) {
+    bool done = false;
+    rc = iterate_chunks(s, c, sweeps, done);
+    if (rc != NX_OK) break;
+    rc = exact_residual(s, &rel);
+    if (rc != NX_OK) break;
+    if (rel <= tol) {
+      converged = true;
+      break;
+    }
+    if (!done || sweeps >= max_sweeps) break;
+  }
+  s->prm = saved;
+  if (rc != NX_OK) return rc;
+  s->step_sweeps += sweeps;
+  s->stats.sweeps = s->step_sweeps;
+  s->stats.converged = converged ? 1 : 0;
+  s->stats.residual = rel;
+  s->stats.residual_sweeps = s->step_sweeps;
+  if (out) *out = s->stats;
+  return NX_OK;
+}
+
+int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
+                         uint8_t* coarse_mask) {
+  if (!s || !coarse_rhs || !coarse_mask || factor < 1)
+    return fail(NX_ERR_ARG, "bad restriction arguments");
+  if (!s->in_step) return fail(NX_ERR_STATE, "restriction outside a step");
+  const int Wc = (s->W + factor - 1) / factor, Hc = (s->H + factor - 1) / factor;
+  k_restrict<<<dim3((Wc + 63) / 64, (Hc + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+      s->p[s->pcur], s->div, s->mask[s->mcur], coarse_rhs, coarse_mask, s->W, s->H,
+      factor, Wc, Hc);
+  NX_HIP(hipGetLastError());
+  return NX_OK;
+}
+
+int nx_prolong_add(nx_sim* s, int factor, const float* coarse, float scale) {
+  if (!s || !coarse || factor < 1) return fail(NX_ERR_ARG, "bad prolongation arguments");
+  if (!s->in_step) return fail(NX_ERR_STATE, "prolongation outside a step");
+  const int Wc = (s->W + factor - 1) / factor, Hc = (s->H + factor - 1) / factor;
+  k_prolong_add<<<dim3((s->W + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+      s->p[s->pcur], s->mask[s->mcur], coarse, s->W, s->H, factor, Wc, Hc, scale);
+  NX_HIP(hipGetLastError());
+  return NX_OK;
+}
+
+int nx_step_finish(nx_sim* s) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (!s->in_step) return fail(NX_ERR_STATE, "nx_step_finish without nx_step_begin");
+  const nx_params& P = s->prm;
+  const int W = s->W, H = s->H;
+  const int a = s->vcur;
+  NX_LAUNCH_TIMED(s, NX_TIME_PROJECT,
+                  launch_subtract_gradient(s->vel[a], s->p[s->pcur],
+                                           s->mask[s->mcur], s->vel[1 - a],
+                                           s->mask[1 - s->mcur], W, H,
+                                           s->stream));
+  s->vcur = 1 - a;
+  s->mcur ^= 1;
+  if (s->dye[0] && P.dye_in_step) {
+    NX_LAUNCH_TIMED(s, NX_TIME_DYE,
+                    launch_advect_dye(s->dye[s->dcur], s->dye[1 - s->dcur],
+                                      s->vel[s->vcur], s->mask[s->mcur],
+                                      s->dyeW, s->dyeH, W, H, s->phase_dt,
+                                      P.dye_speed, P.dye_dissipation, s->stream));
+    s->dcur ^= 1;
+  }
+  s->in_step = false;
   return NX_OK;
 }
 

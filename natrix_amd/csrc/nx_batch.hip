@@ -669,6 +669,41 @@ int nx_batch_download(nx_batch* b, int scene, int field, void* dst, size_t bytes
   return NX_OK;
 }
 
+int nx_batch_residuals(nx_batch* b, double* out) {
+  if (!b || !out) return bfail(NX_ERR_ARG, "null argument");
+  // per scene: FP64 sums of the exact residual of the current pressure,
+  // reduced in a fixed order; out[6*s + {sum r, sum r^2, sum b, sum b^2,
+  // fluid cells, relative residual}]
+  int maxblk = 0;
+  for (int s = 0; s < b->S; ++s) maxblk = std::max(maxblk, residual_blocks(b->W[s], b->H[s]));
+  double* red = nullptr;
+  double* sums = nullptr;
+  NXB_HIP(hipMalloc(&red, (size_t)maxblk * 5 * sizeof(double) * b->S));
+  NXB_HIP(hipMalloc(&sums, (size_t)8 * b->S * sizeof(double)));
+  int rc = NX_OK;
+  for (int s = 0; s < b->S && rc == NX_OK; ++s) {
+    const long long o = b->off[s];
+    double* rs = red + (size_t)maxblk * 5 * s;
+    if (launch_sums(b->p[b->pcur] + o, b->div + o, b->mask[b->mcur] + o, b->W[s],
+                    b->H[s], rs, b->stream) != hipSuccess ||
+        launch_reduce(rs, residual_blocks(b->W[s], b->H[s]), 5, 0, sums + 8 * s, -1.0,
+                      nullptr, 0, 0, b->stream) != hipSuccess)
+      rc = bfail(NX_ERR_HIP, "batch residual launch failed");
+  }
+  std::vector<double> h(8 * b->S);
+  if (rc == NX_OK &&
+      (hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                      b->stream) != hipSuccess ||
+       hipStreamSynchronize(b->stream) != hipSuccess))
+    rc = bfail(NX_ERR_HIP, "batch residual copy failed");
+  (void)hipFree(red);
+  (void)hipFree(sums);
+  if (rc != NX_OK) return rc;
+  for (int s = 0; s < b->S; ++s)
+    for (int j = 0; j < 6; ++j) out[6 * s + j] = h[8 * s + j];
+  return NX_OK;
+}
+
 int nx_batch_synchronize(nx_batch* b) {
   if (!b) return bfail(NX_ERR_ARG, "null batch");
   NXB_HIP(hipStreamSynchronize(b->stream));

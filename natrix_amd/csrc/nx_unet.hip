@@ -1,0 +1,497 @@
+// U-Net building blocks for the learned pressure guess (gfx950).
+//
+// Activations are NHWC bfloat16, 3x3 conv weights KRSC bfloat16
+// (w[cout][tap][cin], tap = 3*ky + kx), accumulation float32 on the matrix
+// cores with v_mfma_f32_32x32x16_bf16:
+//   * forward: implicit GEMM, M = pixels, N = output channels,
+//     K = 9 * input channels.  A lane's A fragment (8 consecutive K = 8
+//     channels of one tap of one pixel) and B fragment (8 input channels of
+//     one output channel) are each one contiguous 16-byte load, so the
+//     operands stream from the caches straight into registers.  Bias and
+//     ReLU are fused into the epilogue.
+//   * input gradient: the same kernel on the output gradient with the
+//     weights flipped and transposed (w'[cin][8-tap][cout]).
+//   * weight gradient: GEMM over the pixel dimension (K = pixels); 64-pixel
+//     chunks of the output gradient and the tap-shifted input are staged in
+//     LDS transposed to [channel][pixel] so each lane again reads one
+//     16-byte fragment; the pixel range is split across blocks and the
+//     float32 partials are summed in a fixed order.
+// The MFMA operand layout used (32x32x16): lane l holds row (l mod 32) of A
+// and column (l mod 32) of B, with K elements 8*(l/32) .. 8*(l/32)+7; the
+// accumulator's 16 registers of lane l hold column (l mod 32), rows
+// 8*(r/4) + 4*(l/32) + (r mod 4).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/natrix.h"
+#include "nx_common.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ bf16x8 ld_frag(const uint16_t* p) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ bf16x8 zero_frag() {
+  return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+}
+
+__device__ __forceinline__ float bf(uint16_t h) { return nx::bf16_to_f32(h); }
+__device__ __forceinline__ uint16_t tobf(float f) { return (uint16_t)nx::f32_to_bf16(f); }
+
+// ---------------------------------------------------------------- forward
+// grid: (ceil(M / (4*32*MT)), Cout / (32*NT)); block 256 = 4 waves along M
+template <int MT, int NT>
+__global__ void __launch_bounds__(256)
+    k_conv3x3(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+              const float* __restrict__ bias, uint16_t* __restrict__ y, int N,
+              int H, int W, int Cin, int Cout, int relu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 31, kb = lane >> 5;
+  const long long M = (long long)N * H * W;
+  const long long m_base = ((long long)blockIdx.x * 4 + wave) * 32 * MT;
+  const int n_base = blockIdx.y * 32 * NT;
+  int py[MT], px[MT];
+  long long pimg[MT];  // first pixel of the pixel's image
+  bool valid[MT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i) {
+    const long long m = m_base + 32 * i + r;
+    valid[i] = m < M;
+    const long long mm = valid[i] ? m : 0;
+    const int hw = H * W;
+    const long long img = mm / hw;
+    const int rem = (int)(mm - img * hw);
+    py[i] = rem / W;
+    px[i] = rem - py[i] * W;
+    pimg[i] = img * hw;
+  }
+  f32x16 acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+
+  const int K = 9 * Cin;
+  if ((Cin & 15) == 0) {
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      long long src[MT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) {
+        const int yy = py[i] + dy, xx = px[i] + dx;
+        src[i] = (valid[i] && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                     ? (pimg[i] + (long long)yy * W + xx) * Cin
+                     : -1;
+      }
+      for (int c0 = 0; c0 < Cin; c0 += 16) {
+        const int ci = c0 + kb * 8;
+        bf16x8 a[MT], b[NT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) a[i] = src[i] >= 0 ? ld_frag(x + src[i] + ci) : zero_frag();
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          b[j] = ld_frag(w + ((long long)(n_base + 32 * j + r) * 9 + tap) * Cin + ci);
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  } else {
+    // Cin a multiple of 8: one 8-wide fragment never crosses a tap
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      const int kk = k0 + kb * 8;
+      const int tap = kk / Cin, ci = kk - tap * Cin;
+      bf16x8 a[MT], b[NT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) {
+        a[i] = zero_frag();
+        if (tap < 9) {
+          const int yy = py[i] + tap / 3 - 1, xx = px[i] + tap % 3 - 1;
+          if (valid[i] && yy >= 0 && yy < H && xx >= 0 && xx < W)
+            a[i] = ld_frag(x + (pimg[i] + (long long)yy * W + xx) * Cin + ci);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        b[j] = tap < 9 ? ld_frag(w + ((long long)(n_base + 32 * j + r) * 9 + tap) * Cin + ci)
+                       : zero_frag();
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int co = n_base + 32 * j + r;
+      const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const long long m = m_base + 32 * i + 8 * (e >> 2) + 4 * kb + (e & 3);
+        if (m < M) {
+          float v = acc[i][j][e] + bv;
+          if (relu) v = fmaxf(v, 0.0f);
+          y[m * Cout + co] = tobf(v);
+        }
+      }
+    }
+}
+
+// w[co][tap][ci] -> w'[ci][8 - tap][co]
+__global__ void k_flip_weights(const uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
+                               int Cin, int Cout) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= Cout * 9 * Cin) return;
+  const int ci = i % Cin;
+  const int tap = (i / Cin) % 9;
+  const int co = i / (9 * Cin);
+  wt[((long long)ci * 9 + (8 - tap)) * Cout + co] = w[i];
+}
+
+// ---------------------------------------------------------------- wgrad
+// dw[co][tap][ci] = sum_m dy[m][co] * x[m shifted by tap][ci]
+// grid: (Cout/64 * Cin/64, 9, splits); block 256 (2x2 waves of 32x32)
+constexpr int WG_CHUNK = 64;
+
+__global__ void __launch_bounds__(256)
+    k_conv3x3_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+                    float* __restrict__ part, int N, int H, int W, int Cin,
+                    int Cout, long long pix_per_split) {
+  __shared__ uint16_t sA[64][WG_CHUNK + 8];  // [co][pixel]
+  __shared__ uint16_t sB[64][WG_CHUNK + 8];  // [ci][pixel]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 31, kb = lane >> 5;
+  const int tiles_ci = (Cin + 63) / 64;
+  const int co0 = (blockIdx.x / tiles_ci) * 64;
+  const int ci0 = (blockIdx.x % tiles_ci) * 64;
+  const int tap = blockIdx.y;
+  const int dy_ = tap / 3 - 1, dx_ = tap % 3 - 1;
+  const long long M = (long long)N * H * W;
+  const long long p_begin = (long long)blockIdx.z * pix_per_split;
+  const long long p_end = p_begin + pix_per_split < M ? p_begin + pix_per_split : M;
+  const int wm = wave >> 1, wn = wave & 1;  // wave's 32x32 quadrant
+  f32x16 acc;
+  for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+  const int hw = H * W;
+  for (long long p0 = p_begin; p0 < p_end; p0 += WG_CHUNK) {
+    // stage: 64 pixels x 64 channels of dy and of the shifted x, 8
+    // channels (one 16-byte load) per thread and pass, zero beyond C
+    for (int it = 0; it < 2; ++it) {
+      const int idx = threadIdx.x + it * 256;
+      const int pl = idx >> 3;         // pixel within chunk (0..63)
+      const int cq = (idx & 7) * 8;    // channel offset (0..56)
+      const long long m = p0 + pl;
+      uint4 va = make_uint4(0, 0, 0, 0), vb = va;
+      if (m < p_end) {
+        if (co0 + cq < Cout)
+          va = *reinterpret_cast<const uint4*>(dy + m * Cout + co0 + cq);
+        const long long img = m / hw;
+        const int rem = (int)(m - img * hw);
+        const int yy = rem / W + dy_, xx = rem % W + dx_;
+        if (ci0 + cq < Cin && yy >= 0 && yy < H && xx >= 0 && xx < W)
+          vb = *reinterpret_cast<const uint4*>(
+              x + (img * hw + (long long)yy * W + xx) * Cin + ci0 + cq);
+      }
+      const uint16_t* ta = reinterpret_cast<const uint16_t*>(&va);
+      const uint16_t* tb = reinterpret_cast<const uint16_t*>(&vb);
+      for (int t = 0; t < 8; ++t) {
+        sA[cq + t][pl] = ta[t];
+        sB[cq + t][pl] = tb[t];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k0 = 0; k0 < WG_CHUNK; k0 += 16) {
+      const bf16x8 a = ld_frag(&sA[wm * 32 + r][k0 + kb * 8]);
+      const bf16x8 b = ld_frag(&sB[wn * 32 + r][k0 + kb * 8]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // partial[split][co][tap][ci]
+  const long long base = (long long)blockIdx.z * Cout * 9 * Cin;
+  for (int e = 0; e < 16; ++e) {
+    const int co = co0 + wm * 32 + 8 * (e >> 2) + 4 * kb + (e & 3);
+    const int ci = ci0 + wn * 32 + r;
+    if (co < Cout && ci < Cin) part[base + ((long long)co * 9 + tap) * Cin + ci] = acc[e];
+  }
+}
+
+// out[i] = sum_s part[s*n + i], fixed order
+__global__ void k_sum_splits(const float* __restrict__ part, float* __restrict__ out,
+                             long long n, int splits) {
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int k = 0; k < splits; ++k) s += part[(long long)k * n + i];
+  out[i] = s;
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void k_relu_bwd(const uint16_t* __restrict__ y, const uint16_t* __restrict__ dy,
+                           uint16_t* __restrict__ dz, long long n) {
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  dz[i] = bf(y[i]) > 0.0f ? dy[i] : (uint16_t)0;
+}
+
+// per-channel column sums of a (rows x C) bf16 matrix: partials per block
+__global__ void k_colsum(const uint16_t* __restrict__ a, float* __restrict__ part,
+                         long long rows, int C, long long rows_per_block) {
+  const long long r0 = blockIdx.x * rows_per_block;
+  const long long r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  for (int c = threadIdx.x; c < C; c += 256) {
+    float s = 0.0f;
+    for (long long r = r0; r < r1; ++r) s += bf(a[r * C + c]);
+    part[(long long)blockIdx.x * C + c] = s;
+  }
+}
+
+__global__ void k_avgpool2(const uint16_t* __restrict__ x, uint16_t* __restrict__ y,
+                           int N, int H, int W, int C) {  // H, W: input size
+  const int Ho = H / 2, Wo = W / 2;
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= (long long)N * Ho * Wo * C) return;
+  const int c = i % C;
+  const long long p = i / C;
+  const int xo = p % Wo, yo = (p / Wo) % Ho;
+  const long long n = p / ((long long)Wo * Ho);
+  const long long b = ((n * H + 2 * yo) * W + 2 * xo) * C + c;
+  const float s = ((bf(x[b]) + bf(x[b + C])) + bf(x[b + (long long)W * C])) +
+                  bf(x[b + (long long)W * C + C]);
+  y[i] = tobf(s * 0.25f);
+}
+
+__global__ void k_avgpool2_bwd(const uint16_t* __restrict__ dy, uint16_t* __restrict__ dx,
+                               int N, int H, int W, int C) {  // H, W: input size
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= (long long)N * H * W * C) return;
+  const int c = i % C;
+  const long long p = i / C;
+  const int xx = p % W, yy = (p / W) % H;
+  const long long n = p / ((long long)W * H);
+  dx[i] = tobf(bf(dy[((n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c]) * 0.25f);
+}
+
+__global__ void k_upsample2(const uint16_t* __restrict__ x, uint16_t* __restrict__ y,
+                            int N, int H, int W, int C) {  // H, W: output size
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= (long long)N * H * W * C) return;
+  const int c = i % C;
+  const long long p = i / C;
+  const int xx = p % W, yy = (p / W) % H;
+  const long long n = p / ((long long)W * H);
+  y[i] = x[((n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c];
+}
+
+__global__ void k_upsample2_bwd(const uint16_t* __restrict__ dy, uint16_t* __restrict__ dx,
+                                int N, int H, int W, int C) {  // H, W: output size
+  const int Hi = H / 2, Wi = W / 2;
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= (long long)N * Hi * Wi * C) return;
+  const int c = i % C;
+  const long long p = i / C;
+  const int xi = p % Wi, yi = (p / Wi) % Hi;
+  const long long n = p / ((long long)Wi * Hi);
+  const long long b = ((n * H + 2 * yi) * W + 2 * xi) * C + c;
+  const float s = ((bf(dy[b]) + bf(dy[b + C])) + bf(dy[b + (long long)W * C])) +
+                  bf(dy[b + (long long)W * C + C]);
+  dx[i] = tobf(s);
+}
+
+// y[p][0:Ca] = a[p], y[p][Ca:Ca+Cb] = b[p]; backward with split = 1
+__global__ void k_concat(uint16_t* __restrict__ y, uint16_t* __restrict__ a,
+                         uint16_t* __restrict__ b, long long P, int Ca, int Cb,
+                         int split) {
+  const int C = Ca + Cb;
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= P * C) return;
+  const int c = i % C;
+  const long long p = i / C;
+  if (!split) {
+    y[i] = c < Ca ? a[p * Ca + c] : b[p * Cb + (c - Ca)];
+  } else {
+    if (c < Ca) a[p * Ca + c] = y[i]; else b[p * Cb + (c - Ca)] = y[i];
+  }
+}
+
+// 1x1 head: out[p] = sum_c x[p][c] w[c] + bias (float32 out)
+__global__ void k_head(const uint16_t* __restrict__ x, const float* __restrict__ w,
+                       const float* __restrict__ bias, float* __restrict__ out,
+                       long long P, int C) {
+  const long long p = blockIdx.x * 256ll + threadIdx.x;
+  if (p >= P) return;
+  float s = 0.0f;
+  for (int c = 0; c < C; ++c) s += bf(x[p * C + c]) * w[c];
+  out[p] = s + bias[0];
+}
+
+// head backward: dx[p][c] = dout[p] w[c]; per-block partials of
+// dw[c] = sum_p dout[p] x[p][c] and db = sum_p dout[p] in part[blk][C+1]
+__global__ void k_head_bwd(const uint16_t* __restrict__ x, const float* __restrict__ w,
+                           const float* __restrict__ dout, uint16_t* __restrict__ dx,
+                           float* __restrict__ part, long long P, int C,
+                           long long rows_per_block) {
+  const long long r0 = blockIdx.x * rows_per_block;
+  const long long r1 = r0 + rows_per_block < P ? r0 + rows_per_block : P;
+  for (long long i = r0 * C + threadIdx.x; i < r1 * C; i += 256) {
+    const long long p = i / C;
+    dx[i] = tobf(dout[p] * w[i % C]);
+  }
+  for (int c = threadIdx.x; c <= C; c += 256) {
+    float s = 0.0f;
+    for (long long p = r0; p < r1; ++p) s += c < C ? dout[p] * bf(x[p * C + c]) : dout[p];
+    part[(long long)blockIdx.x * (C + 1) + c] = s;
+  }
+}
+
+inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+#define NX_U(stream) ((hipStream_t)(stream))
+#define NX_UNET_RET()                                         \
+  do {                                                        \
+    hipError_t e_ = hipGetLastError();                        \
+    return e_ == hipSuccess ? NX_OK                           \
+                            : nx::set_error(NX_ERR_HIP, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace nx {
+int set_error(int code, const std::string& msg);
+}
+
+extern "C" {
+
+int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int n,
+                   int h, int wd, int cin, int cout, int relu, void* stream) {
+  if (!x || !w || !y || n < 1 || h < 1 || wd < 1 || cin < 8 || cin % 8 || cout % 32)
+    return nx::set_error(NX_ERR_ARG, "conv3x3: channels must be cin % 8 == 0, cout % 32 == 0");
+  const long long M = (long long)n * h * wd;
+  const hipStream_t s = NX_U(stream);
+  if (cout % 64 == 0) {
+    dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64);
+    k_conv3x3<2, 2><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)w, bias,
+                                          (uint16_t*)y, n, h, wd, cin, cout, relu);
+  } else {
+    dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 32);
+    k_conv3x3<2, 1><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)w, bias,
+                                          (uint16_t*)y, n, h, wd, cin, cout, relu);
+  }
+  NX_UNET_RET();
+}
+
+int nx_conv3x3_dgrad(const void* dy, const void* w, void* dx, void* wt, int n, int h,
+                     int wd, int cin, int cout, void* stream) {
+  if (!dy || !w || !dx || !wt || cin % 32 || cout % 16)
+    return nx::set_error(NX_ERR_ARG, "conv3x3 dgrad: cin % 32 == 0 and cout % 16 == 0");
+  const hipStream_t s = NX_U(stream);
+  k_flip_weights<<<blocks_for((long long)cout * 9 * cin), 256, 0, s>>>(
+      (const uint16_t*)w, (uint16_t*)wt, cin, cout);
+  if (hipGetLastError() != hipSuccess) return nx::set_error(NX_ERR_HIP, "flip failed");
+  return nx_conv3x3_fwd(dy, wt, nullptr, dx, n, h, wd, cout, cin, 0, stream);
+}
+
+// dw: float32 [cout][9][cin]; work: float32, splits * cout * 9 * cin
+int nx_conv3x3_wgrad(const void* x, const void* dy, float* dw, float* work, int splits,
+                     int n, int h, int wd, int cin, int cout, void* stream) {
+  if (!x || !dy || !dw || !work || splits < 1 || cin % 8 || cout % 8)
+    return nx::set_error(NX_ERR_ARG, "conv3x3 wgrad: cin % 8 == 0 and cout % 8 == 0");
+  const long long M = (long long)n * h * wd;
+  long long per = (M + splits - 1) / splits;
+  per = (per + WG_CHUNK - 1) / WG_CHUNK * WG_CHUNK;
+  const int used = (int)((M + per - 1) / per);
+  const hipStream_t s = NX_U(stream);
+  dim3 grid(((cout + 63) / 64) * ((cin + 63) / 64), 9, used);
+  k_conv3x3_wgrad<<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)dy, work, n,
+                                       h, wd, cin, cout, per);
+  if (hipGetLastError() != hipSuccess) return nx::set_error(NX_ERR_HIP, "wgrad failed");
+  const long long nw = (long long)cout * 9 * cin;
+  k_sum_splits<<<blocks_for(nw), 256, 0, s>>>(work, dw, nw, used);
+  NX_UNET_RET();
+}
+
+int nx_relu_bwd(const void* y, const void* dy, void* dz, long long count, void* stream) {
+  k_relu_bwd<<<blocks_for(count), 256, 0, NX_U(stream)>>>(
+      (const uint16_t*)y, (const uint16_t*)dy, (uint16_t*)dz, count);
+  NX_UNET_RET();
+}
+
+// column sums of a bf16 (rows x c) matrix into out[c] (float32);
+// work: float32 of blocks * c, blocks = min(1024, rows)
+int nx_colsum(const void* a, float* out, float* work, long long rows, int c,
+              void* stream) {
+  const long long blocks = rows < 1024 ? rows : 1024;
+  const long long per = (rows + blocks - 1) / blocks;
+  const hipStream_t s = NX_U(stream);
+  k_colsum<<<(unsigned)blocks, 256, 0, s>>>((const uint16_t*)a, work, rows, c, per);
+  if (hipGetLastError() != hipSuccess) return nx::set_error(NX_ERR_HIP, "colsum failed");
+  k_sum_splits<<<blocks_for(c), 256, 0, s>>>(work, out, c, (int)blocks);
+  NX_UNET_RET();
+}
+
+int nx_avgpool2(const void* x, void* y, int n, int h, int wd, int c, int backward,
+                void* stream) {
+  const long long cnt = (long long)n * (backward ? h * wd : (h / 2) * (wd / 2)) * c;
+  if (backward)
+    k_avgpool2_bwd<<<blocks_for(cnt), 256, 0, NX_U(stream)>>>(
+        (const uint16_t*)x, (uint16_t*)y, n, h, wd, c);
+  else
+    k_avgpool2<<<blocks_for(cnt), 256, 0, NX_U(stream)>>>(
+        (const uint16_t*)x, (uint16_t*)y, n, h, wd, c);
+  NX_UNET_RET();
+}
+
+int nx_upsample2(const void* x, void* y, int n, int h, int wd, int c, int backward,
+                 void* stream) {
+  const long long cnt = (long long)n * (backward ? (h / 2) * (wd / 2) : h * wd) * c;
+  if (backward)
+    k_upsample2_bwd<<<blocks_for(cnt), 256, 0, NX_U(stream)>>>(
+        (const uint16_t*)x, (uint16_t*)y, n, h, wd, c);
+  else
+    k_upsample2<<<blocks_for(cnt), 256, 0, NX_U(stream)>>>(
+        (const uint16_t*)x, (uint16_t*)y, n, h, wd, c);
+  NX_UNET_RET();
+}
+
+int nx_concat(void* y, void* a, void* b, long long pixels, int ca, int cb, int split,
+              void* stream) {
+  k_concat<<<blocks_for(pixels * (ca + cb)), 256, 0, NX_U(stream)>>>(
+      (uint16_t*)y, (uint16_t*)a, (uint16_t*)b, pixels, ca, cb, split);
+  NX_UNET_RET();
+}
+
+int nx_head_fwd(const void* x, const float* w, const float* bias, float* out,
+                long long pixels, int c, void* stream) {
+  k_head<<<blocks_for(pixels), 256, 0, NX_U(stream)>>>((const uint16_t*)x, w, bias, out,
+                                                      pixels, c);
+  NX_UNET_RET();
+}
+
+// dw[c] (c floats) and db (1 float) into dwb[0..c]; work: 1024 * (c+1) floats
+int nx_head_bwd(const void* x, const float* w, const float* dout, void* dx, float* dwb,
+                float* work, long long pixels, int c, void* stream) {
+  const long long blocks = pixels < 1024 ? pixels : 1024;
+  const long long per = (pixels + blocks - 1) / blocks;
+  const hipStream_t s = NX_U(stream);
+  k_head_bwd<<<(unsigned)blocks, 256, 0, s>>>((const uint16_t*)x, w, dout, (uint16_t*)dx,
+                                              work, pixels, c, per);
+  if (hipGetLastError() != hipSuccess) return nx::set_error(NX_ERR_HIP, "head bwd failed");
+  k_sum_splits<<<blocks_for(c + 1), 256, 0, s>>>(work, dwb, c + 1, (int)blocks);
+  NX_UNET_RET();
+}
+
+}  // extern "C"

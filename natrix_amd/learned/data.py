@@ -1,0 +1,81 @@
+"""Training pairs for the learned pressure guess, produced by the project's
+own batched float32 Jacobi solver.
+
+Each sample is a coarse Poisson problem (the grid the U-Net runs on): a
+right-hand side b, a solid mask and the pressure p that Jacobi converged to.
+The right-hand sides are smooth random fields (sums of Gaussian blobs of
+random width and sign plus low Fourier modes), like the restricted
+divergence and residual fields the solver hands the network; masks are
+disks (the cylinder scenes), rectangles, or empty.  b is centred on its
+fluid mean (the operator's null space) before solving.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ..batch import BatchedScenes, random_rect_obstacles
+
+
+def random_problem(rng, n):
+    X = np.arange(n, dtype=np.float64)[None, :]
+    Y = np.arange(n, dtype=np.float64)[:, None]
+    kind = rng.random()
+    solid = np.zeros((n, n), bool)
+    if kind < 0.5:
+        c = rng.uniform(0.2, 0.8, 2) * n
+        r = rng.uniform(0.05, 0.2) * n
+        solid = (X - c[0]) ** 2 + (Y - c[1]) ** 2 <= r * r
+    elif kind < 0.75:
+        solid = random_rect_obstacles(rng, n, n, rng.uniform(0.0, 0.15))
+    b = np.zeros((n, n))
+    for _ in range(int(rng.integers(2, 13))):
+        c = rng.uniform(0, n, 2)
+        s = np.exp(rng.uniform(np.log(1.5), np.log(n / 4)))
+        b += rng.normal() * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2) / (2 * s * s))
+    if rng.random() < 0.5:
+        for _ in range(3):
+            kx, ky = rng.integers(1, 6, 2)
+            ph = rng.uniform(0, 2 * np.pi, 2)
+            b += 0.3 * rng.normal() * np.cos(np.pi * kx * (X + 0.5) / n + ph[0]) * \
+                np.cos(np.pi * ky * (Y + 0.5) / n + ph[1])
+    b[solid] = 0.0
+    fluid = ~solid
+    b[fluid] -= b[fluid].mean()
+    b /= max(np.sqrt((b[fluid] ** 2).mean()), 1e-12)
+    return b.astype(np.float32), solid
+
+
+def make_dataset(count, n=128, seed=500000, tol=1e-6, max_sweeps=400000,
+                 chunk_sweeps=20000, per_batch=1024, tile=None, log=print):
+    """Returns (b [count,n,n] f32, solid [count,n,n] bool, p [count,n,n] f32,
+    residual [count]) with p from Jacobi from zero on the GPU."""
+    bs, ss, ps, rs = [], [], [], []
+    done = 0
+    while done < count:
+        m = min(per_batch, count - done)
+        rng = [np.random.default_rng(seed + done + i) for i in range(m)]
+        probs = [random_problem(r, n) for r in rng]
+        batch = BatchedScenes([n] * m)
+        if tile is not None:
+            batch.set_params(jacobi_tile=tile)
+        k, rows, waves, cols = batch.jacobi_tile
+        for i, (b, solid) in enumerate(probs):
+            batch.upload(i, "divergence", b)
+            batch.upload(i, "pressure", np.zeros_like(b))
+            batch.set_obstacle_map(i, solid)
+        sweeps = 0
+        res = batch.residuals()
+        while sweeps < max_sweeps and res.max() > tol:
+            batch.jacobi_blocked(chunk_sweeps, k, rows, waves, cols)
+            sweeps += chunk_sweeps
+            res = batch.residuals()
+            log(f"  data batch {done // per_batch}: {sweeps} sweeps, residual "
+                f"median {np.median(res):.2e} max {res.max():.2e}")
+        for i, (b, solid) in enumerate(probs):
+            bs.append(b)
+            ss.append(solid)
+            ps.append(batch.download(i, "pressure"))
+        rs.append(res)
+        batch.destroy()
+        done += m
+    return np.stack(bs), np.stack(ss), np.stack(ps), np.concatenate(rs)

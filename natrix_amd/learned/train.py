@@ -1,0 +1,137 @@
+"""Train the U-Net pressure guess on Jacobi-generated pairs.
+
+    python -m natrix_amd.learned.train --samples 20000 --size 128 --epochs 6 \\
+        --out natrix_amd/learned/unet_weights.pt
+
+Loss: mean squared error of the mean-free prediction against the mean-free
+Jacobi pressure over fluid cells (the operator fixes p only up to a
+constant), relative to the target's mean square.  The problem is invariant
+under the 8 symmetries of the square, so every batch is randomly flipped and
+transposed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+from .unet import UNet, pack_input
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(HERE, "unet_weights.pt")
+
+
+def masked_loss(pred, target, fluid):
+    f = fluid.float()
+    cnt = f.sum(dim=(1, 2)).clamp_min(1.0)
+    pm = (pred * f).sum(dim=(1, 2)) / cnt
+    tm = (target * f).sum(dim=(1, 2)) / cnt
+    d = ((pred - pm[:, None, None]) - (target - tm[:, None, None])) * f
+    t = (target - tm[:, None, None]) * f
+    return ((d * d).sum(dim=(1, 2)) / (t * t).sum(dim=(1, 2)).clamp_min(1e-12)).mean()
+
+
+def augment(x, *ys, gen):
+    k = int(torch.randint(0, 8, (1,), generator=gen))
+    def t(a, nd):
+        # spatial dims are 1 and 2 for [N,H,W] and [N,H,W,C]
+        if k & 1:
+            a = a.flip(1)
+        if k & 2:
+            a = a.flip(2)
+        if k & 4:
+            a = a.transpose(1, 2)
+        return a.contiguous()
+    return (t(x, 4),) + tuple(t(y, 3) for y in ys)
+
+
+def train(b, solid, p, epochs=6, batch=32, lr=1e-3, seed=0, log=print, val_frac=0.05):
+    dev = "cuda"
+    torch.manual_seed(seed)
+    gen = torch.Generator().manual_seed(seed)
+    n = len(b)
+    nval = max(1, int(n * val_frac))
+    B = torch.from_numpy(b).to(dev)
+    S = torch.from_numpy(solid).to(dev)
+    P = torch.from_numpy(p).to(dev)
+    net = UNet().to(dev)
+    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    steps = epochs * ((n - nval) // batch)
+    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=max(1, steps))
+    hist = []
+    t0 = time.time()
+    for ep in range(epochs):
+        perm = torch.randperm(n - nval, generator=gen) + nval
+        tot = 0.0
+        nb = 0
+        for i in range(0, n - nval - batch + 1, batch):
+            idx = perm[i:i + batch].to(dev)
+            x = pack_input(B[idx], S[idx])
+            x, tgt, fl = augment(x, P[idx], ~S[idx], gen=gen)
+            out = net(x)
+            loss = masked_loss(out, tgt, fl)
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+            sched.step()
+            tot += float(loss)
+            nb += 1
+        with torch.no_grad():
+            vl = []
+            for i in range(0, nval, batch):
+                sl = slice(i, min(nval, i + batch))
+                vl.append(float(masked_loss(net(pack_input(B[sl], S[sl])), P[sl], ~S[sl])))
+        rec = {"epoch": ep, "train_loss": tot / max(nb, 1), "val_loss": float(np.mean(vl)),
+               "seconds": round(time.time() - t0, 1)}
+        hist.append(rec)
+        log(json.dumps(rec))
+    return net, hist
+
+
+def save(net, path, meta):
+    sd = {k: v.detach().to(torch.bfloat16).cpu() for k, v in net.state_dict().items()}
+    torch.save({"state": sd, "meta": meta}, path)
+
+
+def load(path=DEFAULT_WEIGHTS, device="cuda"):
+    blob = torch.load(path, map_location="cpu", weights_only=True)
+    net = UNet()
+    net.load_state_dict({k: v.float() for k, v in blob["state"].items()})
+    return net.to(device).eval(), blob.get("meta", {})
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--samples", type=int, default=20000)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-sweeps", type=int, default=400000)
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    a = ap.parse_args(argv)
+    from .data import make_dataset
+    log = lambda m: print(m, flush=True)
+    t0 = time.time()
+    b, solid, p, res = make_dataset(a.samples, a.size, tol=a.tol,
+                                    max_sweeps=a.max_sweeps, log=log)
+    log(json.dumps({"dataset": a.samples, "size": a.size, "seconds": round(time.time() - t0, 1),
+                    "residual_median": float(np.median(res)), "residual_max": float(res.max()),
+                    "frac_at_tol": float((res <= a.tol).mean())}))
+    net, hist = train(b, solid, p, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    meta = {"samples": a.samples, "size": a.size, "epochs": a.epochs,
+            "data_residual_median": float(np.median(res)),
+            "data_residual_max": float(res.max()), "history": hist,
+            "params": net.n_params()}
+    save(net, a.out, meta)
+    log(json.dumps({"saved": a.out, "params": net.n_params()}))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
