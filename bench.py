@@ -265,6 +265,85 @@ def bench_mixed(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """Steps/s at rel. residual <= tol: learned guess + Jacobi vs plain
+    fused Jacobi from zero, on the cylinder/inflow scene after a warm-up of
+    fixed-iteration steps that develops the flow."""
+    import torch
+    from natrix_amd import FluidSimulator
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import DEFAULT_WEIGHTS, load
+    N = args.size
+    net, meta = load(args.weights or DEFAULT_WEIGHTS)
+    guess = LearnedGuess(net, coarse=meta.get("size", 128))
+    solid, centre = make_scene(N)
+
+    def fresh():
+        sim = FluidSimulator(N, N)
+        sim.vorticity = 0.3
+        sim.viscosity = 0.0
+        sim.iterations = 80
+        sim.bc_mode = "inflow"
+        sim.set_obstacle_map(solid)
+        for _ in range(args.warmup):
+            sim.update(1 / 60)
+        sim.synchronize()
+        return sim
+
+    res = {}
+    # learned + Jacobi
+    sim = fresh()
+    sweeps, conv, t_l = [], [], 0.0
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solve_step(sim, guess, 1 / 60, tol=args.tol)
+        sim.synchronize()
+        t_l += time.perf_counter() - t0
+        sweeps.append(st["sweeps"])
+        conv.append(st["converged"])
+        log(f"learned step {i}: {st}")
+    sim.destroy()
+    res["learned"] = {"steps_per_s": round(args.steps / t_l, 4),
+                      "ms_per_step": round(1e3 * t_l / args.steps, 3),
+                      "sweeps_per_step": float(np.mean(sweeps)),
+                      "all_converged": bool(all(conv))}
+    # plain Jacobi (same refinement, no learned corrections), same criterion
+    # (fewer steps: it is slow; the sweep cap makes its rate a bound)
+    sim = fresh()
+    psteps = max(1, min(args.steps, args.plain_steps))
+    sweeps, conv, t_p, resid = [], [], 0.0, []
+    for i in range(psteps):
+        t0 = time.perf_counter()
+        st = solve_step(sim, None, 1 / 60, tol=args.tol, max_inner_sweeps=args.plain_cap)
+        sim.synchronize()
+        t_p += time.perf_counter() - t0
+        sweeps.append(st["sweeps"])
+        conv.append(st["converged"])
+        resid.append(st["residual"])
+        log(f"plain step {i}: {st}")
+    sim.destroy()
+    res["plain"] = {"steps_per_s": round(psteps / t_p, 5),
+                    "ms_per_step": round(1e3 * t_p / psteps, 3),
+                    "sweeps_per_step": float(np.mean(sweeps)),
+                    "all_converged": bool(all(conv)), "final_residuals": resid,
+                    "sweep_cap_per_correction_solve": args.plain_cap}
+    speedup = res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"]
+    out = {"metric": METRIC, "value": res["learned"]["steps_per_s"], "unit": "steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res["learned"]["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 (U-Net bf16)",
+           "data": "synthetic",
+           "config": {"workload": f"fluid step {N}x{N} f32, cylinder r=0.1N, left inflow, "
+                                  f"vorticity 0.3, pressure to rel. residual <= {args.tol:g}",
+                      "grid": N, "solver": "learned guess + Jacobi vs plain Jacobi"},
+           "learned": res["learned"], "plain": res["plain"],
+           "speedup_learned_vs_plain": round(speedup, 2),
+           "plain_lower_bound": not res["plain"]["all_converged"],
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,11 +360,15 @@ def main():
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batch", "mixed"],
+    ap.add_argument("--workload", choices=["single", "batch", "mixed", "learned"],
                     default="single",
                     help="batch: 64 uneven scenes in one launch per stage; "
                          "mixed: 4096^2 with bf16/f16 storage")
     ap.add_argument("--storage", choices=["bf16", "f16"], default="bf16")
+    ap.add_argument("--weights", default=None, help="U-Net weights (learned workload)")
+    ap.add_argument("--plain-cap", type=int, default=4_000_000,
+                    help="sweep cap per correction solve of the plain path (learned workload)")
+    ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the copy-bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -314,6 +397,8 @@ def main():
         return bench_batch(args, copy_bandwidth)
     if args.workload == "mixed":
         return bench_mixed(args)
+    if args.workload == "learned":
+        return bench_learned(args)
 
     N = args.size
     solid, centre = make_scene(N)

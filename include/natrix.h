@@ -150,6 +150,19 @@ int nx_restrict_residual(nx_sim* sim, int factor, float* coarse_rhs,
 /* pressure += scale * bilinear interpolation of coarse[Hc*Wc] (device) on
  * fluid cells */
 int nx_prolong_add(nx_sim* sim, int factor, const float* coarse, float scale);
+/* iterative refinement inside a step (for tolerances below the float32
+ * round-off floor of a large pressure): the pressure becomes a float32 pair
+ * ph + pl, initially 0.  nx_refine_residual evaluates r = b - A(ph + pl) in
+ * FP64, returns its relative FP64 residual, and makes (r, p1 of A e = r)
+ * the current right-hand side and pressure, so nx_solve / the coarse
+ * transfers then work on the correction e; nx_refine_accumulate adds the
+ * current pressure (e) to the pair with an error-free two-sum;
+ * nx_refine_end (or nx_step_finish) sets the pressure to ph + pl and
+ * restores the right-hand side. */
+int nx_refine_begin(nx_sim* sim);
+int nx_refine_residual(nx_sim* sim, double* rel);
+int nx_refine_accumulate(nx_sim* sim);
+int nx_refine_end(nx_sim* sim);
 int nx_step_finish(nx_sim* sim);
 
 /* Capture the fixed-iteration step into a HIP graph and replay it on later
@@ -270,6 +283,16 @@ int nx_head_fwd(const void* x, const float* w, const float* bias, float* out,
                 long long pixels, int c, void* stream);
 int nx_head_bwd(const void* x, const float* w, const float* dout, void* dx, float* dwb,
                 float* work, long long pixels, int c, void* stream);
+
+/* ---- FP64 oracle solve on the GPU (conjugate gradients on the same
+ *      operator, fluid cells, right-hand side centred on its fluid mean) ---- */
+/* b: float32 device field, mask: uint8 device field, x: W*H doubles (device);
+ * out[0] = true relative residual of x, out[1] = iterations */
+int nx_fp64_solve(const float* b, const uint8_t* mask, int W, int H, double* x,
+                  double tol, int max_iter, double* out, void* stream);
+/* relative L2 error of the mean-free float32 p against the mean-free x */
+int nx_fp64_compare(const float* p, const double* x, const uint8_t* mask, int W, int H,
+                    double* rel, void* stream);
 
 /* attainable bandwidth: a plain device copy of `bytes` (multiple of 16),
  * `iters` times; *gbps = (read + write bytes) / time */

@@ -136,6 +136,10 @@ SIGNATURES = {
     "nx_solve": [_P, _D, _I, C.POINTER(Stats)],
     "nx_restrict_residual": [_P, _I, _P, _P],
     "nx_prolong_add": [_P, _I, _P, _F],
+    "nx_refine_begin": [_P],
+    "nx_refine_residual": [_P, C.POINTER(_D)],
+    "nx_refine_accumulate": [_P],
+    "nx_refine_end": [_P],
     "nx_step_finish": [_P],
     "nx_conv3x3_fwd": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
     "nx_conv3x3_dgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
@@ -147,6 +151,8 @@ SIGNATURES = {
     "nx_concat": [_P, _P, _P, C.c_longlong, _I, _I, _I, _P],
     "nx_head_fwd": [_P, _P, _P, _P, C.c_longlong, _I, _P],
     "nx_head_bwd": [_P, _P, _P, _P, _P, _P, C.c_longlong, _I, _P],
+    "nx_fp64_solve": [_P, _P, _I, _I, _P, _D, _I, C.POINTER(_D), _P],
+    "nx_fp64_compare": [_P, _P, _P, _I, _I, C.POINTER(_D), _P],
     "nx_copy_bandwidth": [_SZ, _I, C.POINTER(_D)],
     "nx_last_error": [],
 }

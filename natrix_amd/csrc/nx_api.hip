@@ -79,6 +79,11 @@ struct nx_sim {
   std::vector<TimedPair> timed;
   std::vector<hipEvent_t> free_events;
 
+  // iterative refinement: pressure = ph + pl (float32 pair), original rhs
+  float* ph = nullptr;
+  float* pl = nullptr;
+  float* b0 = nullptr;
+  bool refining = false;
   // step phases (nx_step_begin .. nx_step_finish)
   bool in_step = false;
   float phase_dt = 0.0f;
@@ -554,6 +559,76 @@ __global__ void __launch_bounds__(256)
   p[pos] = p[pos] + scale * (a * (1.0f - ty) + c * ty);
 }
 
+// ---- iterative refinement of the pressure ---------------------------------
+// The pressure is held as an unevaluated float32 sum ph + pl.  The outer
+// residual r = b0 - A (ph + pl) is evaluated in FP64 (so its accuracy is not
+// limited by float32 rounding of a large pressure), stored as the float32
+// right-hand side of the correction equation A e = r, and reduced in FP64
+// for the relative residual.  The correction e is then solved in float32
+// and added to the pair with an error-free two-sum.
+__global__ void __launch_bounds__(256)
+    k_refine_residual(const float* __restrict__ ph, const float* __restrict__ pl,
+                      const float* __restrict__ b0, const uint8_t* __restrict__ mask,
+                      float* __restrict__ r_out, float* __restrict__ p1,
+                      double* __restrict__ red, int W, int H) {
+  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  double v[5] = {0, 0, 0, 0, 0};
+  if (x < W && y < H) {
+    const int pos = y * W + x;
+    const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
+    const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
+    const double pc = (double)ph[pos] + (double)pl[pos];
+    const double x1 = mask[iL] ? pc : (double)ph[iL] + (double)pl[iL];
+    const double x2 = mask[iR] ? pc : (double)ph[iR] + (double)pl[iR];
+    const double y1 = mask[iB] ? pc : (double)ph[iB] + (double)pl[iB];
+    const double y2 = mask[iT] ? pc : (double)ph[iT] + (double)pl[iT];
+    const double bd = (double)b0[pos];
+    const double rr = bd - ((((x1 + x2) + y1) + y2) - 4.0 * pc);
+    const float rf = mask[pos] ? 0.0f : (float)rr;
+    r_out[pos] = rf;
+    p1[pos] = (0.0f - rf) * 0.25f;
+    if (mask[pos] == 0) {
+      v[0] = rr;
+      v[1] = rr * rr;
+      v[2] = bd;
+      v[3] = bd * bd;
+      v[4] = 1.0;
+    }
+  }
+  __shared__ double part[4][5];
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  for (int j = 0; j < 5; ++j)
+    for (int off = 32; off > 0; off >>= 1) v[j] += __shfl_xor(v[j], off, 64);
+  if (lane == 0)
+    for (int j = 0; j < 5; ++j) part[wave][j] = v[j];
+  __syncthreads();
+  if (lane == 0 && wave == 0) {
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    for (int j = 0; j < 5; ++j)
+      red[5 * bid + j] = ((part[0][j] + part[1][j]) + part[2][j]) + part[3][j];
+  }
+}
+
+// (ph, pl) += e with Knuth's two-sum: s = ph + e, err = (ph - (s - bb)) +
+// (e - bb) with bb = s - ph; ph = s, pl += err
+__global__ void k_refine_accumulate(float* __restrict__ ph, float* __restrict__ pl,
+                                    const float* __restrict__ e, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float a = ph[i], b = e[i];
+  const float sum = a + b;
+  const float bb = sum - a;
+  const float err = (a - (sum - bb)) + (b - bb);
+  ph[i] = sum;
+  pl[i] = pl[i] + err;
+}
+
+__global__ void k_refine_combine(const float* __restrict__ ph, const float* __restrict__ pl,
+                                 float* __restrict__ p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = ph[i] + pl[i];
+}
+
 // ------------------------------------------------------------- C-ABI
 extern "C" {
 
@@ -668,6 +743,9 @@ int nx_destroy(nx_sim* s) {
   }
   (void)hipFree(s->div);
   (void)hipFree(s->p32);
+  (void)hipFree(s->ph);
+  (void)hipFree(s->pl);
+  (void)hipFree(s->b0);
   (void)hipFree(s->curl);
   (void)hipFree(s->red5);
   (void)hipFree(s->red2);
@@ -952,9 +1030,69 @@ int nx_prolong_add(nx_sim* s, int factor, const float* coarse, float scale) {
   return NX_OK;
 }
 
+int nx_refine_begin(nx_sim* s) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (!s->in_step) return fail(NX_ERR_STATE, "refinement outside a step");
+  const size_t n = (size_t)s->W * s->H;
+  if (!s->ph) {
+    NX_HIP(hipMalloc(&s->ph, n * sizeof(float)));
+    NX_HIP(hipMalloc(&s->pl, n * sizeof(float)));
+    NX_HIP(hipMalloc(&s->b0, n * sizeof(float)));
+  }
+  NX_HIP(hipMemcpyAsync(s->b0, s->div, n * sizeof(float), hipMemcpyDeviceToDevice,
+                        s->stream));
+  NX_HIP(launch_fill(s->ph, 0.0f, n, s->stream));
+  NX_HIP(launch_fill(s->pl, 0.0f, n, s->stream));
+  s->refining = true;
+  return NX_OK;
+}
+
+int nx_refine_residual(nx_sim* s, double* rel) {
+  if (!s || !rel) return fail(NX_ERR_ARG, "null argument");
+  if (!s->refining) return fail(NX_ERR_STATE, "nx_refine_begin first");
+  const dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
+  k_refine_residual<<<grid, dim3(64, 4), 0, s->stream>>>(
+      s->ph, s->pl, s->b0, s->mask[s->mcur], s->div, s->p[0], s->red5, s->W, s->H);
+  NX_HIP(hipGetLastError());
+  NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,
+                       nullptr, 0, 0, s->stream));
+  double sums[6];
+  NX_HIP(hipMemcpyAsync(sums, s->sums, sizeof(sums), hipMemcpyDeviceToHost, s->stream));
+  NX_HIP(hipStreamSynchronize(s->stream));
+  s->pcur = 0;
+  *rel = sums[5];
+  return NX_OK;
+}
+
+int nx_refine_accumulate(nx_sim* s) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (!s->refining) return fail(NX_ERR_STATE, "nx_refine_begin first");
+  const int n = s->W * s->H;
+  k_refine_accumulate<<<(n + 255) / 256, 256, 0, s->stream>>>(s->ph, s->pl,
+                                                              s->p[s->pcur], n);
+  NX_HIP(hipGetLastError());
+  return NX_OK;
+}
+
+int nx_refine_end(nx_sim* s) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (!s->refining) return fail(NX_ERR_STATE, "nx_refine_begin first");
+  const int n = s->W * s->H;
+  // the projection reads the pressure ph + pl; the right-hand side slot
+  // gets the original (centred) divergence back
+  k_refine_combine<<<(n + 255) / 256, 256, 0, s->stream>>>(s->ph, s->pl, s->p[0], n);
+  NX_HIP(hipGetLastError());
+  NX_HIP(hipMemcpyAsync(s->div, s->b0, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice,
+                        s->stream));
+  s->pcur = 0;
+  s->refining = false;
+  return NX_OK;
+}
+
 int nx_step_finish(nx_sim* s) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
   if (!s->in_step) return fail(NX_ERR_STATE, "nx_step_finish without nx_step_begin");
+  if (s->refining) NX_TRY(nx_refine_end(s));
   const nx_params& P = s->prm;
   const int W = s->W, H = s->H;
   const int a = s->vcur;

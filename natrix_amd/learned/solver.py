@@ -68,42 +68,75 @@ def _pressure_view(sim):
     return torch.as_tensor(buf, device="cuda")
 
 
-def solve_step(sim, guess, dt, tol=1e-4, smooth=256, max_cycles=8,
-               max_sweeps=2_000_000):
-    """One fluid step whose pressure solve uses the learned guess.  Returns
-    the solve statistics plus the number of learned corrections kept."""
-    sim._push_params()
-    _lib.call("nx_step_begin", sim._alive(), float(dt))
-    stream = torch.cuda.ExternalStream(sim.stream_handle())
-    kept = 0
-    rejected = False
-    with torch.cuda.stream(stream):
-        rel0, _ = sim.residual()
-        before = rel0
-        for _ in range(max_cycles):
-            if guess is None or rejected:
-                break
-            backup = _pressure_view(sim).clone()
-            guess.correct(sim)
-            st = _solve(sim, tol, smooth)
-            after = st["residual"]
-            if not (after < before):
-                # the correction did not help: restore p and go plain
-                _pressure_view(sim).copy_(backup)
-                rejected = True
-                break
-            kept += 1
-            before = after
-            if st["converged"]:
-                break
-        st = _solve(sim, tol, max_sweeps)
-    _lib.call("nx_step_finish", sim._alive())
-    st["learned_corrections"] = kept
-    st["fallback"] = rejected
-    return st
-
-
 def _solve(sim, tol, max_sweeps):
     st = _lib.Stats()
     _lib.call("nx_solve", sim._alive(), float(tol), int(max_sweeps), C.byref(st))
     return {"sweeps": st.sweeps, "converged": bool(st.converged), "residual": st.residual}
+
+
+def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.9):
+    """Solve the correction equation A e = r (current right-hand side, e from
+    its first sweep) to relative residual `target`: learned corrections with
+    smoothing first (each kept only if it lowers the residual), then plain
+    Jacobi in chunks until the target, the sweep cap, or a stall (a chunk
+    that reduces the residual by less than 1 - `stall`)."""
+    kept, rejected = 0, False
+    before, _ = sim.residual()
+    start = sim.stats()["sweeps"]
+    for _ in range(max_cycles if guess is not None else 0):
+        backup = _pressure_view(sim).clone()
+        guess.correct(sim)
+        st = _solve(sim, target, smooth)
+        if not (st["residual"] < before):
+            _pressure_view(sim).copy_(backup)
+            rejected = True
+            break
+        kept += 1
+        before = st["residual"]
+        if st["converged"]:
+            return kept, rejected
+    while True:
+        st = _solve(sim, target, chunk)
+        if st["converged"] or st["sweeps"] - start >= max_sweeps:
+            break
+        if st["residual"] > stall * before:
+            break
+        before = st["residual"]
+    return kept, rejected
+
+
+def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.03, smooth=512, max_cycles=4,
+               max_outer=12, chunk=65536, max_inner_sweeps=4_000_000):
+    """One fluid step whose pressure is solved to FP64 relative residual
+    <= tol by iterative refinement (pressure held as a float32 pair, outer
+    residual in FP64, float32 correction solves); with `guess` (a
+    LearnedGuess) each correction solve starts with learned corrections,
+    without it the correction solves are plain Jacobi from zero.  Returns
+    the step's statistics."""
+    sim._push_params()
+    _lib.call("nx_step_begin", sim._alive(), float(dt))
+    _lib.call("nx_refine_begin", sim._alive())
+    stream = torch.cuda.ExternalStream(sim.stream_handle())
+    rel = C.c_double()
+    kept = 0
+    fallback = False
+    outer = 0
+    converged = False
+    with torch.cuda.stream(stream):
+        for outer in range(max_outer + 1):
+            _lib.call("nx_refine_residual", sim._alive(), C.byref(rel))
+            if rel.value <= tol:
+                converged = True
+                break
+            if outer == max_outer:
+                break
+            target = max(inner_tol, 0.5 * tol / rel.value)
+            k, rej = _inner(sim, guess, target, smooth, max_cycles, chunk,
+                            max_inner_sweeps)
+            kept += k
+            fallback |= rej
+            _lib.call("nx_refine_accumulate", sim._alive())
+    sweeps = sim.stats()["sweeps"]
+    _lib.call("nx_step_finish", sim._alive())
+    return {"sweeps": sweeps, "converged": converged, "residual": rel.value,
+            "outer": outer, "learned_corrections": kept, "fallback": fallback}

@@ -241,3 +241,33 @@ def test_simulate_false_freezes(gpu):
     assert_bitwise(sim.download("velocity"), u, "frozen")
     assert np.all(sim.download("mask") == 0)
     sim.destroy()
+
+
+def test_fp64_oracle_solve_matches_scipy(gpu):
+    import torch
+    from natrix_amd import fp64
+    H, W = 48, 64
+    rng = np.random.default_rng(12)
+    solid = np.zeros((H, W), bool)
+    solid[10:22, 30:37] = True
+    solid[35:40, 5:9] = True
+    b = rng.standard_normal((H, W)).astype(f32)
+    sim = make_sim(H, W)
+    sim.upload("mask", solid.astype(np.uint8))
+    sim.upload("divergence", b)
+    x, info = fp64.solve(sim, tol=1e-12)
+    assert info["residual"] < 1e-11
+    bb = b.astype(np.float64)
+    bb[solid] = 0
+    ref = P.solve_fp64(bb, solid)
+    got = x.cpu().numpy()
+    fl = ~solid
+    got = got - got[fl].mean()
+    assert np.abs(got[fl] - ref[fl]).max() < 1e-8 * np.abs(ref).max()
+    # error metric: the exact solution rounded to float32 has a tiny error,
+    # a Jacobi iterate a large one
+    sim.upload("pressure", ref.astype(f32))
+    assert fp64.pressure_error(sim, x) < 1e-6
+    sim.upload("pressure", F.jacobi(np.zeros((H, W), f32), b, obs_from_mask(solid.astype(np.uint8)), 20))
+    assert fp64.pressure_error(sim, x) > 0.1
+    sim.destroy()
