@@ -47,7 +47,9 @@ __device__ __forceinline__ void fused_vort_div_tile(
   __shared__ float2 A[(TX + 2 * HMAX) * (TY + 2 * HMAX)];
   __shared__ float Wc[(TX + 2 * HMAX - 2) * (TY + 2 * HMAX - 2)];
   __shared__ float2 V2[(TX + 2 * HMAX - 4) * (TY + 2 * HMAX - 4)];
-  __shared__ float2 V3[(TX + 2) * (TY + 2)];
+  // the viscosity output reuses A's storage: A is last read by the
+  // confinement stage, which a barrier separates from the viscosity stage
+  float2* const V3 = A;
 
   const Tile t{ox, oy, W, H};
   const int hA = 3 + (viscous ? 1 : 0), hW = hA - 1, hV2 = hA - 2;

@@ -50,7 +50,8 @@ def augment(x, *ys, gen):
     return (t(x, 4),) + tuple(t(y, 3) for y in ys)
 
 
-def train(b, solid, p, epochs=6, batch=32, lr=1e-3, seed=0, log=print, val_frac=0.05):
+def train(b, solid, p, epochs=6, batch=32, lr=1e-3, seed=0, log=print, val_frac=0.05,
+          time_budget=None):
     dev = "cuda"
     torch.manual_seed(seed)
     gen = torch.Generator().manual_seed(seed)
@@ -81,6 +82,11 @@ def train(b, solid, p, epochs=6, batch=32, lr=1e-3, seed=0, log=print, val_frac=
             sched.step()
             tot += float(loss)
             nb += 1
+            if nb % 100 == 0:
+                log(json.dumps({"epoch": ep, "batch": nb, "loss": tot / nb,
+                                "seconds": round(time.time() - t0, 1)}))
+            if time_budget is not None and time.time() - t0 > time_budget:
+                break
         with torch.no_grad():
             vl = []
             for i in range(0, nval, batch):
@@ -90,6 +96,9 @@ def train(b, solid, p, epochs=6, batch=32, lr=1e-3, seed=0, log=print, val_frac=
                "seconds": round(time.time() - t0, 1)}
         hist.append(rec)
         log(json.dumps(rec))
+        if time_budget is not None and time.time() - t0 > time_budget:
+            log(json.dumps({"stopped": "time budget", "seconds": round(time.time() - t0, 1)}))
+            break
     return net, hist
 
 
@@ -115,6 +124,8 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-sweeps", type=int, default=400000)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--time-budget", type=float, default=None,
+                    help="stop training after this many seconds (weights are saved)")
     a = ap.parse_args(argv)
     from .data import make_dataset
     log = lambda m: print(m, flush=True)
@@ -124,7 +135,8 @@ def main(argv=None):
     log(json.dumps({"dataset": a.samples, "size": a.size, "seconds": round(time.time() - t0, 1),
                     "residual_median": float(np.median(res)), "residual_max": float(res.max()),
                     "frac_at_tol": float((res <= a.tol).mean())}))
-    net, hist = train(b, solid, p, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    net, hist = train(b, solid, p, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log,
+                      time_budget=a.time_budget)
     meta = {"samples": a.samples, "size": a.size, "epochs": a.epochs,
             "data_residual_median": float(np.median(res)),
             "data_residual_max": float(res.max()), "history": hist,

@@ -48,29 +48,37 @@ def test_mixed_step_matches_replay(gpu, storage, corr):
     sim.destroy()
 
 
-@pytest.mark.parametrize("storage,tol", [("bf16", 3e-2), ("f16", 3e-3)])
-def test_mixed_error_against_fp64(gpu, storage, tol):
-    """Field error of a short rollout against the float64 oracle."""
-    H, W, steps = 96, 96, 5
+def test_mixed_error_against_fp64(gpu):
+    """Relative L2 velocity error of short rollouts against the float64
+    oracle: float32 storage < float16 < bfloat16, each within a bound set by
+    its unit round-off (bf16 2^-9, f16 2^-12) times a modest amplification."""
+    H, W, steps = 96, 96, 3
     rng = np.random.default_rng(23)
-    u0 = random_velocity(rng, H, W, 0.4)
-    u0r = M.rounder(storage)(u0)
+    u0 = random_velocity(rng, H, W, 0.2)
     solid = np.zeros((H, W), bool)
     solid[40:56, 30:38] = True
-    sim = make_sim(H, W, bc_mode=1, storage=storage)
-    sim.vorticity = 0.3
-    sim.viscosity = 0.0
-    sim.iterations = 60
-    sim.upload("velocity", u0r)
-    sim.set_obstacle_map(solid)
-    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=60, bc_mode=F.BC_FREE_SLIP)
-    u64 = u0r.astype(np.float64)
-    p64 = np.zeros((H, W))
-    o64 = np.zeros((H, W, 2))
-    for _ in range(steps):
-        sim.update(1 / 60)
-        u64, p64, o64, _ = F.step(u64, p64, o64, 1 / 60, prm, persistent=solid)
-    got = sim.download("velocity").astype(np.float64)
-    err = np.abs(got - u64).max() / np.abs(u64).max()
-    assert err < tol, err
-    sim.destroy()
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=60, speed=100.0,
+                   bc_mode=F.BC_FREE_SLIP)
+    errs = {}
+    for storage in ("f32", "f16", "bf16"):
+        u0r = M.rounder(storage)(u0)
+        sim = make_sim(H, W, bc_mode=1, storage=storage)
+        sim.vorticity = 0.3
+        sim.viscosity = 0.0
+        sim.speed = 100.0
+        sim.iterations = 60
+        sim.upload("velocity", u0r)
+        sim.set_obstacle_map(solid)
+        u64 = u0r.astype(np.float64)
+        p64 = np.zeros((H, W))
+        o64 = np.zeros((H, W, 2))
+        for _ in range(steps):
+            sim.update(1 / 60)
+            u64, p64, o64, _ = F.step(u64, p64, o64, 1 / 60, prm, persistent=solid)
+        got = sim.download("velocity").astype(np.float64)
+        errs[storage] = float(np.linalg.norm(got - u64) / np.linalg.norm(u64))
+        sim.destroy()
+    print("velocity L2 error vs fp64:", errs)
+    assert errs["f32"] < 1e-4
+    assert errs["f16"] < 1e-2 and errs["bf16"] < 5e-2
+    assert errs["f32"] < errs["f16"] < errs["bf16"]
