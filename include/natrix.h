@@ -143,13 +143,38 @@ int nx_step_begin(nx_sim* sim, float dt);
 int nx_solve(nx_sim* sim, double tol, int max_sweeps, nx_stats* out);
 /* coarse correction problem: coarse_rhs[Hc*Wc] (device, float32) = factor^2
  * x the fluid mean of the fine residual per factor x factor block,
- * coarse_mask[Hc*Wc] (device, uint8) = 1 for blocks at least half solid;
+ * coarse_mask[Hc*Wc] (device, uint8) = 1 for blocks with no fluid cell;
  * Wc = ceil(W/factor), Hc = ceil(H/factor) */
 int nx_restrict_residual(nx_sim* sim, int factor, float* coarse_rhs,
                          uint8_t* coarse_mask);
 /* pressure += scale * bilinear interpolation of coarse[Hc*Wc] (device) on
- * fluid cells */
-int nx_prolong_add(nx_sim* sim, int factor, const float* coarse, float scale);
+ * fluid cells, weights renormalised over the fluid coarse cells of
+ * coarse_mask (as written by nx_restrict_residual) */
+int nx_prolong_add(nx_sim* sim, int factor, const float* coarse,
+                   const uint8_t* coarse_mask, float scale);
+/* pressure += a * d with d the interpolation of coarse[Hc*Wc] as in
+ * nx_prolong_add and a = (d.r)/(d.Ld) the minimiser of the solve's energy
+ * functional along d (r = b - L p, L the Jacobi operator; a = 0 if d has
+ * no energy), so the error in the energy norm does not grow; all on the
+ * simulator's stream.  out (host, may be NULL; then no synchronisation)
+ * receives {a, d.r, d.Ld}. */
+int nx_prolong_line_search(nx_sim* sim, int factor, const float* coarse,
+                           const uint8_t* coarse_mask, double* out);
+/* smoothing of the coarse correction problem L e = rc on a Wc x Hc grid
+ * (at most 128 x 128; device buffers as written by nx_restrict_residual),
+ * e[Hc*Wc] updated in place, one launch on the simulator's stream:
+ * lambda_min <= 0: `iterations` Jacobi sweeps (the fine sweep's operator);
+ * 0 < lambda_min < 8: `iterations` Chebyshev iterations of the same
+ * operator on [lambda_min, 8] (-L has its spectrum in [0, 8]) */
+int nx_coarse_smooth(nx_sim* sim, int Wc, int Hc, float* e, const float* rc,
+                     const uint8_t* coarse_mask, int iterations, float lambda_min);
+/* `count` damped Jacobi sweeps p <- p + omega (J(p) - p) of the current
+ * pressure system (0 < omega <= 1; omega = 1/2 removes the checkerboard
+ * modes the plain sweep cannot damp); counted in the step's sweeps */
+int nx_damped_sweeps(nx_sim* sim, int count, float omega);
+/* the same line search along an arbitrary direction d[H*W] (device,
+ * float32): pressure += a * d on fluid cells */
+int nx_line_search(nx_sim* sim, const float* d, double* out);
 /* iterative refinement inside a step (for tolerances below the float32
  * round-off floor of a large pressure): the pressure becomes a float32 pair
  * ph + pl, initially 0.  nx_refine_residual evaluates r = b - A(ph + pl) in

@@ -15,6 +15,7 @@
 
 #include "../../include/natrix.h"
 #include "nx_kernels.h"
+#include "nx_jacobi_tile.h"
 
 using namespace nx;
 
@@ -83,6 +84,8 @@ struct nx_sim {
   float* ph = nullptr;
   float* pl = nullptr;
   float* b0 = nullptr;
+  float* dcorr = nullptr;   // prolonged coarse correction (line search)
+  double* line = nullptr;   // {alpha, d.r, d.Ld, 0}
   bool refining = false;
   // step phases (nx_step_begin .. nx_step_finish)
   bool in_step = false;
@@ -503,7 +506,8 @@ static int step_fused_enqueue(nx_sim* s, float dt, bool tolerance) {
 // Restriction: coarse cell (I, J) of factor f covers fine cells
 // [fI, fI+f) x [fJ, fJ+f).  rhs_c = f^2 * mean over its fluid fine cells of
 // the float32 residual r = b - A p (the correction equation in coarse grid
-// units); mask_c = 1 when at least half of its fine cells are solid.
+// units); mask_c = 1 only when every one of its fine cells is solid, so the
+// coarse fluid region covers the fine one.
 __global__ void __launch_bounds__(256)
     k_restrict(const float* __restrict__ p, const float* __restrict__ b,
                const uint8_t* __restrict__ mask, float* __restrict__ rc,
@@ -511,14 +515,13 @@ __global__ void __launch_bounds__(256)
   const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y * 4 + threadIdx.y;
   if (X >= Wc || Y >= Hc) return;
   float sum = 0.0f;
-  int fluid = 0, total = 0;
+  int fluid = 0;
   for (int j = 0; j < f; ++j) {
     const int y = Y * f + j;
     if (y >= H) break;
     for (int i = 0; i < f; ++i) {
       const int x = X * f + i;
       if (x >= W) break;
-      ++total;
       const int pos = y * W + x;
       if (mask[pos]) continue;
       const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
@@ -532,31 +535,290 @@ __global__ void __launch_bounds__(256)
       ++fluid;
     }
   }
-  const bool solid = 2 * (total - fluid) >= total;
-  rc[Y * Wc + X] = (solid || fluid == 0) ? 0.0f : (float)(f * f) * (sum / (float)fluid);
-  mc[Y * Wc + X] = solid ? 1 : 0;
+  rc[Y * Wc + X] = fluid == 0 ? 0.0f : (float)(f * f) * (sum / (float)fluid);
+  mc[Y * Wc + X] = fluid == 0 ? 1 : 0;
 }
 
-// Prolongation: p += scale * bilinear(e_c) on fluid cells, coarse cell
-// centres at fine coordinate f*I + (f-1)/2, clamped at the edges.
+// Prolongation: p += scale * e interpolated at the fine cell, on fluid
+// cells.  Coarse cell centres sit at fine coordinate f*I + (f-1)/2
+// (clamped at the edges); the four bilinear weights are renormalised over
+// the fluid coarse cells among them, so solid coarse cells (which carry no
+// correction) do not pull the correction toward zero next to an obstacle.
+template <bool ADD>
 __global__ void __launch_bounds__(256)
-    k_prolong_add(float* __restrict__ p, const uint8_t* __restrict__ mask,
-                  const float* __restrict__ ec, int W, int H, int f, int Wc, int Hc,
-                  float scale) {
+    k_prolong(float* __restrict__ p, const uint8_t* __restrict__ mask,
+              const float* __restrict__ ec, const uint8_t* __restrict__ mc, int W, int H,
+              int f, int Wc, int Hc, float scale) {
   const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
   if (x >= W || y >= H) return;
   const int pos = y * W + x;
-  if (mask[pos]) return;
-  const float cx = ((float)x - 0.5f * (float)(f - 1)) / (float)f;
-  const float cy = ((float)y - 0.5f * (float)(f - 1)) / (float)f;
-  const float fx = fminf(fmaxf(cx, 0.0f), (float)(Wc - 1));
-  const float fy = fminf(fmaxf(cy, 0.0f), (float)(Hc - 1));
-  const int x0 = (int)fx, y0 = (int)fy;
-  const int x1 = min(x0 + 1, Wc - 1), y1 = min(y0 + 1, Hc - 1);
-  const float tx = fx - (float)x0, ty = fy - (float)y0;
-  const float a = ec[y0 * Wc + x0] * (1.0f - tx) + ec[y0 * Wc + x1] * tx;
-  const float c = ec[y1 * Wc + x0] * (1.0f - tx) + ec[y1 * Wc + x1] * tx;
-  p[pos] = p[pos] + scale * (a * (1.0f - ty) + c * ty);
+  float v = 0.0f;
+  if (mask[pos] == 0) {
+    const float cx = ((float)x - 0.5f * (float)(f - 1)) / (float)f;
+    const float cy = ((float)y - 0.5f * (float)(f - 1)) / (float)f;
+    const float fx = fminf(fmaxf(cx, 0.0f), (float)(Wc - 1));
+    const float fy = fminf(fmaxf(cy, 0.0f), (float)(Hc - 1));
+    const int x0 = (int)fx, y0 = (int)fy;
+    const int x1 = min(x0 + 1, Wc - 1), y1 = min(y0 + 1, Hc - 1);
+    const float tx = fx - (float)x0, ty = fy - (float)y0;
+    const int i00 = y0 * Wc + x0, i01 = y0 * Wc + x1, i10 = y1 * Wc + x0,
+              i11 = y1 * Wc + x1;
+    const float w00 = mc[i00] ? 0.0f : (1.0f - tx) * (1.0f - ty);
+    const float w01 = mc[i01] ? 0.0f : tx * (1.0f - ty);
+    const float w10 = mc[i10] ? 0.0f : (1.0f - tx) * ty;
+    const float w11 = mc[i11] ? 0.0f : tx * ty;
+    const float wsum = ((w00 + w01) + w10) + w11;
+    if (wsum > 0.0f)
+      v = (((w00 * ec[i00] + w01 * ec[i01]) + w10 * ec[i10]) + w11 * ec[i11]) / wsum;
+  }
+  if (ADD) {
+    if (mask[pos] == 0) p[pos] = p[pos] + scale * v;
+  } else {
+    p[pos] = scale * v;
+  }
+}
+
+// Smoothing of a coarse correction problem L e = r held entirely by one
+// workgroup: lane l of wave w owns columns 2l, 2l+1 and rows 8w .. 8w+7 of a
+// grid of at most 128 x 128 cells, in registers, for every iteration.
+// Horizontal neighbours come from the adjacent lane by DPP, the rows next to
+// a wave's strip from double-buffered LDS (one barrier per stencil).  The
+// operator is the fine sweep's: a solid or out-of-grid neighbour reads the
+// centre value, so -L is symmetric positive semi-definite with spectrum in
+// [0, 8].  CHEB = false: Jacobi sweeps e <- e + (L e - r) / 4.  CHEB = true:
+// Chebyshev iteration for -L e = -r on the interval [lmin, 8] (three-term
+// recurrence of Saad, Iterative Methods for Sparse Linear Systems, 2nd ed.,
+// Algorithm 12.1), which damps every mode above lmin by 1 / T_k((8 + lmin)
+// / (8 - lmin)) after k iterations and never amplifies the modes below.
+constexpr int CJ_ROWS = 8, CJ_WAVES = 16, CJ_MAX = 128;
+
+struct CoarseTile {
+  float (*top)[CJ_WAVES][CJ_MAX];
+  float (*bot)[CJ_WAVES][CJ_MAX];
+  int code[CJ_ROWS][2];  // bit 0: left, 1: right, 2: up (y-1), 3: down (y+1)
+  int lane, wave, x0, buf;
+
+  // out = L v for this lane's cells (all threads of the block call it)
+  __device__ __forceinline__ void lap(const float (&v)[CJ_ROWS][2],
+                                      float (&out)[CJ_ROWS][2]) {
+    top[buf][wave][x0] = v[0][0];
+    top[buf][wave][x0 + 1] = v[0][1];
+    bot[buf][wave][x0] = v[CJ_ROWS - 1][0];
+    bot[buf][wave][x0 + 1] = v[CJ_ROWS - 1][1];
+    __syncthreads();
+    float fl[CJ_ROWS], fr[CJ_ROWS];
+#pragma unroll
+    for (int i = 0; i < CJ_ROWS; ++i) {
+      fl[i] = jt::lane_from_left(v[i][1]);
+      fr[i] = jt::lane_from_right(v[i][0]);
+    }
+    const float* above = bot[buf][wave > 0 ? wave - 1 : 0];
+    const float* below = top[buf][wave < CJ_WAVES - 1 ? wave + 1 : 0];
+#pragma unroll
+    for (int i = 0; i < CJ_ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float ctr = v[i][c];
+        const int cd = code[i][c];
+        float L = c == 1 ? v[i][0] : fl[i];
+        float R = c == 0 ? v[i][1] : fr[i];
+        float U = i > 0 ? v[i - 1][c] : above[x0 + c];
+        float D = i < CJ_ROWS - 1 ? v[i + 1][c] : below[x0 + c];
+        L = (cd & 1) ? ctr : L;
+        R = (cd & 2) ? ctr : R;
+        U = (cd & 4) ? ctr : U;
+        D = (cd & 8) ? ctr : D;
+        out[i][c] = (((L + R) + U) + D) - 4.0f * ctr;
+      }
+    buf ^= 1;
+  }
+};
+
+template <bool CHEB>
+__global__ void __launch_bounds__(1024)
+    k_coarse_smooth(float* __restrict__ e, const float* __restrict__ rhs,
+                    const uint8_t* __restrict__ mc, int Wc, int Hc, int iters, float lmin) {
+  __shared__ float top[2][CJ_WAVES][CJ_MAX], bot[2][CJ_WAVES][CJ_MAX];
+  CoarseTile t;
+  t.top = top;
+  t.bot = bot;
+  t.lane = threadIdx.x & 63;
+  t.wave = threadIdx.x >> 6;
+  t.x0 = 2 * t.lane;
+  t.buf = 0;
+  const int y0 = CJ_ROWS * t.wave;
+  auto solid = [&](int x, int y) {
+    return x < 0 || y < 0 || x >= Wc || y >= Hc || mc[y * Wc + x] != 0;
+  };
+  float x[CJ_ROWS][2], r[CJ_ROWS][2], lv[CJ_ROWS][2];
+#pragma unroll
+  for (int i = 0; i < CJ_ROWS; ++i)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int gx = t.x0 + c, gy = y0 + i;
+      const bool in = gx < Wc && gy < Hc;
+      x[i][c] = in ? e[gy * Wc + gx] : 0.0f;
+      r[i][c] = in ? rhs[gy * Wc + gx] : 0.0f;
+      t.code[i][c] = (solid(gx - 1, gy) ? 1 : 0) | (solid(gx + 1, gy) ? 2 : 0) |
+                     (solid(gx, gy - 1) ? 4 : 0) | (solid(gx, gy + 1) ? 8 : 0);
+    }
+  if (!CHEB) {
+    for (int s = 0; s < iters; ++s) {
+      t.lap(x, lv);
+#pragma unroll
+      for (int i = 0; i < CJ_ROWS; ++i)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) x[i][c] = x[i][c] + (lv[i][c] - r[i][c]) * 0.25f;
+    }
+  } else {
+    // A = -L, f = -rhs; residual q = f - A x = L x - rhs
+    const float theta = 0.5f * (8.0f + lmin), delta = 0.5f * (8.0f - lmin);
+    const float sigma = theta / delta;
+    float rho = 1.0f / sigma;
+    float d[CJ_ROWS][2];
+    t.lap(x, lv);
+#pragma unroll
+    for (int i = 0; i < CJ_ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        r[i][c] = lv[i][c] - r[i][c];
+        d[i][c] = r[i][c] / theta;
+      }
+    for (int s = 0; s < iters; ++s) {
+      t.lap(d, lv);
+      const float rho1 = 1.0f / (2.0f * sigma - rho);
+      const float cd = rho1 * rho, cr = 2.0f * rho1 / delta;
+#pragma unroll
+      for (int i = 0; i < CJ_ROWS; ++i)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          x[i][c] = x[i][c] + d[i][c];
+          r[i][c] = r[i][c] + lv[i][c];  // q -= A d
+          d[i][c] = cd * d[i][c] + cr * r[i][c];
+        }
+      rho = rho1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < CJ_ROWS; ++i)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int gx = t.x0 + c, gy = y0 + i;
+      if (gx < Wc && gy < Hc) e[gy * Wc + gx] = x[i][c];
+    }
+}
+
+// Damped Jacobi sweep p <- p + w (J(p) - p), J the plain sweep (same
+// operator and summation order).  The plain sweep's iteration matrix has
+// eigenvalues (cos kx + cos ky) / 2 in [-1, 1]: the near-checkerboard modes
+// decay as slowly as the smoothest ones, and the coarse correction cannot
+// reach them (restriction averages them away).  w = 1/2 maps an eigenvalue
+// m to (1 + m) / 2, which removes them.
+__global__ void __launch_bounds__(256)
+    k_damped_sweep(const float* __restrict__ pin, const float* __restrict__ b,
+                   const uint8_t* __restrict__ mask, float* __restrict__ pout, int W,
+                   int H, float w) {
+  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  if (x >= W || y >= H) return;
+  const int pos = y * W + x;
+  const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
+  const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
+  const float p = pin[pos];
+  const float x1 = mask[iL] ? p : pin[iL];
+  const float x2 = mask[iR] ? p : pin[iR];
+  const float y1 = mask[iB] ? p : pin[iB];
+  const float y2 = mask[iT] ? p : pin[iT];
+  const float jac = ((((x1 + x2) + y1) + y2) - b[pos]) * 0.25f;
+  pout[pos] = p + w * (jac - p);
+}
+
+// Energy line search along a correction d.  With L the 5-point operator of
+// the Jacobi sweep (a solid neighbour reads the centre value, an edge
+// neighbour is the cell itself), -L is symmetric positive semi-definite on
+// the fluid cells and the pressure minimises phi(p) = 1/2 p.(-L)p + b.p.
+// Along p + a d, phi is smallest at a = (d.r) / (d.Ld) with r = b - L p, and
+// phi(p) - phi(p*) is half the squared energy norm of the error, so the step
+// never increases that error.  Per block: FP64 partial sums of d.r, d.Ld.
+__global__ void __launch_bounds__(256)
+    k_line_terms(const float* __restrict__ p, const float* __restrict__ b,
+                 const uint8_t* __restrict__ mask, const float* __restrict__ d,
+                 double* __restrict__ red, int W, int H) {
+  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  double v0 = 0.0, v1 = 0.0;
+  if (x < W && y < H) {
+    const int pos = y * W + x;
+    if (mask[pos] == 0) {
+      const int nb[4] = {y * W + max(x - 1, 0), y * W + min(x + 1, W - 1),
+                         max(y - 1, 0) * W + x, min(y + 1, H - 1) * W + x};
+      const double pc = (double)p[pos], dc = (double)d[pos];
+      double lp = 0.0, ld = 0.0;
+      for (int j = 0; j < 4; ++j) {
+        if (mask[nb[j]]) continue;
+        lp += (double)p[nb[j]] - pc;
+        ld += (double)d[nb[j]] - dc;
+      }
+      v0 = dc * ((double)b[pos] - lp);
+      v1 = dc * ld;
+    }
+  }
+  __shared__ double part[4][2];
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off, 64);
+    v1 += __shfl_xor(v1, off, 64);
+  }
+  if (lane == 0) {
+    part[wave][0] = v0;
+    part[wave][1] = v1;
+  }
+  __syncthreads();
+  if (lane == 0 && wave == 0) {
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    red[2 * bid] = ((part[0][0] + part[1][0]) + part[2][0]) + part[3][0];
+    red[2 * bid + 1] = ((part[0][1] + part[1][1]) + part[2][1]) + part[3][1];
+  }
+}
+
+// line = {a, d.r, d.Ld, 0}; a = 0 when d carries no energy (d.Ld >= 0) or
+// either sum is not finite (a NaN or infinite direction)
+__global__ void __launch_bounds__(256)
+    k_line_alpha(const double* __restrict__ red, int nblocks, double* __restrict__ line) {
+  double v0 = 0.0, v1 = 0.0;
+  for (int i = threadIdx.x; i < nblocks; i += 256) {
+    v0 += red[2 * i];
+    v1 += red[2 * i + 1];
+  }
+  __shared__ double part[4][2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off, 64);
+    v1 += __shfl_xor(v1, off, 64);
+  }
+  if (lane == 0) {
+    part[wave][0] = v0;
+    part[wave][1] = v1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double dr = ((part[0][0] + part[1][0]) + part[2][0]) + part[3][0];
+    const double dld = ((part[0][1] + part[1][1]) + part[2][1]) + part[3][1];
+    const bool ok = isfinite(dr) && isfinite(dld) && dld < 0.0;
+    line[0] = ok ? dr / dld : 0.0;
+    line[1] = dr;
+    line[2] = dld;
+    line[3] = 0.0;
+  }
+}
+
+// p += a d on fluid cells, a read from the device; a = 0 leaves p untouched
+// (d may hold non-finite values then)
+__global__ void k_line_apply(float* __restrict__ p, const float* __restrict__ d,
+                             const uint8_t* __restrict__ mask,
+                             const double* __restrict__ line, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const float a = (float)line[0];
+  if (i >= n || mask[i] || a == 0.0f) return;
+  p[i] = p[i] + a * d[i];
 }
 
 // ---- iterative refinement of the pressure ---------------------------------
@@ -746,6 +1008,8 @@ int nx_destroy(nx_sim* s) {
   (void)hipFree(s->ph);
   (void)hipFree(s->pl);
   (void)hipFree(s->b0);
+  (void)hipFree(s->dcorr);
+  (void)hipFree(s->line);
   (void)hipFree(s->curl);
   (void)hipFree(s->red5);
   (void)hipFree(s->red2);
@@ -1020,14 +1284,98 @@ int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
   return NX_OK;
 }
 
-int nx_prolong_add(nx_sim* s, int factor, const float* coarse, float scale) {
-  if (!s || !coarse || factor < 1) return fail(NX_ERR_ARG, "bad prolongation arguments");
+int nx_prolong_add(nx_sim* s, int factor, const float* coarse,
+                   const uint8_t* coarse_mask, float scale) {
+  if (!s || !coarse || !coarse_mask || factor < 1)
+    return fail(NX_ERR_ARG, "bad prolongation arguments");
   if (!s->in_step) return fail(NX_ERR_STATE, "prolongation outside a step");
   const int Wc = (s->W + factor - 1) / factor, Hc = (s->H + factor - 1) / factor;
-  k_prolong_add<<<dim3((s->W + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0, s->stream>>>(
-      s->p[s->pcur], s->mask[s->mcur], coarse, s->W, s->H, factor, Wc, Hc, scale);
+  k_prolong<true><<<dim3((s->W + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+      s->p[s->pcur], s->mask[s->mcur], coarse, coarse_mask, s->W, s->H, factor, Wc, Hc,
+      scale);
   NX_HIP(hipGetLastError());
   return NX_OK;
+}
+
+int nx_coarse_smooth(nx_sim* s, int Wc, int Hc, float* e, const float* rc,
+                     const uint8_t* coarse_mask, int iterations, float lambda_min) {
+  if (!s || !e || !rc || !coarse_mask) return fail(NX_ERR_ARG, "null argument");
+  if (Wc < 1 || Hc < 1 || Wc > CJ_MAX || Hc > CJ_ROWS * CJ_WAVES || iterations < 0)
+    return fail(NX_ERR_ARG, "coarse grid must be at most 128 x 128, iterations >= 0");
+  if (!(lambda_min < 8.0f)) return fail(NX_ERR_ARG, "lambda_min must be below 8");
+  if (iterations == 0) return NX_OK;
+  if (lambda_min > 0.0f)
+    k_coarse_smooth<true><<<1, 1024, 0, s->stream>>>(e, rc, coarse_mask, Wc, Hc,
+                                                     iterations, lambda_min);
+  else
+    k_coarse_smooth<false><<<1, 1024, 0, s->stream>>>(e, rc, coarse_mask, Wc, Hc,
+                                                      iterations, 0.0f);
+  NX_HIP(hipGetLastError());
+  return NX_OK;
+}
+
+int nx_damped_sweeps(nx_sim* s, int count, float omega) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (!s->in_step) return fail(NX_ERR_STATE, "damped sweeps outside a step");
+  if (count < 0 || !(omega > 0.0f && omega <= 1.0f))
+    return fail(NX_ERR_ARG, "count >= 0 and 0 < omega <= 1 required");
+  const dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
+  for (int i = 0; i < count; ++i) {
+    k_damped_sweep<<<grid, dim3(64, 4), 0, s->stream>>>(
+        s->p[s->pcur], s->div, s->mask[s->mcur], s->p[1 - s->pcur], s->W, s->H, omega);
+    NX_HIP(hipGetLastError());
+    s->pcur ^= 1;
+  }
+  s->step_sweeps += count;
+  return NX_OK;
+}
+
+static int ensure_line(nx_sim* s) {
+  if (s->dcorr) return NX_OK;
+  NX_HIP(hipMalloc(&s->dcorr, (size_t)s->W * s->H * sizeof(float)));
+  NX_HIP(hipMalloc(&s->line, 4 * sizeof(double)));
+  return NX_OK;
+}
+
+static int line_search(nx_sim* s, const float* d, double* out) {
+  const size_t n = (size_t)s->W * s->H;
+  const dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
+  // red5 holds 5 doubles per block of this grid; the line terms use 2
+  k_line_terms<<<grid, dim3(64, 4), 0, s->stream>>>(s->p[s->pcur], s->div, s->mask[s->mcur],
+                                                    d, s->red5, s->W, s->H);
+  NX_HIP(hipGetLastError());
+  k_line_alpha<<<1, 256, 0, s->stream>>>(s->red5, residual_blocks(s->W, s->H), s->line);
+  NX_HIP(hipGetLastError());
+  k_line_apply<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(
+      s->p[s->pcur], d, s->mask[s->mcur], s->line, (int)n);
+  NX_HIP(hipGetLastError());
+  if (out) {
+    NX_HIP(hipMemcpyAsync(out, s->line, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                          s->stream));
+    NX_HIP(hipStreamSynchronize(s->stream));
+  }
+  return NX_OK;
+}
+
+int nx_line_search(nx_sim* s, const float* d, double* out) {
+  if (!s || !d) return fail(NX_ERR_ARG, "null argument");
+  if (!s->in_step) return fail(NX_ERR_STATE, "line search outside a step");
+  NX_TRY(ensure_line(s));
+  return line_search(s, d, out);
+}
+
+int nx_prolong_line_search(nx_sim* s, int factor, const float* coarse,
+                           const uint8_t* coarse_mask, double* out) {
+  if (!s || !coarse || !coarse_mask || factor < 1)
+    return fail(NX_ERR_ARG, "bad prolongation arguments");
+  if (!s->in_step) return fail(NX_ERR_STATE, "prolongation outside a step");
+  NX_TRY(ensure_line(s));
+  const int Wc = (s->W + factor - 1) / factor, Hc = (s->H + factor - 1) / factor;
+  const dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
+  k_prolong<false><<<grid, dim3(64, 4), 0, s->stream>>>(
+      s->dcorr, s->mask[s->mcur], coarse, coarse_mask, s->W, s->H, factor, Wc, Hc, 1.0f);
+  NX_HIP(hipGetLastError());
+  return line_search(s, s->dcorr, out);
 }
 
 int nx_refine_begin(nx_sim* s) {

@@ -4,25 +4,30 @@ fallback, and judges the result.
 The network runs on a coarse grid (`coarse` x `coarse`, the resolution it
 was trained at) whose cells are f x f blocks of the fine grid, so its
 receptive field spans the whole domain and it supplies the low-frequency
-part of the pressure, which is what plain Jacobi converges slowly on
-(its slowest mode decays like 1 - pi^2 / (4 N^2) per sweep).
+part of the pressure, which is what plain Jacobi converges slowly on (its
+slowest mode decays like 1 - pi^2 / (4 N^2) per sweep).
 
-One learned correction: restrict the current residual to the coarse grid
-(f^2 x block mean, so the coarse problem is in coarse grid units), scale it
-to unit RMS, run the U-Net, scale back, remove the fluid mean and add the
-bilinear interpolation of the result to the pressure.  The first correction
-acts on the first sweep from zero, i.e. it is the initial guess; the
-following ones act on the residual left after `smooth` Jacobi sweeps (the
-same network applied to the correction equation A e = r).  A correction is
-kept only if, after its smoothing sweeps, the residual is below the
-residual before it; otherwise the pressure is restored and the network is
-not used again in that step.  Plain Jacobi then runs until the relative
-residual is <= tol: the step's result always satisfies the same criterion
-as the plain path.
+One learned cycle:
+  1. restrict the current residual to the coarse grid (f^2 x block mean, so
+     the coarse problem is in coarse grid units) and scale it to unit RMS;
+  2. run the U-Net and scale back: the coarse correction's first guess;
+  3. Chebyshev-accelerated Jacobi on the coarse problem from that guess
+     (one resident launch): removes the network's error at short coarse
+     wavelengths, where its relative error is largest and which the fine
+     sweeps would need tens of thousands of sweeps to undo;
+  4. interpolate to the fine grid and add it scaled by the energy line
+     search (the step along the correction that minimises the solve's
+     energy functional: it never increases the error in the energy norm,
+     and a useless, NaN or infinite correction gets step 0);
+  5. `smooth` fine Jacobi sweeps.
+Cycles repeat while they reduce the residual by a useful factor; then plain
+Jacobi chunks run until the relative residual is <= the target, so the
+step's result satisfies the same criterion as the plain path.
 """
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import torch
 
@@ -30,10 +35,26 @@ from .. import _lib
 from .unet import pack_input
 
 
+def chebyshev_lmin(iterations, damping):
+    """Lower end of the Chebyshev interval [lmin, 8] on which `iterations`
+    iterations reduce every mode by `damping`: 1 / T_k(s) = damping with
+    s = (8 + lmin) / (8 - lmin)."""
+    s = math.cosh(math.acosh(1.0 / damping) / iterations)
+    return 8.0 * (s - 1.0) / (s + 1.0)
+
+
 class LearnedGuess:
-    def __init__(self, net, coarse=128):
-        self.net = net.eval()
+    """`net` maps (coarse right-hand side, coarse mask) to the coarse
+    solution; None gives the same cycle without a network (coarse smoothing
+    from zero), for comparison.  `coarse_iters` Chebyshev iterations with
+    `damping` on the modes above their interval's lower end (0 iterations:
+    the network output as is)."""
+
+    def __init__(self, net, coarse=128, coarse_iters=100, damping=1e-3):
+        self.net = net.eval() if net is not None else None
         self.coarse = coarse
+        self.coarse_iters = coarse_iters
+        self.lmin = chebyshev_lmin(coarse_iters, damping) if coarse_iters else 0.0
 
     def _factor(self, sim):
         W, H = sim.width, sim.height
@@ -43,9 +64,15 @@ class LearnedGuess:
         return W // self.coarse
 
     @torch.no_grad()
-    def correct(self, sim):
+    def correct(self, sim, report=False, line_search=True):
         """Add one learned correction to the simulator's pressure (on the
-        simulator's stream)."""
+        simulator's stream), scaled by the energy line search (or by 1
+        without `line_search`).  With `report` returns (a, d.r, d.Ld)
+        (synchronises); otherwise None."""
+        with torch.cuda.stream(torch.cuda.ExternalStream(sim.stream_handle())):
+            return self._correct(sim, report, line_search)
+
+    def _correct(self, sim, report, line_search):
         f = self._factor(sim)
         c = self.coarse
         rc = torch.empty(c * c, dtype=torch.float32, device="cuda")
@@ -55,12 +82,23 @@ class LearnedGuess:
         fluid = mc == 0
         n = fluid.sum().clamp_min(1)
         s = torch.sqrt((rc * rc * fluid).sum() / n).clamp_min(1e-30)
-        x = pack_input((rc / s).view(1, c, c), (~fluid).view(1, c, c))
-        e = self.net(x).view(-1)
-        e = (e - (e * fluid).sum() / n) * s
-        e = e * fluid
-        _lib.call("nx_prolong_add", sim._alive(), f, C.c_void_p(e.data_ptr()), 1.0)
-        return e
+        if self.net is not None:
+            x = pack_input((rc / s).view(1, c, c), (~fluid).view(1, c, c))
+            e = (self.net(x).view(-1) * s).contiguous()
+        else:
+            e = torch.zeros_like(rc)
+        _lib.call("nx_coarse_smooth", sim._alive(), c, c, C.c_void_p(e.data_ptr()),
+                  C.c_void_p(rc.data_ptr()), C.c_void_p(mc.data_ptr()),
+                  int(self.coarse_iters), float(self.lmin))
+        e = (e - (e * fluid).sum() / n) * fluid
+        if not line_search:
+            _lib.call("nx_prolong_add", sim._alive(), f, C.c_void_p(e.data_ptr()),
+                      C.c_void_p(mc.data_ptr()), 1.0)
+            return None
+        out = (C.c_double * 3)() if report else None
+        _lib.call("nx_prolong_line_search", sim._alive(), f, C.c_void_p(e.data_ptr()),
+                  C.c_void_p(mc.data_ptr()), out)
+        return tuple(out) if report else None
 
 
 def _pressure_view(sim):
@@ -74,45 +112,66 @@ def _solve(sim, tol, max_sweeps):
     return {"sweeps": st.sweeps, "converged": bool(st.converged), "residual": st.residual}
 
 
-def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.9):
-    """Solve the correction equation A e = r (current right-hand side, e from
-    its first sweep) to relative residual `target`: learned corrections with
-    smoothing first (each kept only if it lowers the residual), then plain
-    Jacobi in chunks until the target, the sweep cap, or a stall (a chunk
-    that reduces the residual by less than 1 - `stall`)."""
-    kept, rejected = 0, False
-    before, _ = sim.residual()
+def _smooth(sim, target, sweeps):
+    """`sweeps` fine sweeps of the current system ending in a damped one
+    (omega = 1/2, which removes the near-checkerboard modes the plain sweep
+    keeps), stopping early at relative residual <= target."""
+    st = _solve(sim, target, max(1, sweeps - 2))
+    if st["converged"]:
+        return st
+    _lib.call("nx_damped_sweeps", sim._alive(), 1, 0.5)
+    return _solve(sim, target, 1)
+
+
+def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
+           plain_stall=0.9, trace=None):
+    """Solve the correction equation (current right-hand side, iterate
+    from its first sweep) to relative residual `target`: learned cycles
+    while each reduces the residual below `stall` x the previous one, then
+    plain Jacobi chunks until the target, the sweep cap, or a chunk that
+    reduces it by less than 1 - `plain_stall`.  Returns (cycles whose line
+    search took a nonzero step, True if the learned cycles were abandoned
+    before the target)."""
+    kept, abandoned = 0, False
+    before = None
     start = sim.stats()["sweeps"]
     for _ in range(max_cycles if guess is not None else 0):
-        backup = _pressure_view(sim).clone()
-        guess.correct(sim)
-        st = _solve(sim, target, smooth)
-        if not (st["residual"] < before):
-            _pressure_view(sim).copy_(backup)
-            rejected = True
+        a = guess.correct(sim, report=True)
+        st = _smooth(sim, target, smooth)
+        if trace is not None:
+            trace.append(("cycle", round(a[0], 4), st["sweeps"], st["residual"]))
+        if st["converged"]:
+            return kept + (a[0] != 0.0), False
+        if a[0] == 0.0 or (before is not None and st["residual"] > stall * before):
+            abandoned = True
+            before = st["residual"]
             break
         kept += 1
         before = st["residual"]
-        if st["converged"]:
-            return kept, rejected
     while True:
-        st = _solve(sim, target, chunk)
+        st = _smooth(sim, target, chunk) if guess is not None else _solve(sim, target, chunk)
+        if trace is not None:
+            trace.append(("plain", st["sweeps"], st["residual"]))
         if st["converged"] or st["sweeps"] - start >= max_sweeps:
             break
-        if st["residual"] > stall * before:
+        if before is not None and st["residual"] > plain_stall * before:
             break
         before = st["residual"]
-    return kept, rejected
+    return kept, abandoned
 
 
-def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.03, smooth=512, max_cycles=4,
-               max_outer=12, chunk=65536, max_inner_sweeps=4_000_000):
+def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.03, smooth=256, max_cycles=6,
+               max_outer=12, chunk=65536, max_inner_sweeps=4_000_000, trace=None):
     """One fluid step whose pressure is solved to FP64 relative residual
     <= tol by iterative refinement (pressure held as a float32 pair, outer
     residual in FP64, float32 correction solves); with `guess` (a
-    LearnedGuess) each correction solve starts with learned corrections,
-    without it the correction solves are plain Jacobi from zero.  Returns
-    the step's statistics."""
+    LearnedGuess) each correction solve starts with learned cycles, without
+    it the correction solves are plain Jacobi from zero.  Returns the
+    step's statistics: sweeps (fine Jacobi sweeps), converged, residual
+    (FP64 relative), outer (refinement iterations), learned_corrections
+    (cycles that moved the pressure), fallback (learned cycles abandoned
+    for plain Jacobi in some correction solve).  `trace` (a list) receives
+    one record per outer iteration, learned cycle and plain chunk."""
     sim._push_params()
     _lib.call("nx_step_begin", sim._alive(), float(dt))
     _lib.call("nx_refine_begin", sim._alive())
@@ -125,16 +184,18 @@ def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.03, smooth=512, max_cycles=
     with torch.cuda.stream(stream):
         for outer in range(max_outer + 1):
             _lib.call("nx_refine_residual", sim._alive(), C.byref(rel))
+            if trace is not None:
+                trace.append(("outer", outer, rel.value))
             if rel.value <= tol:
                 converged = True
                 break
             if outer == max_outer:
                 break
             target = max(inner_tol, 0.5 * tol / rel.value)
-            k, rej = _inner(sim, guess, target, smooth, max_cycles, chunk,
-                            max_inner_sweeps)
+            k, ab = _inner(sim, guess, target, smooth, max_cycles, chunk,
+                           max_inner_sweeps, trace=trace)
             kept += k
-            fallback |= rej
+            fallback |= ab
             _lib.call("nx_refine_accumulate", sim._alive())
     sweeps = sim.stats()["sweeps"]
     _lib.call("nx_step_finish", sim._alive())

@@ -1,0 +1,219 @@
+"""The learned pressure solve's building blocks against NumPy statements of
+their mathematics, and the solve itself: a useless or non-finite network
+cannot spoil it (plain Jacobi still reaches the tolerance), and the trained
+network reduces the work to the tolerance on an unseen scene.
+
+Coarse smoother: Jacobi sweeps and the Chebyshev iteration for -L e = -r on
+[lmin, 8], against the float64 statements of tests/smoothers.py (pinned on
+eigenmodes by tests/test_smoother_statements.py).  Line search:
+a = d.r / d.Ld minimises the energy phi(p) = 1/2 p.(-L)p + b.p along d.  Damped sweep: p + w (J(p) - p), the
+float32 operations of the plain sweep (shader.Poisson.comp:26-37)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from natrix_amd import FluidSimulator, _lib
+
+from smoothers import cheb_ref, jacobi_ref, lap
+
+pytestmark = pytest.mark.gpu
+
+
+def in_step_sim(rng, H, W):
+    sim = FluidSimulator(W, H)
+    sim.viscosity = 0.0
+    Y, X = np.mgrid[0:H, 0:W]
+    sim.set_obstacle_map((X - 0.4 * W) ** 2 + (Y - 0.5 * H) ** 2 <= (0.15 * H) ** 2)
+    sim._push_params()
+    _lib.call("nx_step_begin", sim._alive(), 1 / 60)
+    sim.synchronize()
+    return sim
+
+
+def dev(sim, name):
+    return torch.as_tensor(sim.field_buffer(name), device="cuda")
+
+
+@pytest.mark.parametrize("Wc,Hc", [(128, 128), (100, 72), (7, 128)])
+@pytest.mark.parametrize("cheb", [False, True])
+def test_coarse_smoother_matches_float64_statement(gpu, rng, Wc, Hc, cheb):
+    from natrix_amd.learned.solver import chebyshev_lmin
+    sim = in_step_sim(rng, 64, 64)
+    solid = rng.random((Hc, Wc)) < 0.1
+    e0 = rng.normal(size=(Hc, Wc)).astype(np.float32)
+    rc = rng.normal(size=(Hc, Wc)).astype(np.float32)
+    e = torch.from_numpy(e0).cuda().contiguous()
+    r = torch.from_numpy(rc).cuda()
+    m = torch.from_numpy(solid.astype(np.uint8)).cuda()
+    iters = 40 if cheb else 25
+    lmin = chebyshev_lmin(iters, 1e-3) if cheb else 0.0
+    torch.cuda.synchronize()
+    _lib.call("nx_coarse_smooth", sim._alive(), Wc, Hc, C.c_void_p(e.data_ptr()),
+              C.c_void_p(r.data_ptr()), C.c_void_p(m.data_ptr()), iters, lmin)
+    sim.synchronize()
+    ref = (cheb_ref if cheb else jacobi_ref)(e0.astype(np.float64), rc.astype(np.float64),
+                                            solid, iters, *((lmin,) if cheb else ()))
+    got = e.cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+def test_coarse_smoother_rejects_oversized_grids(gpu, rng):
+    sim = in_step_sim(rng, 32, 32)
+    t = torch.zeros(129 * 4, device="cuda")
+    m = torch.zeros(129 * 4, dtype=torch.uint8, device="cuda")
+    with pytest.raises(ValueError):
+        _lib.call("nx_coarse_smooth", sim._alive(), 129, 4, C.c_void_p(t.data_ptr()),
+                  C.c_void_p(t.data_ptr()), C.c_void_p(m.data_ptr()), 5, 0.0)
+    with pytest.raises(ValueError):
+        _lib.call("nx_coarse_smooth", sim._alive(), 4, 4, C.c_void_p(t.data_ptr()),
+                  C.c_void_p(t.data_ptr()), C.c_void_p(m.data_ptr()), 5, 8.0)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+def test_line_search_is_the_energy_minimiser(gpu, rng):
+    H, W = 80, 96
+    sim = in_step_sim(rng, H, W)
+    solid = sim.download("mask") != 0
+    b = rng.normal(size=(H, W)).astype(np.float32)
+    p = rng.normal(size=(H, W)).astype(np.float32)
+    d = rng.normal(size=(H, W)).astype(np.float32)
+    d[solid] = 0
+    dev(sim, "divergence").copy_(torch.from_numpy(b))
+    dev(sim, "pressure").copy_(torch.from_numpy(p))
+    dd = torch.from_numpy(d).cuda()
+    torch.cuda.synchronize()
+    out = (C.c_double * 3)()
+    _lib.call("nx_line_search", sim._alive(), C.c_void_p(dd.data_ptr()), out)
+    got = sim.download("pressure")
+    fl = ~solid
+    p64, b64, d64 = p.astype(np.float64), b.astype(np.float64), d.astype(np.float64)
+    r = b64 - lap(p64, solid)
+    dr = float((d64 * r)[fl].sum())
+    dld = float((d64 * lap(d64, solid))[fl].sum())
+    a = dr / dld
+    assert abs(out[0] - a) <= 1e-9 * abs(a)
+    assert abs(out[1] - dr) <= 1e-9 * abs(dr) and abs(out[2] - dld) <= 1e-9 * abs(dld)
+    want = np.where(fl, p + np.float32(a) * d, p)
+    assert np.array_equal(got, want.astype(np.float32))
+
+    def phi(q):
+        q = q.astype(np.float64)
+        return float((0.5 * q * -lap(q, solid) + b64 * q)[fl].sum())
+    # a is the minimiser: the energy is below both neighbours along d
+    assert phi(got) < phi(p)
+    assert phi(got) <= phi(p + np.float32(1.1 * a) * d)
+    assert phi(got) <= phi(p + np.float32(0.9 * a) * d)
+    # a non-finite direction gets step 0 and leaves the pressure as it was
+    dd[0] = float("nan")
+    torch.cuda.synchronize()
+    _lib.call("nx_line_search", sim._alive(), C.c_void_p(dd.data_ptr()), out)
+    assert out[0] == 0.0
+    assert np.array_equal(sim.download("pressure"), got)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+def test_damped_sweep_matches_float32_statement(gpu, rng):
+    H, W = 72, 100
+    sim = in_step_sim(rng, H, W)
+    solid = sim.download("mask") != 0
+    b = rng.normal(size=(H, W)).astype(np.float32)
+    p = rng.normal(size=(H, W)).astype(np.float32)
+    dev(sim, "divergence").copy_(torch.from_numpy(b))
+    dev(sim, "pressure").copy_(torch.from_numpy(p))
+    torch.cuda.synchronize()
+    _lib.call("nx_damped_sweeps", sim._alive(), 3, 0.5)
+    got = sim.download("pressure")
+    w = np.float32(0.5)
+    q = p
+    for _ in range(3):
+        P = np.pad(q, 1, mode="edge")
+        S = np.pad(solid, 1, mode="edge")
+        nb = []
+        for dy, dx in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+            v = P[1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+            s = S[1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+            nb.append(np.where(s, q, v))
+        jac = ((((nb[0] + nb[1]) + nb[2]) + nb[3]) - b) * np.float32(0.25)
+        q = (q + w * (jac - q)).astype(np.float32)
+    assert np.array_equal(got, q)
+    with pytest.raises(ValueError):
+        _lib.call("nx_damped_sweeps", sim._alive(), 1, 0.0)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+def cylinder_sim(N, warm=3):
+    sim = FluidSimulator(N, N, bc_mode="inflow")
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    X = np.arange(N)[None, :]
+    Y = np.arange(N)[:, None]
+    sim.set_obstacle_map((X - 0.4 * N) ** 2 + (Y - 0.5 * N) ** 2 <= (0.1 * N) ** 2)
+    for _ in range(warm):
+        sim.update(1 / 60)
+    return sim
+
+
+class _Const(torch.nn.Module):
+    def __init__(self, fn):
+        super().__init__()
+        self.fn = fn
+
+    def forward(self, x):
+        return self.fn(x.shape[:3], x.device)
+
+
+@pytest.mark.parametrize("kind", ["nan", "huge"])
+def test_learned_solve_survives_a_bad_network(gpu, kind):
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    fn = {"nan": lambda s, dv: torch.full(s, float("nan"), device=dv),
+          "huge": lambda s, dv: 1e6 * torch.randn(s, device=dv)}[kind]
+    sim = cylinder_sim(128)
+    st = solve_step(sim, LearnedGuess(_Const(fn), coarse=32), 1 / 60, tol=1e-4,
+                    smooth=64, chunk=8192)
+    assert st["converged"] and st["residual"] <= 1e-4
+    if kind == "nan":
+        # every line search took step 0: the network never moved the pressure
+        assert st["learned_corrections"] == 0 and st["fallback"]
+    # the step's pressure (ph + pl rounded to float32) is finite and the
+    # oracle's residual of it is at the float32 floor of this grid
+    from oracle import fluid as F
+    p = sim.download("pressure")
+    assert np.isfinite(p).all()
+    res = F.relative_residual(p, sim.download("divergence"),
+                              F.obstacles_from_mask(sim.download("mask")))
+    assert res < 5e-4
+    sim.destroy()
+
+
+def test_trained_network_reduces_the_work_to_the_tolerance(gpu):
+    """Unseen scene (cylinder at (0.4, 0.5) N, r = 0.1 N, 256^2): sweeps and
+    time to FP64 relative residual <= 1e-4 with the trained network against
+    the plain Jacobi correction solves and against the same cycle without the
+    network (coarse smoothing from zero)."""
+    import time
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    res = {}
+    for name, guess in (("learned", LearnedGuess(net, meta.get("size", 128))),
+                        ("no_net", LearnedGuess(None, meta.get("size", 128))),
+                        ("plain", None)):
+        sim = cylinder_sim(256, warm=5)
+        solve_step(sim, guess, 1 / 60)  # first-call costs outside the clock
+        sim.synchronize()
+        t0 = time.perf_counter()
+        st = solve_step(sim, guess, 1 / 60)
+        sim.synchronize()
+        res[name] = (st, time.perf_counter() - t0)
+        assert st["converged"] and st["residual"] <= 1e-4, (name, st)
+        sim.destroy()
+    print({k: (v[0]["sweeps"], round(v[1] * 1e3, 2)) for k, v in res.items()})
+    assert res["learned"][0]["sweeps"] * 3 <= res["plain"][0]["sweeps"]
+    assert res["learned"][0]["sweeps"] <= res["no_net"][0]["sweeps"]
