@@ -1,18 +1,21 @@
 """Benchmark of the Natrix fluid step on one MI355X.
 
-Default workload: a 2048x2048 float32 scene (velocity, pressure, dye off)
-with a circular obstacle (seeded centre in [0.3, 0.7]^2, radius 0.1*N, held
-in the persistent obstacle map), inflow 1.0 on the left wall, vorticity
-confinement 0.3, no viscosity, dt = 1/60, speed 500, 80 Jacobi sweeps per
-step from a zero guess.  One step = one FluidSimulator.update(dt) on inputs
-already resident in HBM.
+Scene: a 2048x2048 float32 grid (velocity, pressure, dye off) with a
+circular obstacle (seeded centre in [0.3, 0.7]^2, radius 0.1*N, held in the
+persistent obstacle map), inflow 1.0 on the left wall, vorticity
+confinement 0.3, no viscosity, dt = 1/60, speed 500, developed by 10
+fixed-iteration steps (80 Jacobi sweeps each).  Inputs are resident in HBM.
 
-Prints one JSON line (steps/s, roofline of the dominant Jacobi kernel
-measured with HIP events on the simulator's stream, and the CPU oracle timed
-on a bounded sample of the same workload).
+Default workload (`--workload learned`): steps whose pressure is solved to
+FP64 relative residual <= 1e-4.  `value` is the steps/s of the learned
+two-grid cycle + Jacobi; the same criterion with plain Jacobi correction
+solves is timed on one step (capped; its rate is then an upper bound).
+The JSON line also carries the roofline of the Jacobi kernel (HIP events on
+the simulator's stream, fixed-sweep steps), the plain fused step at fixed
+iterations, and the CPU oracle timed on a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--size N] [--iters I]
-                    [--mode fixed|tol] [--tol 1e-4] [--tile k,rows,waves]
+    python bench.py [--steps K] [--warmup W] [--size N] [--tol 1e-4]
+                    [--workload learned|single|batch|mixed] [--no-plain] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -265,10 +268,16 @@ def bench_mixed(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_learned(args):
-    """Steps/s at rel. residual <= tol: learned guess + Jacobi vs plain
-    fused Jacobi from zero, on the cylinder/inflow scene after a warm-up of
-    fixed-iteration steps that develops the flow."""
+def bench_learned(args, copy_bandwidth):
+    """The headline workload: fluid steps at N^2 whose pressure is solved to
+    FP64 relative residual <= tol, with the learned two-grid cycle + Jacobi
+    (timed: `value`), against plain Jacobi correction solves under the same
+    criterion (one step, capped at `plain_cap` sweeps: when the cap is hit
+    its rate is an upper bound and the speedup a lower bound).  The roofline
+    of the Jacobi kernel comes from a fixed-sweep phase of the plain fused
+    step on the same grid (exact sweep count per launch); the CPU oracle is
+    timed on fixed-sweep steps of the same scene and scaled by the plain
+    path's sweeps per step."""
     import torch
     from natrix_amd import FluidSimulator
     from natrix_amd.learned.solver import LearnedGuess, solve_step
@@ -277,78 +286,139 @@ def bench_learned(args):
     net, meta = load(args.weights or DEFAULT_WEIGHTS)
     guess = LearnedGuess(net, coarse=meta.get("size", 128))
     solid, centre = make_scene(N)
+    dt = 1.0 / 60.0
+    cells = N * N
 
     def fresh():
         sim = FluidSimulator(N, N)
         sim.vorticity = 0.3
         sim.viscosity = 0.0
-        sim.iterations = 80
+        sim.iterations = args.iters
         sim.bc_mode = "inflow"
+        sim.inflow = 1.0
         sim.set_obstacle_map(solid)
         for _ in range(args.warmup):
-            sim.update(1 / 60)
+            sim.update(dt)
         sim.synchronize()
         return sim
 
-    res = {}
-    # learned + Jacobi
+    log(f"bench learned: {N}x{N}, warmup {args.warmup} fixed steps (I={args.iters})")
     sim = fresh()
-    sweeps, conv, t_l = [], [], 0.0
+    for _ in range(2):  # first-call costs (allocator, code objects) off the clock
+        solve_step(sim, guess, dt, tol=args.tol)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    sim.set_timing(True)
+    sim.collect_timing()
+    sweeps, outer, conv = [], [], []
+    t0 = time.perf_counter()
     for i in range(args.steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        st = solve_step(sim, guess, 1 / 60, tol=args.tol)
-        sim.synchronize()
-        t_l += time.perf_counter() - t0
+        st = solve_step(sim, guess, dt, tol=args.tol)
         sweeps.append(st["sweeps"])
-        conv.append(st["converged"])
-        log(f"learned step {i}: {st}")
+        outer.append(st["outer"])
+        conv.append(st["converged"] and st["residual"] <= args.tol)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    timing = sim.collect_timing()
+    sim.set_timing(False)
     sim.destroy()
-    res["learned"] = {"steps_per_s": round(args.steps / t_l, 4),
-                      "ms_per_step": round(1e3 * t_l / args.steps, 3),
-                      "sweeps_per_step": float(np.mean(sweeps)),
-                      "all_converged": bool(all(conv))}
-    # plain Jacobi (same refinement, no learned corrections), same criterion
-    # (fewer steps: it is slow; the sweep cap makes its rate a bound)
+    learned = {"steps_per_s": round(args.steps / el, 3), "ms_per_step": round(1e3 * el / args.steps, 3),
+               "sweeps_per_step": float(np.mean(sweeps)), "outer_per_step": float(np.mean(outer)),
+               "all_converged": bool(all(conv)),
+               "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in timing.items() if v[1]}}
+    log(f"bench learned: {learned}")
+
+    plain = None
+    if not args.no_plain:
+        sim = fresh()
+        t0 = time.perf_counter()
+        st = solve_step(sim, None, dt, tol=args.tol, max_inner_sweeps=args.plain_cap,
+                        max_outer=args.plain_outer)
+        sim.synchronize()
+        t_p = time.perf_counter() - t0
+        sim.destroy()
+        done = st["converged"] and st["residual"] <= args.tol
+        plain = {"steps_per_s": round(1.0 / t_p, 6), "ms_per_step": round(1e3 * t_p, 1),
+                 "sweeps_per_step": st["sweeps"], "converged": bool(done),
+                 "residual_reached": st["residual"], "sweep_cap_per_correction_solve": args.plain_cap,
+                 "rate_is_upper_bound": not done}
+        log(f"bench plain: {plain}")
+
+    # fixed-sweep plain fused step: exact sweeps per Jacobi launch
     sim = fresh()
-    psteps = max(1, min(args.steps, args.plain_steps))
-    sweeps, conv, t_p, resid = [], [], 0.0, []
-    for i in range(psteps):
-        t0 = time.perf_counter()
-        st = solve_step(sim, None, 1 / 60, tol=args.tol, max_inner_sweeps=args.plain_cap)
-        sim.synchronize()
-        t_p += time.perf_counter() - t0
-        sweeps.append(st["sweeps"])
-        conv.append(st["converged"])
-        resid.append(st["residual"])
-        log(f"plain step {i}: {st}")
+    sim.set_timing(True)
+    sim.collect_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.fixed_steps):
+        sim.update(dt)
+    sim.synchronize()
+    el_f = time.perf_counter() - t0
+    tf = sim.collect_timing()
+    sim.set_timing(False)
+    tile = list(sim.jacobi_tile)
     sim.destroy()
-    res["plain"] = {"steps_per_s": round(psteps / t_p, 5),
-                    "ms_per_step": round(1e3 * t_p / psteps, 3),
-                    "sweeps_per_step": float(np.mean(sweeps)),
-                    "all_converged": bool(all(conv)), "final_residuals": resid,
-                    "sweep_cap_per_correction_solve": args.plain_cap}
-    speedup = res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"]
-    out = {"metric": METRIC, "value": res["learned"]["steps_per_s"], "unit": "steps/s",
-           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": res["learned"]["ms_per_step"], "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32 (U-Net bf16)",
+    jac_ms, jac_n = tf["jacobi"]
+    tb_sweeps = (args.iters - 1) * args.fixed_steps  # sweep 1 runs in the fused divergence kernel
+    ach = JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps / (jac_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_jacobi_tb",
+            "avg_launch_us": round(1e3 * jac_ms / jac_n, 3),
+            "sweeps_per_launch": round(tb_sweeps / jac_n, 3),
+            "algorithmic_bytes_per_launch": int(JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps / jac_n)}
+    ms_f = 1e3 * el_f / args.fixed_steps
+    step_bytes = cells * (STEP_BYTES_PER_CELL + JACOBI_BYTES_PER_CELL_SWEEP * args.iters)
+    fixed = {"steps_per_s": round(args.fixed_steps / el_f, 2), "ms_per_step": round(ms_f, 4),
+             "iterations": args.iters, "jacobi_tile": tile,
+             "step_algorithmic_GBps": round(step_bytes / (ms_f * 1e-3) / 1e9, 1),
+             "step_roofline_frac": round(step_bytes / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "stage_ms_per_step": {k: round(v[0] / args.fixed_steps, 5) for k, v in tf.items() if v[1]}}
+    log(f"bench fixed: {fixed} roofline {roof}")
+    copy_gbs = copy_bandwidth(1 << 30, 20)
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU oracle baseline")
+        sps, nst, el_c = cpu_baseline(N, args.iters, args.cpu_budget)
+        cpu_sweeps_per_s = sps * args.iters
+        plain_sweeps = plain["sweeps_per_step"] if plain else None
+        cpu = {"value": (float(f"{cpu_sweeps_per_s / plain_sweeps:.4g}") if plain_sweeps else None),
+               "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/fluid.py step() on the same {N}x{N} scene with {args.iters} "
+                         f"Jacobi sweeps: {nst} steps in {el_c:.1f} s = {cpu_sweeps_per_s:.1f} "
+                         f"sweeps/s (NumPy float32, one thread; host reports {os.cpu_count()} "
+                         f"cpus); value = that rate / the plain path's sweeps per step to the "
+                         f"tolerance ({plain_sweeps}"
+                         + (", a lower bound: capped" if plain and plain["rate_is_upper_bound"] else "")
+                         + ") = the CPU's plain-Jacobi steps/s at the tolerance",
+               "fixed_iteration_steps_per_s": round(sps, 4)}
+
+    speedup = (learned["steps_per_s"] / plain["steps_per_s"]) if plain else None
+    out = {"metric": METRIC, "value": learned["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": learned["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
-           "config": {"workload": f"fluid step {N}x{N} f32, cylinder r=0.1N, left inflow, "
-                                  f"vorticity 0.3, pressure to rel. residual <= {args.tol:g}",
-                      "grid": N, "solver": "learned guess + Jacobi vs plain Jacobi"},
-           "learned": res["learned"], "plain": res["plain"],
-           "speedup_learned_vs_plain": round(speedup, 2),
-           "plain_lower_bound": not res["plain"]["all_converged"],
-           "roofline": None, "cpu_baseline": None}
+           "config": {"workload": f"fluid step {N}x{N} f32, cylinder r=0.1N at "
+                                  f"({centre[0]:.3f},{centre[1]:.3f}), left inflow 1.0, vorticity 0.3, "
+                                  f"pressure to FP64 rel. residual <= {args.tol:g}",
+                      "grid": N, "solver": "learned two-grid cycle (U-Net bf16 MFMA guess, "
+                                           "Chebyshev coarse smoothing) + Jacobi, FP64 refinement",
+                      "weights": os.path.basename(args.weights or DEFAULT_WEIGHTS)},
+           "learned": learned, "plain": plain,
+           "speedup_learned_vs_plain": round(speedup, 1) if speedup else None,
+           "speedup_is_lower_bound": bool(plain and plain["rate_is_upper_bound"]),
+           "plain_fixed_iterations": fixed, "copy_probe_GBps": round(copy_gbs, 1),
+           "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 20 learned, 1000 single/batch/mixed)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default 10 learned, 20 otherwise)")
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--iters", type=int, default=80)
     ap.add_argument("--mode", choices=["fixed", "tol"], default="fixed")
@@ -360,18 +430,28 @@ def main():
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batch", "mixed", "learned"],
-                    default="single",
-                    help="batch: 64 uneven scenes in one launch per stage; "
-                         "mixed: 4096^2 with bf16/f16 storage")
+    ap.add_argument("--workload", choices=["learned", "single", "batch", "mixed"],
+                    default="learned",
+                    help="learned: steps to rel. residual <= tol, learned two-grid cycle "
+                         "vs plain Jacobi (the headline); single: the plain fused step at "
+                         "fixed iterations; batch: 64 uneven scenes in one launch per "
+                         "stage; mixed: 4096^2 with bf16/f16 storage")
     ap.add_argument("--storage", choices=["bf16", "f16"], default="bf16")
     ap.add_argument("--weights", default=None, help="U-Net weights (learned workload)")
-    ap.add_argument("--plain-cap", type=int, default=4_000_000,
+    ap.add_argument("--plain-cap", type=int, default=1_000_000,
                     help="sweep cap per correction solve of the plain path (learned workload)")
-    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--plain-outer", type=int, default=2,
+                    help="refinement iterations of the capped plain step (learned workload)")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--fixed-steps", type=int, default=100,
+                    help="fixed-iteration steps timed for the Jacobi roofline (learned workload)")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the copy-bandwidth probe (counter calibration)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 20 if args.workload == "learned" else 1000
+    if args.warmup is None:
+        args.warmup = 10 if args.workload == "learned" else 20
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the fluid "
                           "step of one grid is not sharded (run with --gpus 1)"}))
@@ -398,7 +478,7 @@ def main():
     if args.workload == "mixed":
         return bench_mixed(args)
     if args.workload == "learned":
-        return bench_learned(args)
+        return bench_learned(args, copy_bandwidth)
 
     N = args.size
     solid, centre = make_scene(N)

@@ -85,6 +85,8 @@ struct nx_sim {
   float* pl = nullptr;
   float* b0 = nullptr;
   float* dcorr = nullptr;   // prolonged coarse correction (line search)
+  float* rrow = nullptr;    // restriction row records (2 x H x Wc)
+  size_t rrow_cap = 0;
   double* line = nullptr;   // {alpha, d.r, d.Ld, 0}
   bool refining = false;
   // step phases (nx_step_begin .. nx_step_finish)
@@ -507,36 +509,50 @@ static int step_fused_enqueue(nx_sim* s, float dt, bool tolerance) {
 // [fI, fI+f) x [fJ, fJ+f).  rhs_c = f^2 * mean over its fluid fine cells of
 // the float32 residual r = b - A p (the correction equation in coarse grid
 // units); mask_c = 1 only when every one of its fine cells is solid, so the
-// coarse fluid region covers the fine one.
+// coarse fluid region covers the fine one.  Two passes: per fine row y and
+// coarse column I, the sum and fluid count of r over the row's f cells
+// (row-major lanes, one H x Wc record each), then per coarse cell the sum of
+// its f row records.
 __global__ void __launch_bounds__(256)
-    k_restrict(const float* __restrict__ p, const float* __restrict__ b,
-               const uint8_t* __restrict__ mask, float* __restrict__ rc,
-               uint8_t* __restrict__ mc, int W, int H, int f, int Wc, int Hc) {
-  const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y * 4 + threadIdx.y;
-  if (X >= Wc || Y >= Hc) return;
+    k_restrict_rows(const float* __restrict__ p, const float* __restrict__ b,
+                    const uint8_t* __restrict__ mask, float* __restrict__ rsum,
+                    float* __restrict__ rcnt, int W, int H, int f, int Wc) {
+  const int X = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  if (X >= Wc || y >= H) return;
   float sum = 0.0f;
   int fluid = 0;
-  for (int j = 0; j < f; ++j) {
-    const int y = Y * f + j;
-    if (y >= H) break;
-    for (int i = 0; i < f; ++i) {
-      const int x = X * f + i;
-      if (x >= W) break;
-      const int pos = y * W + x;
-      if (mask[pos]) continue;
-      const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
-      const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
-      const float pc = p[pos];
-      const float x1 = mask[iL] ? pc : p[iL];
-      const float x2 = mask[iR] ? pc : p[iR];
-      const float y1 = mask[iB] ? pc : p[iB];
-      const float y2 = mask[iT] ? pc : p[iT];
-      sum += b[pos] - ((((x1 + x2) + y1) + y2) - 4.0f * pc);
-      ++fluid;
-    }
+  const int x0 = X * f, x1 = min(x0 + f, W);
+  const int rowB = max(y - 1, 0) * W, rowT = min(y + 1, H - 1) * W, row = y * W;
+  for (int x = x0; x < x1; ++x) {
+    const int pos = row + x;
+    if (mask[pos]) continue;
+    const int iL = row + max(x - 1, 0), iR = row + min(x + 1, W - 1);
+    const int iB = rowB + x, iT = rowT + x;
+    const float pc = p[pos];
+    const float xl = mask[iL] ? pc : p[iL];
+    const float xr = mask[iR] ? pc : p[iR];
+    const float yb = mask[iB] ? pc : p[iB];
+    const float yt = mask[iT] ? pc : p[iT];
+    sum += b[pos] - ((((xl + xr) + yb) + yt) - 4.0f * pc);
+    ++fluid;
   }
-  rc[Y * Wc + X] = fluid == 0 ? 0.0f : (float)(f * f) * (sum / (float)fluid);
-  mc[Y * Wc + X] = fluid == 0 ? 1 : 0;
+  rsum[y * Wc + X] = sum;
+  rcnt[y * Wc + X] = (float)fluid;
+}
+
+__global__ void __launch_bounds__(256)
+    k_restrict_cols(const float* __restrict__ rsum, const float* __restrict__ rcnt,
+                    float* __restrict__ rc, uint8_t* __restrict__ mc, int H, int f,
+                    int Wc, int Hc) {
+  const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y * 4 + threadIdx.y;
+  if (X >= Wc || Y >= Hc) return;
+  float sum = 0.0f, cnt = 0.0f;
+  for (int y = Y * f; y < min(Y * f + f, H); ++y) {
+    sum += rsum[y * Wc + X];
+    cnt += rcnt[y * Wc + X];
+  }
+  rc[Y * Wc + X] = cnt == 0.0f ? 0.0f : (float)(f * f) * (sum / cnt);
+  mc[Y * Wc + X] = cnt == 0.0f ? 1 : 0;
 }
 
 // Prolongation: p += scale * e interpolated at the fine cell, on fluid
@@ -743,23 +759,24 @@ __global__ void __launch_bounds__(256)
     k_line_terms(const float* __restrict__ p, const float* __restrict__ b,
                  const uint8_t* __restrict__ mask, const float* __restrict__ d,
                  double* __restrict__ red, int W, int H) {
-  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  const int x = blockIdx.x * 64 + threadIdx.x;
   double v0 = 0.0, v1 = 0.0;
-  if (x < W && y < H) {
+  for (int r = 0; r < RES_ROWS / 4; ++r) {
+    const int y = blockIdx.y * RES_ROWS + threadIdx.y + 4 * r;
+    if (x >= W || y >= H) continue;
     const int pos = y * W + x;
-    if (mask[pos] == 0) {
-      const int nb[4] = {y * W + max(x - 1, 0), y * W + min(x + 1, W - 1),
-                         max(y - 1, 0) * W + x, min(y + 1, H - 1) * W + x};
-      const double pc = (double)p[pos], dc = (double)d[pos];
-      double lp = 0.0, ld = 0.0;
-      for (int j = 0; j < 4; ++j) {
-        if (mask[nb[j]]) continue;
-        lp += (double)p[nb[j]] - pc;
-        ld += (double)d[nb[j]] - dc;
-      }
-      v0 = dc * ((double)b[pos] - lp);
-      v1 = dc * ld;
+    if (mask[pos] != 0) continue;
+    const int nb[4] = {y * W + max(x - 1, 0), y * W + min(x + 1, W - 1),
+                       max(y - 1, 0) * W + x, min(y + 1, H - 1) * W + x};
+    const double pc = (double)p[pos], dc = (double)d[pos];
+    double lp = 0.0, ld = 0.0;
+    for (int j = 0; j < 4; ++j) {
+      if (mask[nb[j]]) continue;
+      lp += (double)p[nb[j]] - pc;
+      ld += (double)d[nb[j]] - dc;
     }
+    v0 += dc * ((double)b[pos] - lp);
+    v1 += dc * ld;
   }
   __shared__ double part[4][2];
   const int lane = threadIdx.x, wave = threadIdx.y;
@@ -781,14 +798,14 @@ __global__ void __launch_bounds__(256)
 
 // line = {a, d.r, d.Ld, 0}; a = 0 when d carries no energy (d.Ld >= 0) or
 // either sum is not finite (a NaN or infinite direction)
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     k_line_alpha(const double* __restrict__ red, int nblocks, double* __restrict__ line) {
   double v0 = 0.0, v1 = 0.0;
-  for (int i = threadIdx.x; i < nblocks; i += 256) {
+  for (int i = threadIdx.x; i < nblocks; i += 1024) {
     v0 += red[2 * i];
     v1 += red[2 * i + 1];
   }
-  __shared__ double part[4][2];
+  __shared__ double part[16][2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int off = 32; off > 0; off >>= 1) {
     v0 += __shfl_xor(v0, off, 64);
@@ -800,8 +817,11 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double dr = ((part[0][0] + part[1][0]) + part[2][0]) + part[3][0];
-    const double dld = ((part[0][1] + part[1][1]) + part[2][1]) + part[3][1];
+    double dr = 0.0, dld = 0.0;
+    for (int w = 0; w < 16; ++w) {
+      dr += part[w][0];
+      dld += part[w][1];
+    }
     const bool ok = isfinite(dr) && isfinite(dld) && dld < 0.0;
     line[0] = ok ? dr / dld : 0.0;
     line[1] = dr;
@@ -833,9 +853,11 @@ __global__ void __launch_bounds__(256)
                       const float* __restrict__ b0, const uint8_t* __restrict__ mask,
                       float* __restrict__ r_out, float* __restrict__ p1,
                       double* __restrict__ red, int W, int H) {
-  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  const int x = blockIdx.x * 64 + threadIdx.x;
   double v[5] = {0, 0, 0, 0, 0};
-  if (x < W && y < H) {
+  for (int rr_ = 0; rr_ < RES_ROWS / 4; ++rr_) {
+    const int y = blockIdx.y * RES_ROWS + threadIdx.y + 4 * rr_;
+    if (x >= W || y >= H) continue;
     const int pos = y * W + x;
     const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
     const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
@@ -850,11 +872,11 @@ __global__ void __launch_bounds__(256)
     r_out[pos] = rf;
     p1[pos] = (0.0f - rf) * 0.25f;
     if (mask[pos] == 0) {
-      v[0] = rr;
-      v[1] = rr * rr;
-      v[2] = bd;
-      v[3] = bd * bd;
-      v[4] = 1.0;
+      v[0] += rr;
+      v[1] += rr * rr;
+      v[2] += bd;
+      v[3] += bd * bd;
+      v[4] += 1.0;
     }
   }
   __shared__ double part[4][5];
@@ -1009,6 +1031,7 @@ int nx_destroy(nx_sim* s) {
   (void)hipFree(s->pl);
   (void)hipFree(s->b0);
   (void)hipFree(s->dcorr);
+  (void)hipFree(s->rrow);
   (void)hipFree(s->line);
   (void)hipFree(s->curl);
   (void)hipFree(s->red5);
@@ -1277,9 +1300,20 @@ int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
     return fail(NX_ERR_ARG, "bad restriction arguments");
   if (!s->in_step) return fail(NX_ERR_STATE, "restriction outside a step");
   const int Wc = (s->W + factor - 1) / factor, Hc = (s->H + factor - 1) / factor;
-  k_restrict<<<dim3((Wc + 63) / 64, (Hc + 3) / 4), dim3(64, 4), 0, s->stream>>>(
-      s->p[s->pcur], s->div, s->mask[s->mcur], coarse_rhs, coarse_mask, s->W, s->H,
-      factor, Wc, Hc);
+  const size_t need = (size_t)2 * s->H * Wc;
+  if (need > s->rrow_cap) {
+    if (s->rrow) NX_HIP(hipFree(s->rrow));
+    s->rrow = nullptr;
+    NX_HIP(hipMalloc(&s->rrow, need * sizeof(float)));
+    s->rrow_cap = need;
+  }
+  float* rsum = s->rrow;
+  float* rcnt = s->rrow + (size_t)s->H * Wc;
+  k_restrict_rows<<<dim3((Wc + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+      s->p[s->pcur], s->div, s->mask[s->mcur], rsum, rcnt, s->W, s->H, factor, Wc);
+  NX_HIP(hipGetLastError());
+  k_restrict_cols<<<dim3((Wc + 63) / 64, (Hc + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+      rsum, rcnt, coarse_rhs, coarse_mask, s->H, factor, Wc, Hc);
   NX_HIP(hipGetLastError());
   return NX_OK;
 }
@@ -1339,12 +1373,12 @@ static int ensure_line(nx_sim* s) {
 
 static int line_search(nx_sim* s, const float* d, double* out) {
   const size_t n = (size_t)s->W * s->H;
-  const dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
+  const dim3 grid((s->W + 63) / 64, (s->H + RES_ROWS - 1) / RES_ROWS);
   // red5 holds 5 doubles per block of this grid; the line terms use 2
   k_line_terms<<<grid, dim3(64, 4), 0, s->stream>>>(s->p[s->pcur], s->div, s->mask[s->mcur],
                                                     d, s->red5, s->W, s->H);
   NX_HIP(hipGetLastError());
-  k_line_alpha<<<1, 256, 0, s->stream>>>(s->red5, residual_blocks(s->W, s->H), s->line);
+  k_line_alpha<<<1, 1024, 0, s->stream>>>(s->red5, residual_blocks(s->W, s->H), s->line);
   NX_HIP(hipGetLastError());
   k_line_apply<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(
       s->p[s->pcur], d, s->mask[s->mcur], s->line, (int)n);
@@ -1398,7 +1432,7 @@ int nx_refine_begin(nx_sim* s) {
 int nx_refine_residual(nx_sim* s, double* rel) {
   if (!s || !rel) return fail(NX_ERR_ARG, "null argument");
   if (!s->refining) return fail(NX_ERR_STATE, "nx_refine_begin first");
-  const dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
+  const dim3 grid((s->W + 63) / 64, (s->H + RES_ROWS - 1) / RES_ROWS);
   k_refine_residual<<<grid, dim3(64, 4), 0, s->stream>>>(
       s->ph, s->pl, s->b0, s->mask[s->mcur], s->div, s->p[0], s->red5, s->W, s->H);
   NX_HIP(hipGetLastError());

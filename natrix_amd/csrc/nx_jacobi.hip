@@ -47,31 +47,34 @@ __global__ void __launch_bounds__(256)
     k_sums(const float* __restrict__ p, const float* __restrict__ b,
            const uint8_t* __restrict__ mask, int W, int H,
            double* __restrict__ red) {
+  // a block covers 64 x RES_ROWS cells: wave w takes rows w, w+4, ... so
+  // every load instruction reads one contiguous row segment
   const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
   double v[5] = {0, 0, 0, 0, 0};
-  if (x < W && y < H) {
+#pragma unroll 2
+  for (int r = 0; r < RES_ROWS / 4; ++r) {
+    const int y = blockIdx.y * RES_ROWS + threadIdx.y + 4 * r;
+    if (x >= W || y >= H) continue;
     const int pos = y * W + x;
-    if (mask[pos] == 0) {
-      const double bd = (double)b[pos];
-      double rr = bd;
-      if (p != nullptr) {
-        const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
-        const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
-        const float pc = p[pos];
-        const float x1 = mask[iL] ? pc : p[iL];
-        const float x2 = mask[iR] ? pc : p[iR];
-        const float y1 = mask[iB] ? pc : p[iB];
-        const float y2 = mask[iT] ? pc : p[iT];
-        rr = bd - (((((double)x1 + (double)x2) + (double)y1) + (double)y2) -
-                   4.0 * (double)pc);
-      }
-      v[0] = rr;
-      v[1] = rr * rr;
-      v[2] = bd;
-      v[3] = bd * bd;
-      v[4] = 1.0;
+    if (mask[pos] != 0) continue;
+    const double bd = (double)b[pos];
+    double rr = bd;
+    if (p != nullptr) {
+      const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
+      const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
+      const float pc = p[pos];
+      const float x1 = mask[iL] ? pc : p[iL];
+      const float x2 = mask[iR] ? pc : p[iR];
+      const float y1 = mask[iB] ? pc : p[iB];
+      const float y2 = mask[iT] ? pc : p[iT];
+      rr = bd - (((((double)x1 + (double)x2) + (double)y1) + (double)y2) -
+                 4.0 * (double)pc);
     }
+    v[0] += rr;
+    v[1] += rr * rr;
+    v[2] += bd;
+    v[3] += bd * bd;
+    v[4] += 1.0;
   }
   __shared__ double part[4][5];
   const int lane = threadIdx.x, wave = threadIdx.y;
@@ -101,24 +104,30 @@ __device__ double rel_from(const double* s) {
 // residual.  If state != nullptr (tolerance-driven solve): on the first
 // time rel <= tol it sets state[0] = 1 (done), state[1] = sweeps_now and
 // state[2] = final_idx; state[3] always records sweeps_now.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     k_reduce(const double* __restrict__ part, int nblocks, int width, int off,
              double* __restrict__ sums, double tol, int* __restrict__ state,
              int sweeps_now, int final_idx) {
-  __shared__ double acc[256];
-  for (int j = 0; j < width; ++j) {
-    double a = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += 256) a += part[i * width + j];
-    acc[threadIdx.x] = a;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if ((int)threadIdx.x < st) acc[threadIdx.x] += acc[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) sums[off + j] = acc[0];
-    __syncthreads();
-  }
+  // one pass over the records: thread t adds records t, t + 1024, ... in
+  // order, then the 16 wave sums are added in wave order (deterministic)
+  double a[5] = {0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < nblocks; i += 1024)
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      if (j < width) a[j] += part[i * width + j];
+  __shared__ double acc[16][5];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) a[j] = wave_sum(a[j]);
+  if (lane == 0)
+    for (int j = 0; j < 5; ++j) acc[wave][j] = a[j];
+  __syncthreads();
   if (threadIdx.x == 0) {
+    for (int j = 0; j < width; ++j) {
+      double t = 0.0;
+      for (int w = 0; w < 16; ++w) t += acc[w][j];
+      sums[off + j] = t;
+    }
     const double rel = rel_from(sums);
     sums[5] = rel;
     if (state != nullptr) {
@@ -216,11 +225,13 @@ hipError_t launch_center_rhs(float* b, float* p, const uint8_t* mask, int n,
   return hipGetLastError();
 }
 
-int residual_blocks(int W, int H) { return ((W + 63) / 64) * ((H + 3) / 4); }
+int residual_blocks(int W, int H) {
+  return ((W + 63) / 64) * ((H + RES_ROWS - 1) / RES_ROWS);
+}
 
 hipError_t launch_sums(const float* p, const float* b, const uint8_t* mask,
                        int W, int H, double* red, hipStream_t s) {
-  dim3 grid((W + 63) / 64, (H + 3) / 4, 1);
+  dim3 grid((W + 63) / 64, (H + RES_ROWS - 1) / RES_ROWS, 1);
   k_sums<<<grid, dim3(64, 4), 0, s>>>(p, b, mask, W, H, red);
   return hipGetLastError();
 }
@@ -228,8 +239,8 @@ hipError_t launch_sums(const float* p, const float* b, const uint8_t* mask,
 hipError_t launch_reduce(const double* part, int nblocks, int width, int off,
                          double* sums, double tol, int* state, int sweeps_now,
                          int final_idx, hipStream_t s) {
-  k_reduce<<<1, 256, 0, s>>>(part, nblocks, width, off, sums, tol, state,
-                             sweeps_now, final_idx);
+  k_reduce<<<1, 1024, 0, s>>>(part, nblocks, width, off, sums, tol, state,
+                              sweeps_now, final_idx);
   return hipGetLastError();
 }
 

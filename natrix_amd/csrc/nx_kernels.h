@@ -80,7 +80,9 @@ hipError_t launch_jacobi_tb(const float* pin, const float* b,
 bool jacobi_cfg_supported(JacobiCfg cfg);
 
 // Exact FP64 residual sums of `p` (of a zero guess when p == nullptr) per
-// 64x4 block: red[5*blk + {sum r, sum r^2, sum b, sum b^2, fluid count}].
+// 64 x RES_ROWS block of cells (256 threads, RES_ROWS / 4 rows each):
+// red[5*blk + {sum r, sum r^2, sum b, sum b^2, fluid count}].
+constexpr int RES_ROWS = 32;
 int residual_blocks(int W, int H);
 hipError_t launch_sums(const float* p, const float* b, const uint8_t* mask,
                        int W, int H, double* red, hipStream_t s);

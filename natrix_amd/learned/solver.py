@@ -128,8 +128,8 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
     """Solve the correction equation (current right-hand side, iterate
     from its first sweep) to relative residual `target`: learned cycles
     while each reduces the residual below `stall` x the previous one, then
-    plain Jacobi chunks until the target, the sweep cap, or a chunk that
-    reduces it by less than 1 - `plain_stall`.  Returns (cycles whose line
+    Jacobi chunks until the target, the sweep cap, or (after learned
+    cycles) a chunk that reduces it by less than 1 - `plain_stall`.  Returns (cycles whose line
     search took a nonzero step, True if the learned cycles were abandoned
     before the target)."""
     kept, abandoned = 0, False
@@ -154,13 +154,16 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
             trace.append(("plain", st["sweeps"], st["residual"]))
         if st["converged"] or st["sweeps"] - start >= max_sweeps:
             break
-        if before is not None and st["residual"] > plain_stall * before:
+        # the learned path's fallback chunks stop at a stall (the float32
+        # floor of a correction solve); plain correction solves run to the
+        # target or the sweep cap, as the reference's Jacobi would
+        if guess is not None and before is not None and st["residual"] > plain_stall * before:
             break
         before = st["residual"]
     return kept, abandoned
 
 
-def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.03, smooth=256, max_cycles=6,
+def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=128, max_cycles=6,
                max_outer=12, chunk=65536, max_inner_sweeps=4_000_000, trace=None):
     """One fluid step whose pressure is solved to FP64 relative residual
     <= tol by iterative refinement (pressure held as a float32 pair, outer
