@@ -168,6 +168,11 @@ int nx_prolong_line_search(nx_sim* sim, int factor, const float* coarse,
  * operator on [lambda_min, 8] (-L has its spectrum in [0, 8]) */
 int nx_coarse_smooth(nx_sim* sim, int Wc, int Hc, float* e, const float* rc,
                      const uint8_t* coarse_mask, int iterations, float lambda_min);
+/* `sweeps` temporal-blocked Jacobi sweeps of the current pressure system,
+ * then (omega > 0) one damped sweep, then the FP64 relative residual; one
+ * synchronisation.  out (host, may be NULL) = {relative residual, the last
+ * line search's step a}.  Counted in the step's sweeps. */
+int nx_smooth(nx_sim* sim, int sweeps, float omega, double* out);
 /* `count` damped Jacobi sweeps p <- p + omega (J(p) - p) of the current
  * pressure system (0 < omega <= 1; omega = 1/2 removes the checkerboard
  * modes the plain sweep cannot damp); counted in the step's sweeps */
@@ -282,6 +287,14 @@ int nx_batch_collect_timing(nx_batch* b, double* ms, int* launches);
 /* y = [relu](conv3x3(x, w) + bias), zero padding; cin % 8 == 0, cout % 32 == 0 */
 int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                    int h, int wd, int cin, int cout, int relu, void* stream);
+/* the same convolution with the 9 cin / 16 K steps split over `splits`
+ * groups of launches that write float32 partial sums to
+ * work[splits * n*h*w * cout] (device), then summed in split order with
+ * bias and ReLU: more workgroups for small images with many channels
+ * (cin % 16 == 0) */
+int nx_conv3x3_fwd_split(const void* x, const void* w, const float* bias, void* y,
+                         float* work, int splits, int n, int h, int wd, int cin, int cout,
+                         int relu, void* stream);
 /* dx = conv3x3 input gradient of dy; wt: scratch of 9*cin*cout bfloat16 */
 int nx_conv3x3_dgrad(const void* dy, const void* w, void* dx, void* wt, int n, int h,
                      int wd, int cin, int cout, void* stream);

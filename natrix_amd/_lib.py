@@ -139,6 +139,7 @@ SIGNATURES = {
     "nx_prolong_line_search": [_P, _I, _P, _P, _P],
     "nx_line_search": [_P, _P, _P],
     "nx_damped_sweeps": [_P, _I, _F],
+    "nx_smooth": [_P, _I, _F, _P],
     "nx_coarse_smooth": [_P, _I, _I, _P, _P, _P, _I, _F],
     "nx_refine_begin": [_P],
     "nx_refine_residual": [_P, C.POINTER(_D)],
@@ -146,6 +147,7 @@ SIGNATURES = {
     "nx_refine_end": [_P],
     "nx_step_finish": [_P],
     "nx_conv3x3_fwd": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
+    "nx_conv3x3_fwd_split": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
     "nx_conv3x3_dgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
     "nx_conv3x3_wgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
     "nx_relu_bwd": [_P, _P, _P, C.c_longlong, _P],

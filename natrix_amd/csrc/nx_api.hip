@@ -540,6 +540,43 @@ __global__ void __launch_bounds__(256)
   rcnt[y * Wc + X] = (float)fluid;
 }
 
+// The same row records when f is a power of two dividing 64: lane l owns
+// fine column 64 b + l, so every load is one contiguous row segment, and
+// the f lanes of a coarse column add their values with xor shuffles inside
+// their aligned group (lane order differs from the loop above only in the
+// float32 summation order).
+__global__ void __launch_bounds__(256)
+    k_restrict_rows_seg(const float* __restrict__ p, const float* __restrict__ b,
+                        const uint8_t* __restrict__ mask, float* __restrict__ rsum,
+                        float* __restrict__ rcnt, int W, int H, int f, int Wc) {
+  const int lane = threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  const int x = blockIdx.x * 64 + lane;
+  if (y >= H) return;  // whole waves: y is uniform across a wave
+  float v = 0.0f, c = 0.0f;
+  if (x < W) {
+    const int row = y * W, pos = row + x;
+    if (mask[pos] == 0) {
+      const int iL = row + max(x - 1, 0), iR = row + min(x + 1, W - 1);
+      const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
+      const float pc = p[pos];
+      const float xl = mask[iL] ? pc : p[iL];
+      const float xr = mask[iR] ? pc : p[iR];
+      const float yb = mask[iB] ? pc : p[iB];
+      const float yt = mask[iT] ? pc : p[iT];
+      v = b[pos] - ((((xl + xr) + yb) + yt) - 4.0f * pc);
+      c = 1.0f;
+    }
+  }
+  for (int off = f >> 1; off > 0; off >>= 1) {
+    v += __shfl_xor(v, off, 64);
+    c += __shfl_xor(c, off, 64);
+  }
+  if ((lane & (f - 1)) == 0 && x < W) {
+    rsum[y * Wc + x / f] = v;
+    rcnt[y * Wc + x / f] = c;
+  }
+}
+
 __global__ void __launch_bounds__(256)
     k_restrict_cols(const float* __restrict__ rsum, const float* __restrict__ rcnt,
                     float* __restrict__ rc, uint8_t* __restrict__ mc, int H, int f,
@@ -1294,6 +1331,36 @@ int nx_solve(nx_sim* s, double tol, int max_sweeps, nx_stats* out) {
   return NX_OK;
 }
 
+int nx_smooth(nx_sim* s, int sweeps, float omega, double* out) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (!s->in_step) return fail(NX_ERR_STATE, "nx_smooth outside nx_step_begin/finish");
+  if (sweeps < 0 || !(omega >= 0.0f && omega <= 1.0f))
+    return fail(NX_ERR_ARG, "sweeps >= 0 and 0 <= omega <= 1 required");
+  JacobiCfg c{s->prm.jacobi_k, s->prm.jacobi_rows, s->prm.jacobi_waves, s->prm.jacobi_cols};
+  NX_TRY(run_jacobi_tb(s, sweeps, c));
+  s->step_sweeps += sweeps;
+  if (omega > 0.0f) NX_TRY(nx_damped_sweeps(s, 1, omega));
+  NX_HIP(launch_sums(s->p[s->pcur], s->div, s->mask[s->mcur], s->W, s->H, s->red5,
+                     s->stream));
+  NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,
+                       nullptr, 0, 0, s->stream));
+  double rel = 0.0, alpha = 0.0;
+  NX_HIP(hipMemcpyAsync(&rel, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  if (s->line)
+    NX_HIP(hipMemcpyAsync(&alpha, s->line, sizeof(double), hipMemcpyDeviceToHost,
+                          s->stream));
+  NX_HIP(hipStreamSynchronize(s->stream));
+  s->stats.sweeps = s->step_sweeps;
+  s->stats.residual = rel;
+  s->stats.residual_sweeps = s->step_sweeps;
+  if (out) {
+    out[0] = rel;
+    out[1] = alpha;
+  }
+  return NX_OK;
+}
+
 int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
                          uint8_t* coarse_mask) {
   if (!s || !coarse_rhs || !coarse_mask || factor < 1)
@@ -1309,8 +1376,13 @@ int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
   }
   float* rsum = s->rrow;
   float* rcnt = s->rrow + (size_t)s->H * Wc;
-  k_restrict_rows<<<dim3((Wc + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0, s->stream>>>(
-      s->p[s->pcur], s->div, s->mask[s->mcur], rsum, rcnt, s->W, s->H, factor, Wc);
+  if (factor >= 2 && factor <= 64 && (factor & (factor - 1)) == 0)
+    k_restrict_rows_seg<<<dim3((s->W + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0,
+                          s->stream>>>(s->p[s->pcur], s->div, s->mask[s->mcur], rsum, rcnt,
+                                       s->W, s->H, factor, Wc);
+  else
+    k_restrict_rows<<<dim3((Wc + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+        s->p[s->pcur], s->div, s->mask[s->mcur], rsum, rcnt, s->W, s->H, factor, Wc);
   NX_HIP(hipGetLastError());
   k_restrict_cols<<<dim3((Wc + 63) / 64, (Hc + 3) / 4), dim3(64, 4), 0, s->stream>>>(
       rsum, rcnt, coarse_rhs, coarse_mask, s->H, factor, Wc, Hc);

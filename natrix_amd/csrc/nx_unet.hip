@@ -44,12 +44,17 @@ __device__ __forceinline__ float bf(uint16_t h) { return nx::bf16_to_f32(h); }
 __device__ __forceinline__ uint16_t tobf(float f) { return (uint16_t)nx::f32_to_bf16(f); }
 
 // ---------------------------------------------------------------- forward
-// grid: (ceil(M / (4*32*MT)), Cout / (32*NT)); block 256 = 4 waves along M
+// grid: (ceil(M / (4*32*MT)), Cout / (32*NT), splits); block 256 = 4 waves
+// along M.  With part == nullptr (one split) the block runs the whole K loop
+// and writes bias + ReLU in bf16; otherwise (Cin % 16 == 0) split z runs
+// K steps [z * steps_per_split, ...) of 16 channels of one tap each and
+// writes its float32 partial sums to part[z][m][co] for k_conv_finish.
 template <int MT, int NT>
 __global__ void __launch_bounds__(256)
     k_conv3x3(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
               const float* __restrict__ bias, uint16_t* __restrict__ y, int N,
-              int H, int W, int Cin, int Cout, int relu) {
+              int H, int W, int Cin, int Cout, int relu, float* __restrict__ part,
+              int steps_per_split) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 31, kb = lane >> 5;
   const long long M = (long long)N * H * W;
@@ -79,7 +84,10 @@ __global__ void __launch_bounds__(256)
 
   const int K = 9 * Cin;
   if ((Cin & 15) == 0) {
-    for (int tap = 0; tap < 9; ++tap) {
+    const int cps = Cin >> 4;  // K steps per tap
+    const int ks0 = part ? blockIdx.z * steps_per_split : 0;
+    const int ks1 = part ? min(ks0 + steps_per_split, 9 * cps) : 9 * cps;
+    for (int tap = ks0 / cps; tap < 9 && tap * cps < ks1; ++tap) {
       const int dy = tap / 3 - 1, dx = tap % 3 - 1;
       long long src[MT];
 #pragma unroll
@@ -89,7 +97,8 @@ __global__ void __launch_bounds__(256)
                      ? (pimg[i] + (long long)yy * W + xx) * Cin
                      : -1;
       }
-      for (int c0 = 0; c0 < Cin; c0 += 16) {
+      const int cb = max(ks0 - tap * cps, 0) * 16, ce = min(ks1 - tap * cps, cps) * 16;
+      for (int c0 = cb; c0 < ce; c0 += 16) {
         const int ci = c0 + kb * 8;
         bf16x8 a[MT], b[NT];
 #pragma unroll
@@ -131,6 +140,21 @@ __global__ void __launch_bounds__(256)
     }
   }
 
+  if (part) {
+    float* pz = part + (long long)blockIdx.z * M * Cout;
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int co = n_base + 32 * j + r;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const long long m = m_base + 32 * i + 8 * (e >> 2) + 4 * kb + (e & 3);
+          if (m < M) pz[m * Cout + co] = acc[i][j][e];
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -147,6 +171,19 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
+}
+
+// y[m][co] = bf16(relu(sum over splits z in order of part[z][m][co] + bias))
+__global__ void __launch_bounds__(256)
+    k_conv_finish(const float* __restrict__ part, const float* __restrict__ bias,
+                  uint16_t* __restrict__ y, long long MC, int Cout, int splits, int relu) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= MC) return;
+  float v = 0.0f;
+  for (int z = 0; z < splits; ++z) v += part[z * MC + i];
+  v += bias ? bias[i % Cout] : 0.0f;
+  if (relu) v = fmaxf(v, 0.0f);
+  y[i] = tobf(v);
 }
 
 // w[co][tap][ci] -> w'[ci][8 - tap][co]
@@ -376,21 +413,46 @@ int set_error(int code, const std::string& msg);
 
 extern "C" {
 
+static int conv_fwd(const void* x, const void* w, const float* bias, void* y, int n,
+                    int h, int wd, int cin, int cout, int relu, float* part, int splits,
+                    hipStream_t s) {
+  const long long M = (long long)n * h * wd;
+  const int steps = 9 * cin / 16;
+  const int per = part ? (steps + splits - 1) / splits : 0;
+  const int used = part ? (steps + per - 1) / per : 1;
+  if (cout % 64 == 0) {
+    dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64, used);
+    k_conv3x3<2, 2><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)w, bias,
+                                          (uint16_t*)y, n, h, wd, cin, cout, relu, part, per);
+  } else {
+    dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 32, used);
+    k_conv3x3<2, 1><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)w, bias,
+                                          (uint16_t*)y, n, h, wd, cin, cout, relu, part, per);
+  }
+  if (part) {
+    const long long mc = M * cout;
+    k_conv_finish<<<(unsigned)((mc + 255) / 256), 256, 0, s>>>(part, bias, (uint16_t*)y, mc,
+                                                               cout, used, relu);
+  }
+  return 0;
+}
+
 int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                    int h, int wd, int cin, int cout, int relu, void* stream) {
   if (!x || !w || !y || n < 1 || h < 1 || wd < 1 || cin < 8 || cin % 8 || cout % 32)
     return nx::set_error(NX_ERR_ARG, "conv3x3: channels must be cin % 8 == 0, cout % 32 == 0");
-  const long long M = (long long)n * h * wd;
-  const hipStream_t s = NX_U(stream);
-  if (cout % 64 == 0) {
-    dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64);
-    k_conv3x3<2, 2><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)w, bias,
-                                          (uint16_t*)y, n, h, wd, cin, cout, relu);
-  } else {
-    dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 32);
-    k_conv3x3<2, 1><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)w, bias,
-                                          (uint16_t*)y, n, h, wd, cin, cout, relu);
-  }
+  conv_fwd(x, w, bias, y, n, h, wd, cin, cout, relu, nullptr, 1, NX_U(stream));
+  NX_UNET_RET();
+}
+
+int nx_conv3x3_fwd_split(const void* x, const void* w, const float* bias, void* y,
+                         float* work, int splits, int n, int h, int wd, int cin, int cout,
+                         int relu, void* stream) {
+  if (!x || !w || !y || !work || n < 1 || h < 1 || wd < 1 || cin % 16 || cout % 32 ||
+      splits < 1 || splits > 9 * cin / 16)
+    return nx::set_error(NX_ERR_ARG,
+                         "conv3x3 split: cin % 16 == 0, cout % 32 == 0, 1 <= splits <= 9 cin / 16");
+  conv_fwd(x, w, bias, y, n, h, wd, cin, cout, relu, work, splits, NX_U(stream));
   NX_UNET_RET();
 }
 

@@ -112,15 +112,23 @@ def _solve(sim, tol, max_sweeps):
     return {"sweeps": st.sweeps, "converged": bool(st.converged), "residual": st.residual}
 
 
-def _smooth(sim, target, sweeps):
+def _smooth(sim, target, sweeps, early_exit=False):
     """`sweeps` fine sweeps of the current system ending in a damped one
     (omega = 1/2, which removes the near-checkerboard modes the plain sweep
-    keeps), stopping early at relative residual <= target."""
-    st = _solve(sim, target, max(1, sweeps - 2))
-    if st["converged"]:
-        return st
-    _lib.call("nx_damped_sweeps", sim._alive(), 1, 0.5)
-    return _solve(sim, target, 1)
+    keeps) and the residual, in one synchronisation; with `early_exit` the
+    plain sweeps run as a tolerance solve that stops at relative residual
+    <= target.  Also returns the last line search's step."""
+    n = max(0, sweeps - 1)
+    if early_exit:
+        st = _solve(sim, target, max(1, n))
+        if st["converged"]:
+            st["alpha"] = None
+            return st
+        n = 0
+    out = (C.c_double * 2)()
+    _lib.call("nx_smooth", sim._alive(), n, 0.5, out)
+    return {"sweeps": sim.stats()["sweeps"], "converged": out[0] <= target,
+            "residual": out[0], "alpha": out[1]}
 
 
 def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
@@ -129,27 +137,29 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
     from its first sweep) to relative residual `target`: learned cycles
     while each reduces the residual below `stall` x the previous one, then
     Jacobi chunks until the target, the sweep cap, or (after learned
-    cycles) a chunk that reduces it by less than 1 - `plain_stall`.  Returns (cycles whose line
-    search took a nonzero step, True if the learned cycles were abandoned
-    before the target)."""
+    cycles) a chunk that reduces it by less than 1 - `plain_stall`.
+    Returns (cycles whose line search took a nonzero step, True if the
+    learned cycles were abandoned before the target)."""
     kept, abandoned = 0, False
     before = None
     start = sim.stats()["sweeps"]
     for _ in range(max_cycles if guess is not None else 0):
-        a = guess.correct(sim, report=True)
+        guess.correct(sim)
         st = _smooth(sim, target, smooth)
+        a = st["alpha"]
         if trace is not None:
-            trace.append(("cycle", round(a[0], 4), st["sweeps"], st["residual"]))
+            trace.append(("cycle", round(a, 4), st["sweeps"], st["residual"]))
         if st["converged"]:
-            return kept + (a[0] != 0.0), False
-        if a[0] == 0.0 or (before is not None and st["residual"] > stall * before):
+            return kept + (a != 0.0), False
+        if a == 0.0 or (before is not None and st["residual"] > stall * before):
             abandoned = True
             before = st["residual"]
             break
         kept += 1
         before = st["residual"]
     while True:
-        st = _smooth(sim, target, chunk) if guess is not None else _solve(sim, target, chunk)
+        st = (_smooth(sim, target, chunk, early_exit=True) if guess is not None
+              else _solve(sim, target, chunk))
         if trace is not None:
             trace.append(("plain", st["sweeps"], st["residual"]))
         if st["converged"] or st["sweeps"] - start >= max_sweeps:

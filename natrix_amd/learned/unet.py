@@ -39,6 +39,18 @@ def _call(name, *args):
     _lib.check(getattr(_lib.lib(), name)(*args))
 
 
+def conv_splits(m, cin, cout):
+    """K splits of a forward convolution over m pixels: enough workgroups
+    (~512) for small images with many channels, at least 4 K steps of 16
+    channels each; 1 (no split) when the image alone fills the chip."""
+    if cin % 16:
+        return 1
+    base = -(-m // 256) * (cout // (64 if cout % 64 == 0 else 32))
+    if base >= 256:
+        return 1
+    return max(1, min((9 * cin // 16) // 4, -(-512 // base)))
+
+
 class Conv3x3Fn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, w, b, relu):
@@ -46,8 +58,15 @@ class Conv3x3Fn(torch.autograd.Function):
         cout = w.shape[0]
         wb = w.to(BF16).contiguous()
         y = torch.empty((n, h, wd, cout), dtype=BF16, device=x.device)
-        _call("nx_conv3x3_fwd", _p(x), _p(wb), _p(b), _p(y), n, h, wd, cin, cout,
-              1 if relu else 0, _s())
+        splits = conv_splits(n * h * wd, cin, cout)
+        if splits > 1:
+            work = torch.empty(splits * n * h * wd * cout, dtype=torch.float32,
+                               device=x.device)
+            _call("nx_conv3x3_fwd_split", _p(x), _p(wb), _p(b), _p(y), _p(work), splits,
+                  n, h, wd, cin, cout, 1 if relu else 0, _s())
+        else:
+            _call("nx_conv3x3_fwd", _p(x), _p(wb), _p(b), _p(y), n, h, wd, cin, cout,
+                  1 if relu else 0, _s())
         ctx.save_for_backward(x, wb, y)
         ctx.relu = relu
         return y

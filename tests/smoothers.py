@@ -1,10 +1,11 @@
-"""Float64 statements of the learned solve's coarse smoothers, for the GPU
-tests: the 5-point operator L of the Jacobi sweep (a solid or out-of-grid
+"""Float64 statements of the learned solve's grid transfers and coarse
+smoothers, for the GPU tests: the 5-point operator L of the Jacobi sweep (a solid or out-of-grid
 neighbour reads the centre value; shader.Poisson.comp:26-37 with the clamp
 of shader.common.sh:14-17), Jacobi sweeps e <- e + (L e - r) / 4, and the
 Chebyshev iteration for -L e = -r on [lmin, 8] (three-term recurrence of
 Saad, Iterative Methods for Sparse Linear Systems, 2nd ed., Algorithm
-12.1)."""
+12.1), the block-mean restriction of the residual and the bilinear
+prolongation with weights renormalised over fluid coarse cells."""
 import numpy as np
 
 
@@ -40,3 +41,47 @@ def cheb_ref(x, rhs, solid, iters, lmin):
         d = rho1 * rho * d + 2 * rho1 / delta * r
         rho = rho1
     return x
+
+
+def restrict_ref(p, b, solid, f):
+    """(rc, mc): rc = f^2 x the mean of r = b - L p over the fluid cells of
+    each f x f block (0 where the block has none), mc = 1 for such blocks."""
+    H, W = p.shape
+    Hc, Wc = -(-H // f), -(-W // f)
+    r = np.where(solid, 0.0, b - lap(p, solid))
+    fl = (~solid).astype(np.float64)
+    pad_r = np.zeros((Hc * f, Wc * f))
+    pad_c = np.zeros((Hc * f, Wc * f))
+    pad_r[:H, :W] = r
+    pad_c[:H, :W] = fl
+    s = pad_r.reshape(Hc, f, Wc, f).sum(axis=(1, 3))
+    c = pad_c.reshape(Hc, f, Wc, f).sum(axis=(1, 3))
+    rc = np.where(c > 0, f * f * s / np.maximum(c, 1), 0.0)
+    return rc, (c == 0)
+
+
+def prolong_ref(ec, mc, f, H, W, solid):
+    """Bilinear interpolation of the coarse field at every fluid fine cell:
+    coarse centres at fine coordinate f I + (f - 1) / 2, clamped to the
+    coarse grid, weights renormalised over the fluid coarse cells (0 where
+    none); solid fine cells get 0."""
+    Hc, Wc = ec.shape
+    out = np.zeros((H, W))
+    for y in range(H):
+        fy = min(max((y - 0.5 * (f - 1)) / f, 0.0), Hc - 1)
+        y0 = int(fy)
+        y1 = min(y0 + 1, Hc - 1)
+        ty = fy - y0
+        for x in range(W):
+            if solid[y, x]:
+                continue
+            fx = min(max((x - 0.5 * (f - 1)) / f, 0.0), Wc - 1)
+            x0 = int(fx)
+            x1 = min(x0 + 1, Wc - 1)
+            tx = fx - x0
+            ws = [((1 - tx) * (1 - ty), y0, x0), (tx * (1 - ty), y0, x1),
+                  ((1 - tx) * ty, y1, x0), (tx * ty, y1, x1)]
+            num = sum(w * ec[j, i] for w, j, i in ws if not mc[j, i])
+            den = sum(w for w, j, i in ws if not mc[j, i])
+            out[y, x] = num / den if den > 0 else 0.0
+    return out

@@ -16,7 +16,7 @@ import torch
 
 from natrix_amd import FluidSimulator, _lib
 
-from smoothers import cheb_ref, jacobi_ref, lap
+from smoothers import cheb_ref, jacobi_ref, lap, prolong_ref, restrict_ref
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +71,50 @@ def test_coarse_smoother_rejects_oversized_grids(gpu, rng):
     with pytest.raises(ValueError):
         _lib.call("nx_coarse_smooth", sim._alive(), 4, 4, C.c_void_p(t.data_ptr()),
                   C.c_void_p(t.data_ptr()), C.c_void_p(m.data_ptr()), 5, 8.0)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+@pytest.mark.parametrize("N,f", [(64, 2), (96, 3), (128, 4), (128, 16), (80, 32)])
+def test_restriction_matches_float64_statement(gpu, rng, N, f):
+    sim = in_step_sim(rng, N, N)
+    solid = sim.download("mask") != 0
+    b = rng.normal(size=(N, N)).astype(np.float32)
+    p = rng.normal(size=(N, N)).astype(np.float32)
+    dev(sim, "divergence").copy_(torch.from_numpy(b))
+    dev(sim, "pressure").copy_(torch.from_numpy(p))
+    Wc = Hc = -(-N // f)
+    rc = torch.empty(Hc * Wc, device="cuda")
+    mc = torch.empty(Hc * Wc, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    _lib.call("nx_restrict_residual", sim._alive(), f, C.c_void_p(rc.data_ptr()),
+              C.c_void_p(mc.data_ptr()))
+    sim.synchronize()
+    want, wmask = restrict_ref(p.astype(np.float64), b.astype(np.float64), solid, f)
+    got = rc.cpu().numpy().reshape(Hc, Wc)
+    assert np.array_equal(mc.cpu().numpy().reshape(Hc, Wc) != 0, wmask)
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+@pytest.mark.parametrize("N,f", [(64, 4), (96, 3)])
+def test_prolongation_matches_float64_statement(gpu, rng, N, f):
+    sim = in_step_sim(rng, N, N)
+    solid = sim.download("mask") != 0
+    Hc = Wc = -(-N // f)
+    ec = rng.normal(size=(Hc, Wc)).astype(np.float32)
+    cm = rng.random((Hc, Wc)) < 0.15
+    p = rng.normal(size=(N, N)).astype(np.float32)
+    dev(sim, "pressure").copy_(torch.from_numpy(p))
+    e = torch.from_numpy(ec).cuda().contiguous()
+    m = torch.from_numpy(cm.astype(np.uint8)).cuda()
+    torch.cuda.synchronize()
+    _lib.call("nx_prolong_add", sim._alive(), f, C.c_void_p(e.data_ptr()),
+              C.c_void_p(m.data_ptr()), 0.5)
+    got = sim.download("pressure").astype(np.float64)
+    want = p + 0.5 * prolong_ref(ec.astype(np.float64), cm, f, N, N, solid)
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
     _lib.call("nx_step_finish", sim._alive())
     sim.destroy()
 

@@ -41,6 +41,38 @@ def test_conv3x3_forward(gpu, n, h, w, ci, co):
         assert rel(y.float(), r) < 8e-3, (relu, rel(y.float(), r))
 
 
+@pytest.mark.parametrize("n,h,w,ci,co,splits", [(1, 16, 16, 256, 256, 36),
+                                                (1, 32, 32, 384, 128, 54),
+                                                (1, 8, 8, 64, 64, 5),
+                                                (2, 12, 20, 32, 96, 18)])
+def test_conv3x3_forward_split_k(gpu, n, h, w, ci, co, splits):
+    """K split over launches with float32 partials summed in split order:
+    the same float64 oracle as the single launch, and within bf16 rounding
+    of it."""
+    import ctypes as C
+    from natrix_amd import _lib
+    g = torch.Generator().manual_seed(ci * co + splits)
+    x = torch.randn(n, h, w, ci, generator=g).to(BF)
+    wt = (torch.randn(co, 9, ci, generator=g) * 0.1).to(BF)
+    b = torch.randn(co, generator=g)
+    xd, wd, bd = x.cuda(), wt.cuda(), b.cuda()
+    y1 = torch.empty(n, h, w, co, dtype=BF, device="cuda")
+    y2 = torch.empty_like(y1)
+    work = torch.empty(splits * n * h * w * co, device="cuda")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr())
+    _lib.call("nx_conv3x3_fwd", p(xd), p(wd), p(bd), p(y1), n, h, w, ci, co, 1, s)
+    _lib.call("nx_conv3x3_fwd_split", p(xd), p(wd), p(bd), p(y2), p(work), splits,
+              n, h, w, ci, co, 1, s)
+    r = ref_conv(x.float(), wt.float(), b, True)
+    assert rel(y1.float(), r) < 8e-3
+    assert rel(y2.float(), r) < 8e-3
+    assert rel(y2.float(), y1.float()) < 8e-3
+    with pytest.raises(ValueError):
+        _lib.call("nx_conv3x3_fwd_split", p(xd), p(wd), p(bd), p(y2), p(work),
+                  9 * ci // 16 + 1, n, h, w, ci, co, 1, s)
+
+
 @pytest.mark.parametrize("n,h,w,ci,co", [(2, 16, 16, 32, 32), (1, 16, 24, 64, 128),
                                          (2, 8, 8, 96, 32)])
 def test_conv3x3_backward(gpu, n, h, w, ci, co):

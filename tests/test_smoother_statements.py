@@ -49,3 +49,39 @@ def test_solid_neighbour_reads_the_centre():
     assert got[1, 1] == (v[1, 0] + v[1, 1] + v[0, 1] + v[2, 1]) - 4 * v[1, 1]
     # corner (0, 0): clamped left and up neighbours are itself
     assert got[0, 0] == (v[0, 0] + v[0, 1] + v[0, 0] + v[1, 0]) - 4 * v[0, 0]
+
+
+def test_restriction_of_a_blockwise_constant_residual():
+    from smoothers import restrict_ref
+    f, Hc, Wc = 4, 3, 5
+    c = np.arange(Hc * Wc, dtype=np.float64).reshape(Hc, Wc) - 7.0
+    b = np.kron(c, np.ones((f, f)))      # r = b for p = 0
+    solid = np.zeros_like(b, bool)
+    solid[0:4, 0:4] = True               # block (0, 0) entirely solid
+    solid[5, 6] = True                   # one solid cell in block (1, 1)
+    rc, mc = restrict_ref(np.zeros_like(b), b, solid, f)
+    want = f * f * c
+    want[0, 0] = 0.0
+    assert np.allclose(rc, want)
+    assert mc[0, 0] and mc.sum() == 1
+
+
+def test_prolongation_reproduces_constant_and_linear_fields():
+    from smoothers import prolong_ref
+    f, Hc, Wc = 4, 6, 7
+    H, W = Hc * f, Wc * f
+    mc = np.zeros((Hc, Wc), bool)
+    solid = np.zeros((H, W), bool)
+    out = prolong_ref(np.full((Hc, Wc), 2.5), mc, f, H, W, solid)
+    assert np.allclose(out, 2.5)
+    Y, X = np.mgrid[0:Hc, 0:Wc]
+    out = prolong_ref(0.3 * X - 1.7 * Y, mc, f, H, W, solid)
+    y, x = np.mgrid[0:H, 0:W]
+    lin = 0.3 * (x - 0.5 * (f - 1)) / f - 1.7 * (y - 0.5 * (f - 1)) / f
+    inner = (x >= (f - 1) / 2) & (x <= f * (Wc - 1) + (f - 1) / 2) & \
+        (y >= (f - 1) / 2) & (y <= f * (Hc - 1) + (f - 1) / 2)
+    assert np.allclose(out[inner], lin[inner])
+    # a solid coarse cell carries no weight: the constant survives next to it
+    mc[2, 3] = True
+    out = prolong_ref(np.where(mc, 99.0, 2.5), mc, f, H, W, solid)
+    assert np.allclose(out, 2.5)
