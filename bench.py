@@ -41,6 +41,27 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def committed_traffic(N, iters, tile):
+    """HBM bytes per k_jacobi_tb launch from the committed rocprofv3 summary
+    of this grid, sweep count and tile (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
+    the copy-probe calibration), or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"k_jacobi_tb_{N}_fixed{iters}.md")
+    if not os.path.exists(path):
+        return None, None
+    kv = {}
+    for line in open(path):
+        k, sep, v = line.partition(":")
+        if sep and k.strip() in ("traffic_bytes_per_launch", "grid", "iterations", "tile"):
+            kv[k.strip()] = v.strip()
+    try:
+        ok = (int(kv["grid"]) == N and int(kv["iterations"]) == iters and
+              [int(t) for t in kv["tile"].split(",")] == list(tile))
+        return (int(kv["traffic_bytes_per_launch"]) if ok else None,
+                os.path.relpath(path, ROOT) if ok else None)
+    except (KeyError, ValueError):
+        return None, None
+
+
 def make_scene(N, seed=2):
     rng = np.random.default_rng(seed)
     c = rng.uniform(0.3, 0.7, 2)
@@ -181,32 +202,51 @@ def mixed_scene(N, storage, iters, dye=True):
     return sim, splats
 
 
-def mixed_error_vs_fp64(storage, N=256, steps=20, iters=120):
-    """Max-norm velocity error of the reduced-storage GPU rollout and of the
-    float32 GPU rollout against the float64 oracle rollout (CPU)."""
+def mixed_error_vs_fp64(storage, H=96, steps=3, iters=60):
+    """Relative L2 velocity error of short rollouts against the float64
+    oracle (CPU), for the reduced storage and for float32: a 96^2 scene with
+    a rectangle obstacle, random initial velocity (seed 23, |u| <= 0.2),
+    speed 100, free-slip walls, 60 sweeps, 3 steps."""
+    from natrix_amd import FluidSimulator
     from oracle import fluid as F
+    from oracle import mixed as M
+    rng = np.random.default_rng(23)
+    u0 = (rng.uniform(-1, 1, (H, H, 2)) * 0.2).astype(np.float32)
+    solid = np.zeros((H, H), bool)
+    solid[40:56, 30:38] = True
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=iters, speed=100.0,
+                   bc_mode=F.BC_FREE_SLIP)
     errs = {}
-    ref = None
+    for st in (storage, "f32"):
+        u0r = M.rounder(st)(u0)
+        sim = FluidSimulator(H, H, bc_mode="free_slip", storage=st)
+        sim.vorticity, sim.viscosity, sim.speed, sim.iterations = 0.3, 0.0, 100.0, iters
+        sim.upload("velocity", u0r)
+        sim.set_obstacle_map(solid)
+        u64 = u0r.astype(np.float64)
+        p64 = np.zeros((H, H))
+        o64 = np.zeros((H, H, 2))
+        for _ in range(steps):
+            sim.update(1 / 60)
+            u64, p64, o64, _ = F.step(u64, p64, o64, 1 / 60, prm, persistent=solid)
+        got = sim.download("velocity").astype(np.float64)
+        errs[st] = float(np.linalg.norm(got - u64) / np.linalg.norm(u64))
+        sim.destroy()
+    return errs
+
+
+def mixed_error_vs_f32(N, storage, iters, steps=3):
+    """Relative L2 velocity error of the reduced-storage rollout against the
+    float32 GPU rollout of the same full-size scene, 3 steps from rest."""
+    out = {}
     for st in (storage, "f32"):
         sim, splats = mixed_scene(N, st, iters, dye=False)
         for _ in range(steps):
             sim.add_splats(splats)
             sim.update(1 / 60)
-        got = sim.download("velocity").astype(np.float64)
+        out[st] = sim.download("velocity").astype(np.float64)
         sim.destroy()
-        if ref is None:
-            prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=iters,
-                           bc_mode=F.BC_FREE_SLIP)
-            u = np.zeros((N, N, 2))
-            p = np.zeros((N, N))
-            o = np.zeros((N, N, 2))
-            for _ in range(steps):
-                u, _ = F.gaussian_splat(u, None, [tuple(float(np.float32(v)) for v in s)
-                                                  for s in splats])
-                u, p, o, _ = F.step(u, p, o, 1 / 60, prm)
-            ref = u
-        errs[st] = float(np.abs(got - ref).max() / np.abs(ref).max())
-    return errs
+    return float(np.linalg.norm(out[storage] - out["f32"]) / np.linalg.norm(out["f32"]))
 
 
 def bench_mixed(args):
@@ -232,17 +272,9 @@ def bench_mixed(args):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     timing = sim.collect_timing()
-    u_mixed = sim.download("velocity").astype(np.float64)
     sim.destroy()
-    # the same rollout in float32 on the GPU, for the full-size error
-    ref, _ = mixed_scene(N, "f32", iters)
-    for _ in range(args.warmup + args.steps):
-        ref.add_splats(splats)
-        ref.update(1 / 60)
-    u32 = ref.download("velocity").astype(np.float64)
-    ref.destroy()
-    err32 = float(np.abs(u_mixed - u32).max() / np.abs(u32).max())
-    errs = mixed_error_vs_fp64(args.storage)
+    err32 = None if args.no_errors else mixed_error_vs_f32(N, args.storage, iters)
+    errs = None if args.no_errors else mixed_error_vs_fp64(args.storage)
     ms = 1e3 * el / args.steps
     cells = N * N
     # bf16/f16 bytes per cell: 54 + 7*I + splats 12 + the
@@ -262,8 +294,8 @@ def bench_mixed(args):
            "step_roofline_frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "stage_ms_per_step": {k: round(v[0] / args.steps, 5)
                                  for k, v in timing.items() if v[1]},
-           "velocity_rel_err_vs_f32_gpu": err32,
-           "velocity_rel_err_vs_fp64_oracle_256sq_20steps": errs,
+           "velocity_rel_l2_err_vs_f32_gpu_3_steps": err32,
+           "velocity_rel_l2_err_vs_fp64_oracle_96sq_3_steps": errs,
            "roofline": None, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
@@ -361,8 +393,10 @@ def bench_learned(args, copy_bandwidth):
     jac_ms, jac_n = tf["jacobi"]
     tb_sweeps = (args.iters - 1) * args.fixed_steps  # sweep 1 runs in the fused divergence kernel
     ach = JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps / (jac_ms * 1e-3) / 1e9
+    traffic, tsrc = committed_traffic(N, args.iters, tile)
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_jacobi_tb",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
+            "kernel": "k_jacobi_tb",
             "avg_launch_us": round(1e3 * jac_ms / jac_n, 3),
             "sweeps_per_launch": round(tb_sweeps / jac_n, 3),
             "algorithmic_bytes_per_launch": int(JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps / jac_n)}
@@ -437,6 +471,8 @@ def main():
                          "fixed iterations; batch: 64 uneven scenes in one launch per "
                          "stage; mixed: 4096^2 with bf16/f16 storage")
     ap.add_argument("--storage", choices=["bf16", "f16"], default="bf16")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="mixed workload: skip the error rollouts (for profiling)")
     ap.add_argument("--weights", default=None, help="U-Net weights (learned workload)")
     ap.add_argument("--plain-cap", type=int, default=1_000_000,
                     help="sweep cap per correction solve of the plain path (learned workload)")
@@ -524,9 +560,12 @@ def main():
     if jac_launches > 0 and jac_ms > 0:
         jac_bytes = JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps
         achieved = jac_bytes / (jac_ms * 1e-3) / 1e9
+        traffic, tsrc = (committed_traffic(N, args.iters, sim.jacobi_tile)
+                         if args.mode == "fixed" else (None, None))
         roof = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": tsrc,
                 "kernel": "k_jacobi_tb",
                 "avg_launch_us": round(1e3 * jac_ms / jac_launches, 3),
                 "sweeps_per_launch": round(tb_sweeps / jac_launches, 3)}

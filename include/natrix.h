@@ -232,6 +232,8 @@ int nx_get_stats(nx_sim* sim, nx_stats* out);
 int nx_synchronize(nx_sim* sim);
 /* the simulator's HIP stream (hipStream_t) */
 int nx_get_stream(nx_sim* sim, void** stream);
+/* grid size (FluidSimulator.width / height, fluid_simulator.py:37-48) */
+int nx_get_size(const nx_sim* sim, int* width, int* height);
 
 /* ---- timing: HIP events around every launch of the selected kernels ---- */
 #define NX_TIME_ADVECT 0
@@ -321,6 +323,31 @@ int nx_head_fwd(const void* x, const float* w, const float* bias, float* out,
                 long long pixels, int c, void* stream);
 int nx_head_bwd(const void* x, const float* w, const float* dout, void* dx, float* dwb,
                 float* work, long long pixels, int c, void* stream);
+
+/* ---- native learned correction (inference; no reference counterpart).
+ *      The U-Net of natrix_amd.learned.unet on one coarse grid of H x W
+ *      (multiples of 8): conv_w[14] / conv_b[14] are the 3x3 convolutions in
+ *      forward order (e0 a, e0 b, e1 a, e1 b, e2 a, e2 b, e3 a, e3 b, d2 a,
+ *      d2 b, d1 a, d1 b, d0 a, d0 b) as bfloat16 KRSC weights and float32
+ *      biases, head_w[base] / head_b[1] the 1x1 head, all device pointers
+ *      the caller keeps alive; the runner owns its activation buffers. ---- */
+typedef struct nx_unet nx_unet;
+int nx_unet_create(int H, int W, int base, const void* const* conv_w,
+                   const float* const* conv_b, const float* head_w, const float* head_b,
+                   nx_unet** out);
+int nx_unet_destroy(nx_unet* net);
+/* out[H*W] (float32) = network(x[H*W*8] bfloat16 NHWC), batch 1 */
+int nx_unet_forward(nx_unet* net, const void* x, float* out, void* stream);
+/* one learned correction of the simulator's pressure, on its stream, no
+ * synchronisation: restrict the residual (factor), normalise to unit RMS,
+ * the network, rescale, `coarse_iters` Chebyshev iterations on
+ * [lambda_min, 8] (Jacobi sweeps if lambda_min <= 0), remove the fluid
+ * mean, interpolate and add with the energy line search (out as for
+ * nx_prolong_line_search) */
+int nx_learned_correct(nx_sim* sim, nx_unet* net, int factor, int coarse_iters,
+                       float lambda_min, double* out);
+/* the runner's coarse buffers (rhs, mask, correction), for inspection */
+int nx_unet_coarse(nx_unet* net, float** rc, uint8_t** mc, float** e);
 
 /* ---- FP64 oracle solve on the GPU (conjugate gradients on the same
  *      operator, fluid cells, right-hand side centred on its fluid mean) ---- */

@@ -148,6 +148,12 @@ SIGNATURES = {
     "nx_step_finish": [_P],
     "nx_conv3x3_fwd": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
     "nx_conv3x3_fwd_split": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
+    "nx_unet_create": [_I, _I, _I, _P, _P, _P, _P, _P],
+    "nx_unet_destroy": [_P],
+    "nx_unet_forward": [_P, _P, _P, _P],
+    "nx_learned_correct": [_P, _P, _I, _I, _F, _P],
+    "nx_unet_coarse": [_P, _P, _P, _P],
+    "nx_get_size": [_P, _P, _P],
     "nx_conv3x3_dgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
     "nx_conv3x3_wgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
     "nx_relu_bwd": [_P, _P, _P, C.c_longlong, _P],

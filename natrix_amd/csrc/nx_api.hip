@@ -1082,6 +1082,13 @@ int nx_destroy(nx_sim* s) {
   return NX_OK;
 }
 
+int nx_get_size(const nx_sim* s, int* width, int* height) {
+  if (!s || !width || !height) return fail(NX_ERR_ARG, "null argument");
+  *width = s->W;
+  *height = s->H;
+  return NX_OK;
+}
+
 int nx_get_params(const nx_sim* s, nx_params* out) {
   if (!s || !out) return fail(NX_ERR_ARG, "null argument");
   *out = s->prm;

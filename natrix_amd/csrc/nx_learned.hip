@@ -1,0 +1,309 @@
+// Native runner of the learned coarse correction (inference only).
+//
+// One call of nx_learned_correct enqueues, on the simulator's stream and
+// with no host synchronisation, the whole correction of one learned cycle:
+//   1. restriction of the current residual to the coarse grid
+//      (nx_restrict_residual: rc = f^2 x block mean, coarse mask);
+//   2. k_pack_input (one block): s = RMS of rc over fluid coarse cells,
+//      network input x[p] = (bf16(rc/s), bf16(solid), 0 x 6);
+//   3. the U-Net forward (4 levels, 3x3 conv + ReLU pairs, 2x2 average pool,
+//      2x nearest upsample + channel concat, 1x1 head) with the forward
+//      convolution kernels, split over K exactly as the Python wrapper
+//      splits them, so nx_unet_forward equals the Python forward bitwise;
+//   4. k_scale_output: e = s x head output;
+//   5. Chebyshev smoothing of the coarse problem from e (nx_coarse_smooth);
+//   6. k_center_output (one block): e -= fluid mean of e, 0 on solid cells;
+//   7. interpolation and the energy line search (nx_prolong_line_search).
+// The weights stay owned by the caller (device pointers handed to
+// nx_unet_create); the runner owns its activation buffers, sized for one
+// coarse grid.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/natrix.h"
+#include "nx_common.h"
+
+namespace nx {
+int set_error(int code, const std::string& msg);
+}
+
+struct nx_unet {
+  int H = 0, W = 0, base = 0;
+  // 14 convolutions in forward order: e0 a/b, e1 a/b, e2 a/b, e3 a/b,
+  // d2 a/b, d1 a/b, d0 a/b
+  const void* w[14] = {};
+  const float* b[14] = {};
+  int cin[14] = {}, cout[14] = {};
+  const float* head_w = nullptr;
+  const float* head_b = nullptr;
+  std::vector<void*> bufs;
+  uint16_t *x0 = nullptr, *t0 = nullptr, *e0 = nullptr, *p0 = nullptr;
+  uint16_t *t1 = nullptr, *e1 = nullptr, *p1 = nullptr;
+  uint16_t *t2 = nullptr, *e2 = nullptr, *p2 = nullptr;
+  uint16_t *t3 = nullptr, *e3 = nullptr;
+  uint16_t *u2 = nullptr, *c2 = nullptr, *s2 = nullptr, *d2 = nullptr;
+  uint16_t *u1 = nullptr, *c1 = nullptr, *s1 = nullptr, *d1 = nullptr;
+  uint16_t *u0 = nullptr, *c0 = nullptr, *s0 = nullptr, *d0 = nullptr;
+  float* out = nullptr;    // head output / coarse correction e
+  float* rc = nullptr;     // coarse right-hand side
+  uint8_t* mc = nullptr;   // coarse mask
+  float* work = nullptr;   // split-K partial sums
+  size_t work_elems = 0;
+  float* scal = nullptr;   // [0] = s (RMS of rc over fluid cells)
+};
+
+namespace {
+
+int fail(int code, const char* msg) { return nx::set_error(code, msg); }
+
+// Same rule as natrix_amd.learned.unet.conv_splits.
+int conv_splits(long long m, int cin, int cout) {
+  if (cin % 16) return 1;
+  const long long base = ((m + 255) / 256) * (cout / (cout % 64 == 0 ? 64 : 32));
+  if (base >= 256) return 1;
+  const long long a = (9 * cin / 16) / 4, b = (512 + base - 1) / base;
+  const long long s = a < b ? a : b;
+  return s < 1 ? 1 : (int)s;
+}
+
+__device__ __forceinline__ float block_sum_1024(float v, float* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  float t = 0.0f;
+  for (int w = 0; w < 16; ++w) t += sh[w];
+  __syncthreads();
+  return t;
+}
+
+// x[p] = (bf16(rc[p] / s), bf16(mask), 0, ...), s = sqrt(sum_fluid rc^2 / n)
+__global__ void __launch_bounds__(1024)
+    k_pack_input(const float* __restrict__ rc, const uint8_t* __restrict__ mc,
+                 uint16_t* __restrict__ x, float* __restrict__ scal, int n) {
+  __shared__ float sh[16];
+  float ss = 0.0f, cnt = 0.0f;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    if (mc[i] == 0) {
+      ss += rc[i] * rc[i];
+      cnt += 1.0f;
+    }
+  }
+  ss = block_sum_1024(ss, sh);
+  cnt = block_sum_1024(cnt, sh);
+  const float s = fmaxf(sqrtf(ss / fmaxf(cnt, 1.0f)), 1e-30f);
+  if (threadIdx.x == 0) scal[0] = s;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    uint16_t* px = x + 8 * (size_t)i;
+    px[0] = (uint16_t)nx::f32_to_bf16(rc[i] / s);
+    px[1] = (uint16_t)nx::f32_to_bf16(mc[i] ? 1.0f : 0.0f);
+    for (int c = 2; c < 8; ++c) px[c] = 0;
+  }
+}
+
+__global__ void k_scale_output(float* __restrict__ e, const float* __restrict__ scal, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) e[i] = e[i] * scal[0];
+}
+
+// e -= mean of e over fluid cells; e = 0 on solid cells
+__global__ void __launch_bounds__(1024)
+    k_center_output(float* __restrict__ e, const uint8_t* __restrict__ mc, int n) {
+  __shared__ float sh[16];
+  float sum = 0.0f, cnt = 0.0f;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    if (mc[i] == 0) {
+      sum += e[i];
+      cnt += 1.0f;
+    }
+  }
+  sum = block_sum_1024(sum, sh);
+  cnt = block_sum_1024(cnt, sh);
+  const float m = sum / fmaxf(cnt, 1.0f);
+  for (int i = threadIdx.x; i < n; i += 1024) e[i] = mc[i] ? 0.0f : e[i] - m;
+}
+
+int conv(nx_unet* u, int i, const void* x, void* y, int h, int w, hipStream_t s) {
+  const long long m = (long long)h * w;
+  const int sp = conv_splits(m, u->cin[i], u->cout[i]);
+  if (sp > 1)
+    return nx_conv3x3_fwd_split(x, u->w[i], u->b[i], y, u->work, sp, 1, h, w, u->cin[i],
+                                u->cout[i], 1, s);
+  return nx_conv3x3_fwd(x, u->w[i], u->b[i], y, 1, h, w, u->cin[i], u->cout[i], 1, s);
+}
+
+int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s) {
+  const int H = u->H, W = u->W, c = u->base;
+#define NX_TRY_U(expr)          \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != NX_OK) return rc_; \
+  } while (0)
+  NX_TRY_U(conv(u, 0, x, u->t0, H, W, s));
+  NX_TRY_U(conv(u, 1, u->t0, u->e0, H, W, s));
+  NX_TRY_U(nx_avgpool2(u->e0, u->p0, 1, H, W, c, 0, s));
+  NX_TRY_U(conv(u, 2, u->p0, u->t1, H / 2, W / 2, s));
+  NX_TRY_U(conv(u, 3, u->t1, u->e1, H / 2, W / 2, s));
+  NX_TRY_U(nx_avgpool2(u->e1, u->p1, 1, H / 2, W / 2, 2 * c, 0, s));
+  NX_TRY_U(conv(u, 4, u->p1, u->t2, H / 4, W / 4, s));
+  NX_TRY_U(conv(u, 5, u->t2, u->e2, H / 4, W / 4, s));
+  NX_TRY_U(nx_avgpool2(u->e2, u->p2, 1, H / 4, W / 4, 4 * c, 0, s));
+  NX_TRY_U(conv(u, 6, u->p2, u->t3, H / 8, W / 8, s));
+  NX_TRY_U(conv(u, 7, u->t3, u->e3, H / 8, W / 8, s));
+  NX_TRY_U(nx_upsample2(u->e3, u->u2, 1, H / 4, W / 4, 8 * c, 0, s));
+  NX_TRY_U(nx_concat(u->c2, u->u2, u->e2, (long long)(H / 4) * (W / 4), 8 * c, 4 * c, 0, s));
+  NX_TRY_U(conv(u, 8, u->c2, u->s2, H / 4, W / 4, s));
+  NX_TRY_U(conv(u, 9, u->s2, u->d2, H / 4, W / 4, s));
+  NX_TRY_U(nx_upsample2(u->d2, u->u1, 1, H / 2, W / 2, 4 * c, 0, s));
+  NX_TRY_U(nx_concat(u->c1, u->u1, u->e1, (long long)(H / 2) * (W / 2), 4 * c, 2 * c, 0, s));
+  NX_TRY_U(conv(u, 10, u->c1, u->s1, H / 2, W / 2, s));
+  NX_TRY_U(conv(u, 11, u->s1, u->d1, H / 2, W / 2, s));
+  NX_TRY_U(nx_upsample2(u->d1, u->u0, 1, H, W, 2 * c, 0, s));
+  NX_TRY_U(nx_concat(u->c0, u->u0, u->e0, (long long)H * W, 2 * c, c, 0, s));
+  NX_TRY_U(conv(u, 12, u->c0, u->s0, H, W, s));
+  NX_TRY_U(conv(u, 13, u->s0, u->d0, H, W, s));
+  NX_TRY_U(nx_head_fwd(u->d0, u->head_w, u->head_b, out, (long long)H * W, c, s));
+#undef NX_TRY_U
+  return NX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nx_unet_create(int H, int W, int base, const void* const* conv_w,
+                   const float* const* conv_b, const float* head_w, const float* head_b,
+                   nx_unet** out) {
+  if (!out || !conv_w || !conv_b || !head_w || !head_b)
+    return fail(NX_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (H < 8 || W < 8 || H % 8 || W % 8 || base < 16 || base % 16)
+    return fail(NX_ERR_ARG, "U-Net grid must be a multiple of 8 (>= 8), base a multiple of 16");
+  nx_unet* u = new nx_unet();
+  u->H = H;
+  u->W = W;
+  u->base = base;
+  const int c = base;
+  const int ci[14] = {8, c, c, 2 * c, 2 * c, 4 * c, 4 * c, 8 * c, 12 * c, 4 * c, 6 * c, 2 * c,
+                      3 * c, c};
+  const int co[14] = {c, c, 2 * c, 2 * c, 4 * c, 4 * c, 8 * c, 8 * c, 4 * c, 4 * c, 2 * c,
+                      2 * c, c, c};
+  const int lvl[14] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0};
+  for (int i = 0; i < 14; ++i) {
+    if (!conv_w[i] || !conv_b[i]) {
+      delete u;
+      return fail(NX_ERR_ARG, "null convolution weight");
+    }
+    u->w[i] = conv_w[i];
+    u->b[i] = conv_b[i];
+    u->cin[i] = ci[i];
+    u->cout[i] = co[i];
+    const long long m = (long long)(H >> lvl[i]) * (W >> lvl[i]);
+    const int sp = conv_splits(m, ci[i], co[i]);
+    if (sp > 1 && (size_t)(sp * m * co[i]) > u->work_elems)
+      u->work_elems = (size_t)sp * m * co[i];
+  }
+  u->head_w = head_w;
+  u->head_b = head_b;
+  bool ok = true;
+  auto alloc = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes < 256 ? 256 : bytes) != hipSuccess) {
+      ok = false;
+      return nullptr;
+    }
+    u->bufs.push_back(p);
+    return p;
+  };
+  const size_t P0 = (size_t)H * W, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64, B = 2;
+  u->x0 = (uint16_t*)alloc(P0 * 8 * B);
+  u->t0 = (uint16_t*)alloc(P0 * c * B);
+  u->e0 = (uint16_t*)alloc(P0 * c * B);
+  u->p0 = (uint16_t*)alloc(P1 * c * B);
+  u->t1 = (uint16_t*)alloc(P1 * 2 * c * B);
+  u->e1 = (uint16_t*)alloc(P1 * 2 * c * B);
+  u->p1 = (uint16_t*)alloc(P2 * 2 * c * B);
+  u->t2 = (uint16_t*)alloc(P2 * 4 * c * B);
+  u->e2 = (uint16_t*)alloc(P2 * 4 * c * B);
+  u->p2 = (uint16_t*)alloc(P3 * 4 * c * B);
+  u->t3 = (uint16_t*)alloc(P3 * 8 * c * B);
+  u->e3 = (uint16_t*)alloc(P3 * 8 * c * B);
+  u->u2 = (uint16_t*)alloc(P2 * 8 * c * B);
+  u->c2 = (uint16_t*)alloc(P2 * 12 * c * B);
+  u->s2 = (uint16_t*)alloc(P2 * 4 * c * B);
+  u->d2 = (uint16_t*)alloc(P2 * 4 * c * B);
+  u->u1 = (uint16_t*)alloc(P1 * 4 * c * B);
+  u->c1 = (uint16_t*)alloc(P1 * 6 * c * B);
+  u->s1 = (uint16_t*)alloc(P1 * 2 * c * B);
+  u->d1 = (uint16_t*)alloc(P1 * 2 * c * B);
+  u->u0 = (uint16_t*)alloc(P0 * 2 * c * B);
+  u->c0 = (uint16_t*)alloc(P0 * 3 * c * B);
+  u->s0 = (uint16_t*)alloc(P0 * c * B);
+  u->d0 = (uint16_t*)alloc(P0 * c * B);
+  u->out = (float*)alloc(P0 * sizeof(float));
+  u->rc = (float*)alloc(P0 * sizeof(float));
+  u->mc = (uint8_t*)alloc(P0);
+  u->work = (float*)alloc((u->work_elems ? u->work_elems : 1) * sizeof(float));
+  u->scal = (float*)alloc(4 * sizeof(float));
+  if (!ok) {
+    nx_unet_destroy(u);
+    return fail(NX_ERR_HIP, "U-Net workspace allocation failed");
+  }
+  *out = u;
+  return NX_OK;
+}
+
+int nx_unet_destroy(nx_unet* u) {
+  if (!u) return NX_OK;
+  for (void* p : u->bufs) (void)hipFree(p);
+  delete u;
+  return NX_OK;
+}
+
+int nx_unet_forward(nx_unet* u, const void* x, float* out, void* stream) {
+  if (!u || !x || !out) return fail(NX_ERR_ARG, "null argument");
+  return forward(u, (const uint16_t*)x, out, (hipStream_t)stream);
+}
+
+int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
+                       float lambda_min, double* out) {
+  if (!sim || !u || factor < 1) return fail(NX_ERR_ARG, "bad learned correction arguments");
+  void* sv = nullptr;
+  int rc = nx_get_stream(sim, &sv);
+  if (rc != NX_OK) return rc;
+  const hipStream_t s = (hipStream_t)sv;
+  const int n = u->H * u->W;
+  // the coarse grid must be the network's grid
+  int W = 0, H = 0;
+  rc = nx_get_size(sim, &W, &H);
+  if (rc != NX_OK) return rc;
+  if ((W + factor - 1) / factor != u->W || (H + factor - 1) / factor != u->H)
+    return fail(NX_ERR_ARG, "coarse grid does not match the network's grid");
+  rc = nx_restrict_residual(sim, factor, u->rc, u->mc);
+  if (rc != NX_OK) return rc;
+  k_pack_input<<<1, 1024, 0, s>>>(u->rc, u->mc, u->x0, u->scal, n);
+  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "pack launch failed");
+  rc = forward(u, u->x0, u->out, s);
+  if (rc != NX_OK) return rc;
+  k_scale_output<<<(n + 255) / 256, 256, 0, s>>>(u->out, u->scal, n);
+  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "scale launch failed");
+  rc = nx_coarse_smooth(sim, u->W, u->H, u->out, u->rc, u->mc, coarse_iters, lambda_min);
+  if (rc != NX_OK) return rc;
+  k_center_output<<<1, 1024, 0, s>>>(u->out, u->mc, n);
+  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "center launch failed");
+  return nx_prolong_line_search(sim, factor, u->out, u->mc, out);
+}
+
+int nx_unet_coarse(nx_unet* u, float** rc, uint8_t** mc, float** e) {
+  if (!u) return fail(NX_ERR_ARG, "null argument");
+  if (rc) *rc = u->rc;
+  if (mc) *mc = u->mc;
+  if (e) *e = u->out;
+  return NX_OK;
+}
+
+}  // extern "C"

@@ -32,7 +32,44 @@ import math
 import torch
 
 from .. import _lib
-from .unet import pack_input
+from .unet import UNet, pack_input
+
+
+class NativeUNet:
+    """The trained U-Net as a native runner (nx_unet_create): bfloat16 copies
+    of the 14 convolution weights, float32 biases and head, kept alive here;
+    the runner owns its activation buffers for one H x W coarse grid."""
+
+    def __init__(self, net, H, W):
+        convs = [m for blk in (net.e0, net.e1, net.e2, net.e3, net.d2, net.d1, net.d0)
+                 for m in blk]
+        self._w = [c.w.detach().to(torch.bfloat16).contiguous() for c in convs]
+        self._b = [c.b.detach().float().contiguous() for c in convs]
+        self._hw = net.hw.detach().float().contiguous()
+        self._hb = net.hb.detach().float().contiguous()
+        base = convs[0].w.shape[0]
+        h = C.c_void_p()
+        wp = (C.c_void_p * 14)(*[t.data_ptr() for t in self._w])
+        bp = (C.c_void_p * 14)(*[t.data_ptr() for t in self._b])
+        _lib.call("nx_unet_create", H, W, base, wp, bp, C.c_void_p(self._hw.data_ptr()),
+                  C.c_void_p(self._hb.data_ptr()), C.byref(h))
+        self.handle = h
+        self.H, self.W = H, W
+
+    def forward(self, x, out):
+        """out[H*W] float32 = network(x[H, W, 8] bfloat16) on the current stream."""
+        _lib.call("nx_unet_forward", self.handle, C.c_void_p(x.data_ptr()),
+                  C.c_void_p(out.data_ptr()),
+                  C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.call("nx_unet_destroy", h)
+            except Exception:
+                pass
+            self.handle = None
 
 
 def chebyshev_lmin(iterations, damping):
@@ -48,13 +85,18 @@ class LearnedGuess:
     solution; None gives the same cycle without a network (coarse smoothing
     from zero), for comparison.  `coarse_iters` Chebyshev iterations with
     `damping` on the modes above their interval's lower end (0 iterations:
-    the network output as is)."""
+    the network output as is).  A `UNet` runs natively (one C-ABI call per
+    correction, nx_learned_correct) unless `native` is False; other modules
+    run through PyTorch."""
 
-    def __init__(self, net, coarse=128, coarse_iters=100, damping=1e-3):
+    def __init__(self, net, coarse=128, coarse_iters=100, damping=1e-3, native=True):
         self.net = net.eval() if net is not None else None
         self.coarse = coarse
         self.coarse_iters = coarse_iters
         self.lmin = chebyshev_lmin(coarse_iters, damping) if coarse_iters else 0.0
+        self._runner = None
+        if native and isinstance(net, UNet):
+            self._runner = NativeUNet(net, coarse, coarse)
 
     def _factor(self, sim):
         W, H = sim.width, sim.height
@@ -75,6 +117,11 @@ class LearnedGuess:
     def _correct(self, sim, report, line_search):
         f = self._factor(sim)
         c = self.coarse
+        if self._runner is not None and line_search:
+            out = (C.c_double * 3)() if report else None
+            _lib.call("nx_learned_correct", sim._alive(), self._runner.handle, f,
+                      int(self.coarse_iters), float(self.lmin), out)
+            return tuple(out) if report else None
         rc = torch.empty(c * c, dtype=torch.float32, device="cuda")
         mc = torch.empty(c * c, dtype=torch.uint8, device="cuda")
         _lib.call("nx_restrict_residual", sim._alive(), f, C.c_void_p(rc.data_ptr()),

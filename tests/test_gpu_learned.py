@@ -261,3 +261,47 @@ def test_trained_network_reduces_the_work_to_the_tolerance(gpu):
     print({k: (v[0]["sweeps"], round(v[1] * 1e3, 2)) for k, v in res.items()})
     assert res["learned"][0]["sweeps"] * 3 <= res["plain"][0]["sweeps"]
     assert res["learned"][0]["sweeps"] <= res["no_net"][0]["sweeps"]
+
+
+def test_native_unet_forward_equals_the_pytorch_forward(gpu):
+    """The native runner launches the same kernels with the same K splits as
+    the autograd wrapper: bitwise equal output on the trained weights."""
+    from natrix_amd.learned.solver import NativeUNet
+    from natrix_amd.learned.train import load
+    from natrix_amd.learned.unet import pack_input
+    net, meta = load()
+    n = meta.get("size", 128)
+    g = torch.Generator().manual_seed(11)
+    rhs = torch.randn(1, n, n, generator=g).cuda()
+    solid = (torch.rand(1, n, n, generator=g) < 0.1).cuda()
+    x = pack_input(rhs, solid)
+    with torch.no_grad():
+        want = net(x).view(-1)
+    runner = NativeUNet(net, n, n)
+    got = torch.empty(n * n, device="cuda")
+    runner.forward(x, got)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+
+
+def test_native_correction_matches_the_pytorch_path(gpu):
+    """One learned correction through nx_learned_correct and through the
+    PyTorch path on the same state: the same step to within the float32
+    rounding of the input normalisation (bf16 input)."""
+    from natrix_amd.learned.solver import LearnedGuess
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    res = {}
+    for native in (True, False):
+        sim = cylinder_sim(256, warm=4)
+        sim._push_params()
+        _lib.call("nx_step_begin", sim._alive(), 1 / 60)
+        p0 = sim.download("pressure")
+        a = LearnedGuess(net, meta.get("size", 128), native=native).correct(sim, report=True)
+        res[native] = (sim.download("pressure").astype(np.float64) - p0, a)
+        _lib.call("nx_step_finish", sim._alive())
+        sim.destroy()
+    d_n, a_n = res[True]
+    d_p, a_p = res[False]
+    assert np.linalg.norm(d_n - d_p) <= 1e-2 * np.linalg.norm(d_p)
+    assert abs(a_n[0] - a_p[0]) <= 1e-2 * abs(a_p[0])
