@@ -86,6 +86,8 @@ struct nx_sim {
   float* b0 = nullptr;
   float* dcorr = nullptr;   // prolonged coarse correction (line search)
   float* rrow = nullptr;    // restriction row records (2 x H x Wc)
+  float* cws = nullptr;     // banded coarse smoother: x, r, d twice
+  size_t cws_cap = 0;
   size_t rrow_cap = 0;
   double* line = nullptr;   // {alpha, d.r, d.Ld, 0}
   bool refining = false;
@@ -645,30 +647,31 @@ __global__ void __launch_bounds__(256)
 // / (8 - lmin)) after k iterations and never amplifies the modes below.
 constexpr int CJ_ROWS = 8, CJ_WAVES = 16, CJ_MAX = 128;
 
+template <int NR>
 struct CoarseTile {
   float (*top)[CJ_WAVES][CJ_MAX];
   float (*bot)[CJ_WAVES][CJ_MAX];
-  int code[CJ_ROWS][2];  // bit 0: left, 1: right, 2: up (y-1), 3: down (y+1)
+  int code[NR][2];  // bit 0: left, 1: right, 2: up (y-1), 3: down (y+1)
   int lane, wave, x0, buf;
 
   // out = L v for this lane's cells (all threads of the block call it)
-  __device__ __forceinline__ void lap(const float (&v)[CJ_ROWS][2],
-                                      float (&out)[CJ_ROWS][2]) {
+  __device__ __forceinline__ void lap(const float (&v)[NR][2],
+                                      float (&out)[NR][2]) {
     top[buf][wave][x0] = v[0][0];
     top[buf][wave][x0 + 1] = v[0][1];
-    bot[buf][wave][x0] = v[CJ_ROWS - 1][0];
-    bot[buf][wave][x0 + 1] = v[CJ_ROWS - 1][1];
+    bot[buf][wave][x0] = v[NR - 1][0];
+    bot[buf][wave][x0 + 1] = v[NR - 1][1];
     __syncthreads();
-    float fl[CJ_ROWS], fr[CJ_ROWS];
+    float fl[NR], fr[NR];
 #pragma unroll
-    for (int i = 0; i < CJ_ROWS; ++i) {
+    for (int i = 0; i < NR; ++i) {
       fl[i] = jt::lane_from_left(v[i][1]);
       fr[i] = jt::lane_from_right(v[i][0]);
     }
     const float* above = bot[buf][wave > 0 ? wave - 1 : 0];
     const float* below = top[buf][wave < CJ_WAVES - 1 ? wave + 1 : 0];
 #pragma unroll
-    for (int i = 0; i < CJ_ROWS; ++i)
+    for (int i = 0; i < NR; ++i)
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const float ctr = v[i][c];
@@ -676,7 +679,7 @@ struct CoarseTile {
         float L = c == 1 ? v[i][0] : fl[i];
         float R = c == 0 ? v[i][1] : fr[i];
         float U = i > 0 ? v[i - 1][c] : above[x0 + c];
-        float D = i < CJ_ROWS - 1 ? v[i + 1][c] : below[x0 + c];
+        float D = i < NR - 1 ? v[i + 1][c] : below[x0 + c];
         L = (cd & 1) ? ctr : L;
         R = (cd & 2) ? ctr : R;
         U = (cd & 4) ? ctr : U;
@@ -692,7 +695,7 @@ __global__ void __launch_bounds__(1024)
     k_coarse_smooth(float* __restrict__ e, const float* __restrict__ rhs,
                     const uint8_t* __restrict__ mc, int Wc, int Hc, int iters, float lmin) {
   __shared__ float top[2][CJ_WAVES][CJ_MAX], bot[2][CJ_WAVES][CJ_MAX];
-  CoarseTile t;
+  CoarseTile<CJ_ROWS> t;
   t.top = top;
   t.bot = bot;
   t.lane = threadIdx.x & 63;
@@ -783,6 +786,93 @@ __global__ void __launch_bounds__(256)
   const float y2 = mask[iT] ? p : pin[iT];
   const float jac = ((((x1 + x2) + y1) + y2) - b[pos]) * 0.25f;
   pout[pos] = p + w * (jac - p);
+}
+
+// The Chebyshev smoothing of k_coarse_smooth<true> spread over row bands,
+// one workgroup per band of CB_BAND rows: a launch runs CB_T iterations
+// (plus, at the first launch, the initial residual) on its band extended by
+// CB_T + 1 rows on each side, whose outer rows go stale one row per stencil,
+// and stores the band's x, r, d.  Launch boundaries are the only global
+// synchronisation.  Every cell sees the operations of the single-workgroup
+// kernel in the same order, and the recurrence coefficient after s0
+// iterations is recomputed by the same float32 recurrence.
+constexpr int CB_ROWS = 2, CB_EY = CB_ROWS * CJ_WAVES, CB_T = 8;
+constexpr int CB_BAND = CB_EY - 2 * (CB_T + 1);
+
+__global__ void __launch_bounds__(1024)
+    k_coarse_cheb_band(const float* __restrict__ xin, const float* __restrict__ rin,
+                       const float* __restrict__ din, float* __restrict__ xout,
+                       float* __restrict__ rout, float* __restrict__ dout,
+                       const float* __restrict__ rhs, const uint8_t* __restrict__ mc,
+                       int Wc, int Hc, int s0, int T, float lmin) {
+  __shared__ float top[2][CJ_WAVES][CJ_MAX], bot[2][CJ_WAVES][CJ_MAX];
+  CoarseTile<CB_ROWS> t;
+  t.top = top;
+  t.bot = bot;
+  t.lane = threadIdx.x & 63;
+  t.wave = threadIdx.x >> 6;
+  t.x0 = 2 * t.lane;
+  t.buf = 0;
+  const int band0 = blockIdx.x * CB_BAND;
+  const int y0 = band0 - (CB_T + 1) + CB_ROWS * t.wave;
+  const bool first = s0 == 0;
+  auto solid = [&](int x, int y) {
+    return x < 0 || y < 0 || x >= Wc || y >= Hc || mc[y * Wc + x] != 0;
+  };
+  float x[CB_ROWS][2], r[CB_ROWS][2], d[CB_ROWS][2], lv[CB_ROWS][2];
+#pragma unroll
+  for (int i = 0; i < CB_ROWS; ++i)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int gx = t.x0 + c, gy = y0 + i;
+      const bool in = gx < Wc && gy >= 0 && gy < Hc;
+      const int pos = in ? gy * Wc + gx : 0;
+      x[i][c] = in ? xin[pos] : 0.0f;
+      r[i][c] = in ? (first ? rhs[pos] : rin[pos]) : 0.0f;
+      d[i][c] = (in && !first) ? din[pos] : 0.0f;
+      t.code[i][c] = (solid(gx - 1, gy) ? 1 : 0) | (solid(gx + 1, gy) ? 2 : 0) |
+                     (solid(gx, gy - 1) ? 4 : 0) | (solid(gx, gy + 1) ? 8 : 0);
+    }
+  const float theta = 0.5f * (8.0f + lmin), delta = 0.5f * (8.0f - lmin);
+  const float sigma = theta / delta;
+  float rho = 1.0f / sigma;
+  for (int k = 0; k < s0; ++k) rho = 1.0f / (2.0f * sigma - rho);
+  if (first) {
+    t.lap(x, lv);
+#pragma unroll
+    for (int i = 0; i < CB_ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        r[i][c] = lv[i][c] - r[i][c];
+        d[i][c] = r[i][c] / theta;
+      }
+  }
+  for (int it = 0; it < T; ++it) {
+    t.lap(d, lv);
+    const float rho1 = 1.0f / (2.0f * sigma - rho);
+    const float cd = rho1 * rho, cr = 2.0f * rho1 / delta;
+#pragma unroll
+    for (int i = 0; i < CB_ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        x[i][c] = x[i][c] + d[i][c];
+        r[i][c] = r[i][c] + lv[i][c];
+        d[i][c] = cd * d[i][c] + cr * r[i][c];
+      }
+    rho = rho1;
+  }
+#pragma unroll
+  for (int i = 0; i < CB_ROWS; ++i)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int gx = t.x0 + c, gy = y0 + i;
+      if (gx < Wc && gy >= band0 && gy < band0 + CB_BAND && gy < Hc) {
+        const int pos = gy * Wc + gx;
+        xout[pos] = x[i][c];
+        rout[pos] = r[i][c];
+        dout[pos] = d[i][c];
+      }
+    }
 }
 
 // Energy line search along a correction d.  With L the 5-point operator of
@@ -1069,6 +1159,7 @@ int nx_destroy(nx_sim* s) {
   (void)hipFree(s->b0);
   (void)hipFree(s->dcorr);
   (void)hipFree(s->rrow);
+  (void)hipFree(s->cws);
   (void)hipFree(s->line);
   (void)hipFree(s->curl);
   (void)hipFree(s->red5);
@@ -1417,7 +1508,33 @@ int nx_coarse_smooth(nx_sim* s, int Wc, int Hc, float* e, const float* rc,
     return fail(NX_ERR_ARG, "coarse grid must be at most 128 x 128, iterations >= 0");
   if (!(lambda_min < 8.0f)) return fail(NX_ERR_ARG, "lambda_min must be below 8");
   if (iterations == 0) return NX_OK;
-  if (lambda_min > 0.0f)
+  if (lambda_min > 0.0f && iterations > CB_T) {
+    // banded: x, r, d ping-pong between two sets in the sim's workspace;
+    // the last launch writes x to e (e is read only by the first launch)
+    const size_t n = (size_t)Wc * Hc;
+    if (6 * n > s->cws_cap) {
+      if (s->cws) NX_HIP(hipFree(s->cws));
+      s->cws = nullptr;
+      NX_HIP(hipMalloc(&s->cws, 6 * n * sizeof(float)));
+      s->cws_cap = 6 * n;
+    }
+    float* set[2][3] = {{s->cws, s->cws + n, s->cws + 2 * n},
+                        {s->cws + 3 * n, s->cws + 4 * n, s->cws + 5 * n}};
+    const int nb = (Hc + CB_BAND - 1) / CB_BAND;
+    int done = 0, cur = 0;
+    while (done < iterations) {
+      const int T = iterations - done < CB_T ? iterations - done : CB_T;
+      const bool last = done + T >= iterations;
+      const float* xin = done == 0 ? e : set[cur][0];
+      float** out = set[done == 0 ? 0 : 1 - cur];
+      k_coarse_cheb_band<<<nb, 1024, 0, s->stream>>>(
+          xin, set[cur][1], set[cur][2], last ? e : out[0], out[1], out[2], rc, coarse_mask,
+          Wc, Hc, done, T, lambda_min);
+      NX_HIP(hipGetLastError());
+      cur = done == 0 ? 0 : 1 - cur;
+      done += T;
+    }
+  } else if (lambda_min > 0.0f)
     k_coarse_smooth<true><<<1, 1024, 0, s->stream>>>(e, rc, coarse_mask, Wc, Hc,
                                                      iterations, lambda_min);
   else

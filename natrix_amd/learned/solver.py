@@ -220,7 +220,7 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
     return kept, abandoned
 
 
-def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=128, max_cycles=6,
+def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=96, max_cycles=6,
                max_outer=12, chunk=65536, max_inner_sweeps=4_000_000, trace=None):
     """One fluid step whose pressure is solved to FP64 relative residual
     <= tol by iterative refinement (pressure held as a float32 pair, outer

@@ -85,3 +85,47 @@ def prolong_ref(ec, mc, f, H, W, solid):
             den = sum(w for w, j, i in ws if not mc[j, i])
             out[y, x] = num / den if den > 0 else 0.0
     return out
+
+
+def lap32(v, solid):
+    """float32 L v in the kernels' summation order (((l + r) + u) + d) - 4c."""
+    f32 = np.float32
+    H, W = v.shape
+    P = np.pad(v, 1, mode="edge")
+    S = np.pad(solid, 1, mode="constant", constant_values=True)
+    nb = []
+    for dy, dx in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+        n = P[1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+        s = S[1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+        nb.append(np.where(s, v, n))
+    return (((nb[0] + nb[1]) + nb[2]) + nb[3]) - f32(4.0) * v
+
+
+def jacobi32(x, rhs, solid, iters):
+    """The coarse Jacobi sweeps' float32 operations: x + (L x - r) * 0.25."""
+    for _ in range(iters):
+        x = x + (lap32(x, solid) - rhs) * np.float32(0.25)
+    return x
+
+
+def cheb32(x, rhs, solid, iters, lmin):
+    """The coarse Chebyshev iteration's float32 operations, in the kernels'
+    order (no fused multiply-add)."""
+    f32 = np.float32
+    lmin = f32(lmin)
+    theta = f32(0.5) * (f32(8.0) + lmin)
+    delta = f32(0.5) * (f32(8.0) - lmin)
+    sigma = theta / delta
+    rho = f32(1.0) / sigma
+    r = lap32(x, solid) - rhs
+    d = r / theta
+    for _ in range(iters):
+        ld = lap32(d, solid)
+        rho1 = f32(1.0) / (f32(2.0) * sigma - rho)
+        cd = rho1 * rho
+        cr = f32(2.0) * rho1 / delta
+        x = x + d
+        r = r + ld
+        d = cd * d + cr * r
+        rho = rho1
+    return x
