@@ -326,7 +326,7 @@ int nx_head_bwd(const void* x, const float* w, const float* dout, void* dx, floa
 
 /* ---- native learned correction (inference; no reference counterpart).
  *      The U-Net of natrix_amd.learned.unet on one coarse grid of H x W
- *      (multiples of 8): conv_w[14] / conv_b[14] are the 3x3 convolutions in
+ *      (multiples of 8, at most 128 x 128): conv_w[14] / conv_b[14] are the 3x3 convolutions in
  *      forward order (e0 a, e0 b, e1 a, e1 b, e2 a, e2 b, e3 a, e3 b, d2 a,
  *      d2 b, d1 a, d1 b, d0 a, d0 b) as bfloat16 KRSC weights and float32
  *      biases, head_w[base] / head_b[1] the 1x1 head, all device pointers
@@ -343,7 +343,9 @@ int nx_unet_forward(nx_unet* net, const void* x, float* out, void* stream);
  * the network, rescale, `coarse_iters` Chebyshev iterations on
  * [lambda_min, 8] (Jacobi sweeps if lambda_min <= 0), remove the fluid
  * mean, interpolate and add with the energy line search (out as for
- * nx_prolong_line_search) */
+ * nx_prolong_line_search).  The coarse-grid part (normalisation to mean
+ * removal) is captured as a HIP graph on the second call with the same
+ * stream and smoothing parameters and replayed from then on. */
 int nx_learned_correct(nx_sim* sim, nx_unet* net, int factor, int coarse_iters,
                        float lambda_min, double* out);
 /* the runner's coarse buffers (rhs, mask, correction), for inspection */

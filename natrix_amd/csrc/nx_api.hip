@@ -794,8 +794,8 @@ __global__ void __launch_bounds__(256)
 // CB_T + 1 rows on each side, whose outer rows go stale one row per stencil,
 // and stores the band's x, r, d.  Launch boundaries are the only global
 // synchronisation.  Every cell sees the operations of the single-workgroup
-// kernel in the same order, and the recurrence coefficient after s0
-// iterations is recomputed by the same float32 recurrence.
+// kernel in the same order; the recurrence coefficient after s0 iterations
+// (rho_s0) comes from the host, computed by the same float32 recurrence.
 constexpr int CB_ROWS = 2, CB_EY = CB_ROWS * CJ_WAVES, CB_T = 8;
 constexpr int CB_BAND = CB_EY - 2 * (CB_T + 1);
 
@@ -804,7 +804,7 @@ __global__ void __launch_bounds__(1024)
                        const float* __restrict__ din, float* __restrict__ xout,
                        float* __restrict__ rout, float* __restrict__ dout,
                        const float* __restrict__ rhs, const uint8_t* __restrict__ mc,
-                       int Wc, int Hc, int s0, int T, float lmin) {
+                       int Wc, int Hc, int s0, int T, float lmin, float rho_s0) {
   __shared__ float top[2][CJ_WAVES][CJ_MAX], bot[2][CJ_WAVES][CJ_MAX];
   CoarseTile<CB_ROWS> t;
   t.top = top;
@@ -835,8 +835,7 @@ __global__ void __launch_bounds__(1024)
     }
   const float theta = 0.5f * (8.0f + lmin), delta = 0.5f * (8.0f - lmin);
   const float sigma = theta / delta;
-  float rho = 1.0f / sigma;
-  for (int k = 0; k < s0; ++k) rho = 1.0f / (2.0f * sigma - rho);
+  float rho = rho_s0;
   if (first) {
     t.lap(x, lv);
 #pragma unroll
@@ -874,6 +873,46 @@ __global__ void __launch_bounds__(1024)
       }
     }
 }
+
+namespace nx {
+int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* rc,
+                         const uint8_t* mc, int iterations, float lambda_min, float* ws) {
+  if (iterations == 0) return NX_OK;
+  if (lambda_min > 0.0f && iterations > CB_T) {
+    // banded: x, r, d ping-pong between the two halves of ws (6 Wc Hc
+    // floats); the last launch writes x to e (read only by the first)
+    if (!ws) return set_error(NX_ERR_ARG, "coarse smoother workspace missing");
+    const size_t n = (size_t)Wc * Hc;
+    float* set[2][3] = {{ws, ws + n, ws + 2 * n}, {ws + 3 * n, ws + 4 * n, ws + 5 * n}};
+    const int nb = (Hc + CB_BAND - 1) / CB_BAND;
+    const float theta = 0.5f * (8.0f + lambda_min), delta = 0.5f * (8.0f - lambda_min);
+    const float sigma = theta / delta;
+    float rho = 1.0f / sigma;
+    int done = 0, cur = 0;
+    while (done < iterations) {
+      const int T = iterations - done < CB_T ? iterations - done : CB_T;
+      const bool last = done + T >= iterations;
+      const float* xin = done == 0 ? e : set[cur][0];
+      float** out = set[done == 0 ? 0 : 1 - cur];
+      k_coarse_cheb_band<<<nb, 1024, 0, st>>>(xin, set[cur][1], set[cur][2],
+                                              last ? e : out[0], out[1], out[2], rc, mc,
+                                              Wc, Hc, done, T, lambda_min, rho);
+      if (hipGetLastError() != hipSuccess)
+        return set_error(NX_ERR_HIP, "coarse smoother launch failed");
+      for (int k = 0; k < T; ++k) rho = 1.0f / (2.0f * sigma - rho);
+      cur = done == 0 ? 0 : 1 - cur;
+      done += T;
+    }
+  } else if (lambda_min > 0.0f) {
+    k_coarse_smooth<true><<<1, 1024, 0, st>>>(e, rc, mc, Wc, Hc, iterations, lambda_min);
+  } else {
+    k_coarse_smooth<false><<<1, 1024, 0, st>>>(e, rc, mc, Wc, Hc, iterations, 0.0f);
+  }
+  if (hipGetLastError() != hipSuccess)
+    return set_error(NX_ERR_HIP, "coarse smoother launch failed");
+  return NX_OK;
+}
+}  // namespace nx
 
 // Energy line search along a correction d.  With L the 5-point operator of
 // the Jacobi sweep (a solid neighbour reads the centre value, an edge
@@ -1507,41 +1546,15 @@ int nx_coarse_smooth(nx_sim* s, int Wc, int Hc, float* e, const float* rc,
   if (Wc < 1 || Hc < 1 || Wc > CJ_MAX || Hc > CJ_ROWS * CJ_WAVES || iterations < 0)
     return fail(NX_ERR_ARG, "coarse grid must be at most 128 x 128, iterations >= 0");
   if (!(lambda_min < 8.0f)) return fail(NX_ERR_ARG, "lambda_min must be below 8");
-  if (iterations == 0) return NX_OK;
-  if (lambda_min > 0.0f && iterations > CB_T) {
-    // banded: x, r, d ping-pong between two sets in the sim's workspace;
-    // the last launch writes x to e (e is read only by the first launch)
-    const size_t n = (size_t)Wc * Hc;
-    if (6 * n > s->cws_cap) {
-      if (s->cws) NX_HIP(hipFree(s->cws));
-      s->cws = nullptr;
-      NX_HIP(hipMalloc(&s->cws, 6 * n * sizeof(float)));
-      s->cws_cap = 6 * n;
-    }
-    float* set[2][3] = {{s->cws, s->cws + n, s->cws + 2 * n},
-                        {s->cws + 3 * n, s->cws + 4 * n, s->cws + 5 * n}};
-    const int nb = (Hc + CB_BAND - 1) / CB_BAND;
-    int done = 0, cur = 0;
-    while (done < iterations) {
-      const int T = iterations - done < CB_T ? iterations - done : CB_T;
-      const bool last = done + T >= iterations;
-      const float* xin = done == 0 ? e : set[cur][0];
-      float** out = set[done == 0 ? 0 : 1 - cur];
-      k_coarse_cheb_band<<<nb, 1024, 0, s->stream>>>(
-          xin, set[cur][1], set[cur][2], last ? e : out[0], out[1], out[2], rc, coarse_mask,
-          Wc, Hc, done, T, lambda_min);
-      NX_HIP(hipGetLastError());
-      cur = done == 0 ? 0 : 1 - cur;
-      done += T;
-    }
-  } else if (lambda_min > 0.0f)
-    k_coarse_smooth<true><<<1, 1024, 0, s->stream>>>(e, rc, coarse_mask, Wc, Hc,
-                                                     iterations, lambda_min);
-  else
-    k_coarse_smooth<false><<<1, 1024, 0, s->stream>>>(e, rc, coarse_mask, Wc, Hc,
-                                                      iterations, 0.0f);
-  NX_HIP(hipGetLastError());
-  return NX_OK;
+  const size_t n = (size_t)Wc * Hc;
+  if (lambda_min > 0.0f && iterations > CB_T && 6 * n > s->cws_cap) {
+    if (s->cws) NX_HIP(hipFree(s->cws));
+    s->cws = nullptr;
+    NX_HIP(hipMalloc(&s->cws, 6 * n * sizeof(float)));
+    s->cws_cap = 6 * n;
+  }
+  return nx::launch_coarse_smooth(s->stream, Wc, Hc, e, rc, coarse_mask, iterations,
+                                  lambda_min, s->cws);
 }
 
 int nx_damped_sweeps(nx_sim* s, int count, float omega) {

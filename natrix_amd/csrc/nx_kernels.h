@@ -83,6 +83,12 @@ bool jacobi_cfg_supported(JacobiCfg cfg);
 // 64 x RES_ROWS block of cells (256 threads, RES_ROWS / 4 rows each):
 // red[5*blk + {sum r, sum r^2, sum b, sum b^2, fluid count}].
 constexpr int RES_ROWS = 32;
+// Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
+// sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 8
+// Chebyshev iterations run banded over several workgroups and need
+// ws = 6 Wc Hc floats.
+int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* rc,
+                         const uint8_t* mc, int iterations, float lambda_min, float* ws);
 int residual_blocks(int W, int H);
 hipError_t launch_sums(const float* p, const float* b, const uint8_t* mask,
                        int W, int H, double* red, hipStream_t s);

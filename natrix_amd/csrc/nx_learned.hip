@@ -14,6 +14,9 @@
 //   5. Chebyshev smoothing of the coarse problem from e (nx_coarse_smooth);
 //   6. k_center_output (one block): e -= fluid mean of e, 0 on solid cells;
 //   7. interpolation and the energy line search (nx_prolong_line_search).
+// Steps 2-6 touch only the runner's buffers; from the second call with the
+// same stream and smoothing parameters they replay as one HIP graph (no
+// memset nodes: every buffer in them is fully written by a kernel).
 // The weights stay owned by the caller (device pointers handed to
 // nx_unet_create); the runner owns its activation buffers, sized for one
 // coarse grid.
@@ -26,10 +29,7 @@
 
 #include "../../include/natrix.h"
 #include "nx_common.h"
-
-namespace nx {
-int set_error(int code, const std::string& msg);
-}
+#include "nx_kernels.h"
 
 struct nx_unet {
   int H = 0, W = 0, base = 0;
@@ -54,6 +54,15 @@ struct nx_unet {
   float* work = nullptr;   // split-K partial sums
   size_t work_elems = 0;
   float* scal = nullptr;   // [0] = s (RMS of rc over fluid cells)
+  float* cws = nullptr;    // Chebyshev workspace, 6 H W floats
+  // steps 2-6 captured as one HIP graph (every pointer in them is the
+  // runner's own), keyed by the stream and the smoothing parameters; the
+  // capture happens on the second call with the same key, when everything
+  // the pipeline touches has been allocated and run once eagerly
+  hipGraphExec_t graph = nullptr;
+  hipStream_t g_stream = nullptr, w_stream = nullptr;
+  int g_iters = -1, w_iters = -1;
+  float g_lmin = 0.0f, w_lmin = 0.0f;
 };
 
 namespace {
@@ -171,6 +180,58 @@ int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s) {
   return NX_OK;
 }
 
+// steps 2-6 of a correction: pack, network, rescale, coarse smoothing,
+// centring (only the runner's buffers)
+int coarse_pipeline(nx_unet* u, int iters, float lmin, hipStream_t s) {
+  const int n = u->H * u->W;
+  k_pack_input<<<1, 1024, 0, s>>>(u->rc, u->mc, u->x0, u->scal, n);
+  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "pack launch failed");
+  int rc = forward(u, u->x0, u->out, s);
+  if (rc != NX_OK) return rc;
+  k_scale_output<<<(n + 255) / 256, 256, 0, s>>>(u->out, u->scal, n);
+  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "scale launch failed");
+  rc = nx::launch_coarse_smooth(s, u->W, u->H, u->out, u->rc, u->mc, iters, lmin, u->cws);
+  if (rc != NX_OK) return rc;
+  k_center_output<<<1, 1024, 0, s>>>(u->out, u->mc, n);
+  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "center launch failed");
+  return NX_OK;
+}
+
+int run_pipeline(nx_unet* u, int iters, float lmin, hipStream_t s) {
+  if (u->graph && u->g_stream == s && u->g_iters == iters && u->g_lmin == lmin)
+    return hipGraphLaunch(u->graph, s) == hipSuccess ? NX_OK
+                                                     : fail(NX_ERR_HIP, "hipGraphLaunch failed");
+  if (u->w_stream != s || u->w_iters != iters || u->w_lmin != lmin) {
+    u->w_stream = s;
+    u->w_iters = iters;
+    u->w_lmin = lmin;
+    return coarse_pipeline(u, iters, lmin, s);
+  }
+  if (u->graph) (void)hipGraphExecDestroy(u->graph);
+  u->graph = nullptr;
+  hipGraph_t g;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(NX_ERR_HIP, "hipStreamBeginCapture failed");
+  const int rc = coarse_pipeline(u, iters, lmin, s);
+  const hipError_t ec = hipStreamEndCapture(s, &g);
+  if (rc != NX_OK) {
+    if (ec == hipSuccess) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(NX_ERR_HIP, "hipStreamEndCapture failed");
+  const hipError_t ei = hipGraphInstantiate(&u->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    u->graph = nullptr;
+    return fail(NX_ERR_HIP, "hipGraphInstantiate failed");
+  }
+  u->g_stream = s;
+  u->g_iters = iters;
+  u->g_lmin = lmin;
+  return hipGraphLaunch(u->graph, s) == hipSuccess ? NX_OK
+                                                   : fail(NX_ERR_HIP, "hipGraphLaunch failed");
+}
+
 }  // namespace
 
 extern "C" {
@@ -181,8 +242,9 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
   if (!out || !conv_w || !conv_b || !head_w || !head_b)
     return fail(NX_ERR_ARG, "null argument");
   *out = nullptr;
-  if (H < 8 || W < 8 || H % 8 || W % 8 || base < 16 || base % 16)
-    return fail(NX_ERR_ARG, "U-Net grid must be a multiple of 8 (>= 8), base a multiple of 16");
+  if (H < 8 || W < 8 || H % 8 || W % 8 || H > 128 || W > 128 || base < 16 || base % 16)
+    return fail(NX_ERR_ARG,
+                "U-Net grid must be a multiple of 8 in [8, 128], base a multiple of 16");
   nx_unet* u = new nx_unet();
   u->H = H;
   u->W = W;
@@ -249,6 +311,7 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
   u->mc = (uint8_t*)alloc(P0);
   u->work = (float*)alloc((u->work_elems ? u->work_elems : 1) * sizeof(float));
   u->scal = (float*)alloc(4 * sizeof(float));
+  u->cws = (float*)alloc(6 * P0 * sizeof(float));
   if (!ok) {
     nx_unet_destroy(u);
     return fail(NX_ERR_HIP, "U-Net workspace allocation failed");
@@ -259,6 +322,7 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
 
 int nx_unet_destroy(nx_unet* u) {
   if (!u) return NX_OK;
+  if (u->graph) (void)hipGraphExecDestroy(u->graph);
   for (void* p : u->bufs) (void)hipFree(p);
   delete u;
   return NX_OK;
@@ -276,7 +340,6 @@ int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
   int rc = nx_get_stream(sim, &sv);
   if (rc != NX_OK) return rc;
   const hipStream_t s = (hipStream_t)sv;
-  const int n = u->H * u->W;
   // the coarse grid must be the network's grid
   int W = 0, H = 0;
   rc = nx_get_size(sim, &W, &H);
@@ -285,16 +348,8 @@ int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
     return fail(NX_ERR_ARG, "coarse grid does not match the network's grid");
   rc = nx_restrict_residual(sim, factor, u->rc, u->mc);
   if (rc != NX_OK) return rc;
-  k_pack_input<<<1, 1024, 0, s>>>(u->rc, u->mc, u->x0, u->scal, n);
-  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "pack launch failed");
-  rc = forward(u, u->x0, u->out, s);
+  rc = run_pipeline(u, coarse_iters, lambda_min, s);
   if (rc != NX_OK) return rc;
-  k_scale_output<<<(n + 255) / 256, 256, 0, s>>>(u->out, u->scal, n);
-  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "scale launch failed");
-  rc = nx_coarse_smooth(sim, u->W, u->H, u->out, u->rc, u->mc, coarse_iters, lambda_min);
-  if (rc != NX_OK) return rc;
-  k_center_output<<<1, 1024, 0, s>>>(u->out, u->mc, n);
-  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "center launch failed");
   return nx_prolong_line_search(sim, factor, u->out, u->mc, out);
 }
 

@@ -319,3 +319,31 @@ def test_native_correction_matches_the_pytorch_path(gpu):
     d_p, a_p = res[False]
     assert np.linalg.norm(d_n - d_p) <= 1e-2 * np.linalg.norm(d_p)
     assert abs(a_n[0] - a_p[0]) <= 1e-2 * abs(a_p[0])
+
+
+def test_native_correction_graph_replays_equal_the_eager_call(gpu):
+    """The first call runs the coarse pipeline eagerly, the second captures
+    and launches it as a HIP graph, later calls replay it: from the same
+    state every call gives the same pressure, bitwise, with eager work and
+    synchronisations between the replays."""
+    from natrix_amd.learned.solver import LearnedGuess
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    g = LearnedGuess(net, meta.get("size", 128))
+    sim = cylinder_sim(256, warm=4)
+    sim._push_params()
+    _lib.call("nx_step_begin", sim._alive(), 1 / 60)
+    p0 = sim.download("pressure")
+    outs = []
+    for k in range(5):
+        sim.upload("pressure", p0)
+        g.correct(sim)
+        outs.append(sim.download("pressure"))
+        # eager work on the same stream between replays
+        _lib.call("nx_damped_sweeps", sim._alive(), 2, 0.5)
+        sim.residual()
+    assert not np.array_equal(outs[0], p0)
+    for k in range(1, 5):
+        assert np.array_equal(outs[k], outs[0]), k
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
