@@ -133,13 +133,13 @@ def make_batch(n_scenes=64, iters=60):
     for i, s in enumerate(solids):
         b.set_obstacle_map(i, s)
     b.set_step_splats(splats)
-    return b, sides
+    return b, sides, solids, splats
 
 
 def bench_batch(args, copy_bandwidth):
     import torch
     iters = 60 if args.iters == 80 else args.iters
-    b, sides = make_batch(iters=iters)
+    b, sides, _, _ = make_batch(iters=iters)
     cells = b.cells
     log(f"bench batch: {len(sides)} scenes, {cells} cells, tile {b.jacobi_tile}")
     b.use_graph = args.graph

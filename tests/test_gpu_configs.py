@@ -1,0 +1,181 @@
+"""The benchmark configurations at their full sizes, through checks whose
+cost the oracle can afford: bitwise equality with the oracle (a 200-step
+rollout at 256², one or two steps at 2048² and 4096²), bitwise equality of
+the fused graph-replayed step with the per-shader path over many steps,
+bitwise equality of sampled scenes of the 64-scene batch with single-grid
+runs, and the FP64 error of the learned solve at 2048²."""
+import numpy as np
+import pytest
+
+from helpers import assert_bitwise, make_sim
+from oracle import fluid as F
+from oracle import mixed as M
+
+pytestmark = pytest.mark.gpu
+
+
+def splats_256(rng, n=3, N=256):
+    out = []
+    for _ in range(n):
+        f = rng.uniform(50, 200) / 60.0
+        a = rng.uniform(0, 2 * np.pi)
+        out.append((rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), f * np.cos(a),
+                    f * np.sin(a), 0.02 * N, 1.0))
+    return out
+
+
+def test_cpu_configuration_200_steps_bitwise(gpu):
+    """256², free-slip box, vorticity 0.3, 40 sweeps from zero, 3 seeded
+    Gaussian splats (radius 0.02 N, |F| ~ U(50, 200), dye 1.0) every step,
+    dye on, 200 steps: GPU and oracle bitwise equal throughout."""
+    N, steps = 256, 200
+    rng = np.random.default_rng(0)
+    splats = splats_256(rng)
+    s32 = [tuple(np.float32(v) for v in s) for s in splats]
+    sim = make_sim(N, N, dye=True, bc_mode="free_slip")
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    sim.iterations = 40
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=40, bc_mode=F.BC_FREE_SLIP)
+    u = np.zeros((N, N, 2), np.float32)
+    p = np.zeros((N, N), np.float32)
+    q = np.zeros((N, N), np.float32)
+    obs = np.zeros((N, N, 2), np.float32)
+    dt = np.float32(1 / 60)
+    for k in range(steps):
+        sim.add_splats(splats)
+        sim.update(dt)
+        u, q = F.gaussian_splat(u, q, s32)
+        u, p, obs, q = F.step(u, p, obs, dt, prm, dye=q)
+        if k in (0, 49, 199):
+            assert_bitwise(sim.download("velocity"), u, f"velocity after {k + 1} steps")
+    assert_bitwise(sim.download("pressure"), p, "pressure after 200 steps")
+    assert_bitwise(sim.download("dye"), q, "dye after 200 steps")
+    assert np.abs(u).max() > 0.1
+    sim.destroy()
+
+
+def cylinder_2048():
+    import bench
+    solid, _ = bench.make_scene(2048)
+    return solid
+
+
+def test_single_scene_configuration_2048(gpu):
+    """2048², cylinder r = 0.1 N, inflow 1.0, vorticity 0.3, 80 sweeps: two
+    steps from rest bitwise equal to the oracle."""
+    N = 2048
+    solid = cylinder_2048()
+    sim = make_sim(N, N, bc_mode="inflow")
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    sim.iterations = 80
+    sim.set_obstacle_map(solid)
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=80, bc_mode=F.BC_INFLOW)
+    u = np.zeros((N, N, 2), np.float32)
+    p = np.zeros((N, N), np.float32)
+    obs = np.zeros((N, N, 2), np.float32)
+    dt = np.float32(1 / 60)
+    for _ in range(2):
+        sim.update(dt)
+        u, p, obs, _ = F.step(u, p, obs, dt, prm, persistent=solid)
+    assert_bitwise(sim.download("velocity"), u, "2048² velocity")
+    assert_bitwise(sim.download("pressure"), p, "2048² pressure")
+    sim.destroy()
+
+
+def test_single_scene_graph_replay_equals_per_shader_path_2048(gpu):
+    """The fused step replayed from a HIP graph and the one-kernel-per-shader
+    path, 20 steps each at the full configuration: bitwise equal."""
+    N = 2048
+    solid = cylinder_2048()
+    out = []
+    for fused in (True, False):
+        sim = make_sim(N, N, bc_mode="inflow", fused=fused)
+        sim.vorticity = 0.3
+        sim.viscosity = 0.0
+        sim.iterations = 80
+        sim.set_obstacle_map(solid)
+        if fused:
+            sim.use_graph = True
+        for _ in range(20):
+            sim.update(1 / 60)
+        out.append((sim.download("velocity"), sim.download("pressure")))
+        sim.destroy()
+    assert_bitwise(out[0][0], out[1][0], "graph vs per-shader velocity")
+    assert_bitwise(out[0][1], out[1][1], "graph vs per-shader pressure")
+
+
+def test_learned_solve_error_against_fp64_2048(gpu):
+    """The learned step at the headline configuration meets FP64 relative
+    residual <= 1e-4, and its pressure is within 1e-3 (relative L2, mean
+    free) of the FP64 conjugate-gradient solution of the same system."""
+    from natrix_amd import fp64
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    N = 2048
+    net, meta = load()
+    sim = make_sim(N, N, bc_mode="inflow")
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    sim.iterations = 80
+    sim.set_obstacle_map(cylinder_2048())
+    for _ in range(10):
+        sim.update(1 / 60)
+    st = solve_step(sim, LearnedGuess(net, meta.get("size", 128)), 1 / 60, tol=1e-4)
+    assert st["converged"] and st["residual"] <= 1e-4
+    x, info = fp64.solve(sim, tol=1e-11, max_iter=100000)
+    assert info["residual"] <= 1e-8  # FP64 CG attains ~2e-9 at 2048²
+    err = fp64.pressure_error(sim, x)
+    print("FP64 relative error of the learned pressure:", err)
+    assert err <= 1e-3
+    sim.destroy()
+
+
+def test_batched_configuration_scenes_equal_single_grids(gpu):
+    """The 64-scene batch of the benchmark (sides 128..1024, rectangles,
+    1-5 splats each, 60 sweeps, free-slip, dye), 3 steps under the captured
+    graph: sampled scenes equal single-grid runs bitwise."""
+    import bench
+    from natrix_amd import FluidSimulator
+    b, sides, solids, splats = bench.make_batch()
+    b.use_graph = True
+    for _ in range(3):
+        b.update(1 / 60)
+    for i in (0, 17, 40, 63):
+        sim = FluidSimulator(sides[i], sides[i], dye=True, bc_mode="free_slip")
+        sim.vorticity = 0.3
+        sim.viscosity = 0.0
+        sim.iterations = 60
+        sim.set_obstacle_map(solids[i])
+        mine = [s[1:] for s in splats if s[0] == i]
+        for _ in range(3):
+            sim.add_splats(mine)
+            sim.update(1 / 60)
+        assert_bitwise(b.download(i, "velocity"), sim.download("velocity"), f"scene {i}")
+        assert_bitwise(b.download(i, "dye"), sim.download("dye"), f"scene {i} dye")
+        sim.destroy()
+    b.destroy()
+
+
+def test_mixed_configuration_4096_step_equals_replay(gpu):
+    """4096², bf16 storage, 8 seeded splats, free-slip, vorticity 0.3, dye:
+    one step (16 sweeps, so the oracle stays within seconds) bitwise equal to
+    the oracle's replay of the reduced-storage dataflow."""
+    import bench
+    N, iters = 4096, 16
+    sim, splats = bench.mixed_scene(N, "bf16", iters)
+    k = sim.jacobi_tile[0]
+    corr = sim.correction_sweeps
+    sim.add_splats(splats)
+    sim.update(1 / 60)
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=iters, bc_mode=F.BC_FREE_SLIP)
+    u = np.zeros((N, N, 2), np.float32)
+    q = np.zeros((N, N), np.float32)
+    obs = np.zeros((N, N, 2), np.float32)
+    u, p, p32, b, q = M.step_mixed(u, obs, np.float32(1 / 60), prm, "bf16", k, corr, dye=q,
+                                   splats=[tuple(np.float32(v) for v in s) for s in splats])
+    assert_bitwise(sim.download("velocity"), u, "4096² bf16 velocity")
+    assert_bitwise(sim.download("pressure32"), p32, "4096² corrected pressure")
+    assert_bitwise(sim.download("dye"), q, "4096² bf16 dye")
+    sim.destroy()
