@@ -6,7 +6,9 @@ candidate runs a fixed number of sweeps on a synthetic scene of the target
 size and is timed with HIP events on the simulator's stream; the fastest
 per sweep wins.  Results are cached per (architecture, dtype, grid) in
 ``tuned.json`` next to this file, which ``FluidSimulator`` consults when it
-is created.
+is created.  A batch of scenes is tuned as a whole (``tune_batch``): its
+best tile depends on the mix of scene sizes, not on the total cell count,
+and is cached under ``batch_key(sizes)`` for ``BatchedScenes``.
 
     python -m natrix_amd.autotune 2048 1024 512   # tunes and updates tuned.json
 """
@@ -45,14 +47,20 @@ def _load():
 
 
 def lookup(W, H, dtype="f32"):
-    """Tuned tile for a W x H grid: the exact entry, else the entry with the
-    nearest cell count, else the built-in default."""
-    table = _load().get(ARCH, {}).get(dtype, {})
+    """Tuned tile for a W x H grid with `dtype` storage: the exact entry of
+    that storage, else the exact float32 entry, else the float32 entry with
+    the nearest cell count, else the built-in default."""
+    data = _load().get(ARCH, {})
     key = f"{W}x{H}"
+    if key in data.get(dtype, {}):
+        return tuple(data[dtype][key]["tile"])
+    table = data.get("f32", {})
     if key in table:
         return tuple(table[key]["tile"])
     best, dist = None, None
     for k, v in table.items():
+        if k.startswith("batch"):
+            continue
         w, h = (int(t) for t in k.split("x"))
         d = abs((w * h) - W * H)
         if dist is None or d < dist:
@@ -100,6 +108,65 @@ def store(W, H, tile, us_per_sweep, dtype="f32"):
     data = _load()
     data.setdefault(ARCH, {}).setdefault(dtype, {})[f"{W}x{H}"] = {
         "tile": list(tile), "us_per_sweep": round(us_per_sweep, 4)}
+    tmp = CACHE + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE)
+
+
+def batch_key(sizes):
+    """Cache key of a batch: scene count and total cells."""
+    return f"batch{len(sizes)}:{sum(w * h for w, h in sizes)}"
+
+
+def lookup_batch(sizes, dtype="f32"):
+    """Tuned tile of a batch of (W, H) scenes: its own entry, else the entry
+    of the single grid nearest its total cell count."""
+    table = _load().get(ARCH, {}).get(dtype, {})
+    key = batch_key(sizes)
+    if key in table:
+        return tuple(table[key]["tile"])
+    side = int(round(sum(w * h for w, h in sizes) ** 0.5))
+    return lookup(side, side, dtype)
+
+
+def tune_batch(batch, sweeps_per_candidate=240, log=print):
+    """Time every candidate on the batch's current state (HIP events on its
+    stream); returns (best tile, results).  The scenes' pressure fields are
+    swept, nothing else changes."""
+    results = []
+    for (k, r, w, c) in candidates():
+        n = max(k, (sweeps_per_candidate // k) * k)
+        batch.jacobi_blocked(2 * k, k, r, w, c)  # warm-up
+        batch.synchronize()
+        batch.set_timing(True)
+        batch.collect_timing()
+        batch.jacobi_blocked(n, k, r, w, c)
+        ms, launches = batch.collect_timing()["jacobi"]
+        batch.set_timing(False)
+        us = 1e3 * ms / n
+        results.append((us, (k, r, w, c)))
+        log(f"  batch k={k:2d} rows={r:2d} waves={w:2d} cols={c}: {us:8.3f} us/sweep")
+    results.sort()
+    return results[0][1], results
+
+
+def store_batch(sizes, tile, us_per_sweep, dtype="f32"):
+    data = _load()
+    data.setdefault(ARCH, {}).setdefault(dtype, {})[batch_key(sizes)] = {
+        "tile": list(tile), "us_per_sweep": round(us_per_sweep, 4)}
+    tmp = CACHE + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE)
+
+
+def store_step_timed(W, H, tile, ms_per_step, dtype):
+    """Entry for reduced storage, chosen by whole-step time (its Jacobi
+    launches include the float32 correction launch)."""
+    data = _load()
+    data.setdefault(ARCH, {}).setdefault(dtype, {})[f"{W}x{H}"] = {
+        "tile": list(tile), "ms_per_step": round(ms_per_step, 4)}
     tmp = CACHE + ".tmp"
     with open(tmp, "w") as f:
         json.dump(data, f, indent=1, sort_keys=True)

@@ -53,10 +53,8 @@ class BatchedScenes:
         self._h = h
         self._base = Params()
         _lib.call("nx_default_params", C.byref(self._base))
-        from .autotune import lookup
-        cells = sum(w * hh for w, hh in self.sizes)
-        side = int(round(cells ** 0.5))
-        self.set_params(jacobi_tile=lookup(side, side))
+        from .autotune import lookup_batch
+        self.set_params(jacobi_tile=lookup_batch(self.sizes))
 
     def __len__(self):
         return len(self.sizes)

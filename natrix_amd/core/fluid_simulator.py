@@ -124,7 +124,7 @@ class FluidSimulator:
         self.check_every = 256
         self.correction_sweeps = 8
         from ..autotune import lookup
-        self.jacobi_tile = lookup(self._width, self._height)
+        self.jacobi_tile = lookup(self._width, self._height, storage)
         self._dye_speed = 500.0
         self._dye_dissipation = 1.0
         self._dye_in_step = True
