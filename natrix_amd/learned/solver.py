@@ -12,17 +12,20 @@ One learned cycle:
      the coarse problem is in coarse grid units) and scale it to unit RMS;
   2. run the U-Net and scale back: the coarse correction's first guess;
   3. Chebyshev-accelerated Jacobi on the coarse problem from that guess
-     (one resident launch): removes the network's error at short coarse
-     wavelengths, where its relative error is largest and which the fine
-     sweeps would need tens of thousands of sweeps to undo;
+     (banded over a few workgroups, 8 iterations per launch): removes the
+     network's error at short coarse wavelengths, where its relative error
+     is largest and which the fine sweeps would need tens of thousands of
+     sweeps to undo;
   4. interpolate to the fine grid and add it scaled by the energy line
      search (the step along the correction that minimises the solve's
      energy functional: it never increases the error in the energy norm,
      and a useless, NaN or infinite correction gets step 0);
-  5. `smooth` fine Jacobi sweeps.
+  5. `smooth` fine Jacobi sweeps, the last one damped (omega = 1/2).
 Cycles repeat while they reduce the residual by a useful factor; then plain
 Jacobi chunks run until the relative residual is <= the target, so the
-step's result satisfies the same criterion as the plain path.
+step's result satisfies the same criterion as the plain path.  With the
+native runner, steps 1-4 are one C-ABI call (nx_learned_correct) whose
+coarse-grid part replays as a HIP graph, and step 5 one more (nx_smooth).
 """
 from __future__ import annotations
 
@@ -146,11 +149,6 @@ class LearnedGuess:
         _lib.call("nx_prolong_line_search", sim._alive(), f, C.c_void_p(e.data_ptr()),
                   C.c_void_p(mc.data_ptr()), out)
         return tuple(out) if report else None
-
-
-def _pressure_view(sim):
-    buf = sim.field_buffer("pressure")
-    return torch.as_tensor(buf, device="cuda")
 
 
 def _solve(sim, tol, max_sweeps):
