@@ -109,6 +109,62 @@ def cpu_baseline(N, iters, budget_s):
     return steps / el, steps, el
 
 
+def config_256_splats(seed=1, n=3, N=256):
+    """The CPU configuration's forcing: 3 Gaussian splats, centres U[0.1, 0.9]^2,
+    radius 0.02 N, |F| ~ U(50, 200) times dt, random direction, dye 1.0."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        f = rng.uniform(50, 200) / 60.0
+        a = rng.uniform(0, 2 * np.pi)
+        out.append((rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), f * np.cos(a),
+                    f * np.sin(a), 0.02 * N, 1.0))
+    return out
+
+
+def config_256(steps=200):
+    """The CPU configuration in full: 256^2, free-slip box, vorticity 0.3, no
+    viscosity, 40 sweeps from zero, dye on, 3 splats every step, 200 steps;
+    seconds on the oracle (NumPy float32, one thread) and on the GPU."""
+    import torch
+    from natrix_amd import FluidSimulator
+    from oracle import fluid as F
+    N = 256
+    splats = config_256_splats()
+    s32 = [tuple(np.float32(v) for v in s) for s in splats]
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=40, bc_mode=F.BC_FREE_SLIP)
+    u = np.zeros((N, N, 2), np.float32)
+    p = np.zeros((N, N), np.float32)
+    q = np.zeros((N, N), np.float32)
+    obs = np.zeros((N, N, 2), np.float32)
+    dt = np.float32(1 / 60)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        u, q = F.gaussian_splat(u, q, s32)
+        u, p, obs, q = F.step(u, p, obs, dt, prm, dye=q)
+    t_cpu = time.perf_counter() - t0
+    sim = FluidSimulator(N, N, dye=True, bc_mode="free_slip")
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    sim.iterations = 40
+    for _ in range(3):
+        sim.add_splats(splats)
+        sim.update(dt)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.add_splats(splats)
+        sim.update(dt)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    sim.destroy()
+    return {"workload": "256x256, free-slip, vorticity 0.3, 40 sweeps, dye, 3 splats/step",
+            "steps": steps, "cpu_seconds": round(t_cpu, 3), "cpu_cores": 1,
+            "gpu_seconds": round(t_gpu, 4),
+            "cpu_steps_per_s": round(steps / t_cpu, 2), "gpu_steps_per_s": round(steps / t_gpu, 1)}
+
+
 BATCH_SIDES = (128, 256, 384, 512, 768, 1024)
 
 
@@ -398,7 +454,8 @@ def bench_learned(args, copy_bandwidth):
                          f"tolerance ({plain_sweeps}"
                          + (", a lower bound: capped" if plain and plain["rate_is_upper_bound"] else "")
                          + ") = the CPU's plain-Jacobi steps/s at the tolerance",
-               "fixed_iteration_steps_per_s": round(sps, 4)}
+               "fixed_iteration_steps_per_s": round(sps, 4),
+               "config_256_200_steps": config_256()}
 
     speedup = (learned["steps_per_s"] / plain["steps_per_s"]) if plain else None
     out = {"metric": METRIC, "value": learned["steps_per_s"], "unit": "steps/s", "n_gpus": 1,

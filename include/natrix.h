@@ -118,6 +118,11 @@ int nx_add_triangle_obstacle(nx_sim* sim, float x1, float y1, float x2,
 /* SmoothParticlesArea.add_particles (smooth_particles_area.py:70-83,
  * shader.AddParticle.comp): cone dye splat clamped to [0,255] */
 int nx_add_dye(nx_sim* sim, float px, float py, float amount, float radius);
+/* the demo's dye display pass (demo/shaders/demo.ComputeShader.comp:9-19):
+ * out[dye_h * dye_w] (device, one uint32 per cell, bytes R,G,B,A) = the dye
+ * value stored to an rgba8 image in every channel: clamp(v, 0, 1) * 255
+ * rounded to nearest (ties up), NaN -> 0; on the simulator's stream */
+int nx_dye_rgba8(nx_sim* sim, void* out);
 /* Gaussian splats (no reference counterpart): n records of 6 floats
  * (px, py, ax, ay, radius, dye); velocity += (ax, ay) * w, dye += dye * w,
  * w = exp(-d^2 / r^2).  Amplitudes are final (already scaled by dt). */

@@ -128,6 +128,9 @@ class FluidSimulator:
         self._dye_speed = 500.0
         self._dye_dissipation = 1.0
         self._dye_in_step = True
+        self._guess = None
+        self._guess_tol = 1e-4
+        self.last_solve = None
         self._use_graph = False
         self._dye_shape = None
         if dye:
@@ -274,10 +277,28 @@ class FluidSimulator:
                       p2[0], p2[1], p3[0], p3[1], 1 if static else 0)
 
     def update(self, time_delta: float):
-        """fluid_simulator.py:174-280 (+ the dye advection, if enabled)."""
+        """fluid_simulator.py:174-280 (+ the dye advection, if enabled).
+        With a learned guess set (set_guess), the pressure is solved by the
+        learned two-grid cycle to the guess's FP64 relative residual, and
+        the step's solver statistics are kept in `last_solve`."""
         if self.simulate:
+            if self._guess is not None:
+                from ..learned.solver import solve_step
+                self.last_solve = solve_step(self, self._guess, time_delta,
+                                             tol=self._guess_tol)
+                return
             self._push_params()
             _lib.call("nx_update", self._alive(), time_delta)
+
+    def set_guess(self, guess, tol=1e-4):
+        """Solve the pressure of every update() with `guess` (a
+        natrix_amd.learned.solver.LearnedGuess) to FP64 relative residual
+        <= tol, by iterative refinement with learned cycles and Jacobi;
+        None restores the fixed-sweep (or `tol`-driven) Jacobi step."""
+        if tol is not None and not (tol > 0):
+            raise ValueError("tol must be positive")
+        self._guess = guess
+        self._guess_tol = float(tol)
 
     def destroy(self):
         """fluid_simulator.py:476-515: release every device resource."""
@@ -365,6 +386,18 @@ class FluidSimulator:
         if self._dye_shape is None:
             raise _lib.NatrixError("dye is not enabled")
         return self._dye_shape
+
+    def dye_rgba8(self, device=False):
+        """The dye as an RGBA8 image (demo/shaders/demo.ComputeShader.comp:9-19:
+        every channel = clamp(dye, 0, 1) * 255 rounded): a (dye_h, dye_w, 4)
+        uint8 numpy array, or with `device` a torch tensor on the GPU."""
+        import torch
+        h, w = self.dye_shape
+        out = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        _lib.call("nx_dye_rgba8", self._alive(), C.c_void_p(out.data_ptr()))
+        self.synchronize()
+        return out if device else out.cpu().numpy()
 
     def download(self, name):
         fid = _FIELD_NAMES[name] if isinstance(name, str) else int(name)

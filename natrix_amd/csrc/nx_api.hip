@@ -914,6 +914,19 @@ int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* 
 }
 }  // namespace nx
 
+// Dye as an RGBA8 image, demo/shaders/demo.ComputeShader.comp:9-19: every
+// channel is the dye value stored to an rgba8 (unsigned normalised) image,
+// i.e. clamp(v, 0, 1) * 255 rounded to nearest (ties up; a NaN stores 0).
+template <class S>
+__global__ void k_dye_rgba8(const typename S::s1* __restrict__ q, uint32_t* __restrict__ out,
+                            int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float v = fminf(fmaxf(S::ld(q, i), 0.0f), 1.0f);
+  const uint32_t c = (uint32_t)floorf(v * 255.0f + 0.5f);
+  out[i] = c | (c << 8) | (c << 16) | (c << 24);
+}
+
 // Energy line search along a correction d.  With L the 5-point operator of
 // the Jacobi sweep (a solid neighbour reads the centre value, an edge
 // neighbour is the cell itself), -L is symmetric positive semi-definite on
@@ -1286,6 +1299,24 @@ int nx_add_triangle_obstacle(nx_sim* s, float x1, float y1, float x2,
   if (!s) return fail(NX_ERR_ARG, "null simulator");
   NX_HIP(launch_triangle(s->mask[s->mcur], s->W, s->H, x1, y1, x2, y2, x3, y3,
                          is_static ? 1 : 0, s->stream));
+  return NX_OK;
+}
+
+int nx_dye_rgba8(nx_sim* s, void* out) {
+  if (!s || !out) return fail(NX_ERR_ARG, "null argument");
+  if (!s->dye[0]) return fail(NX_ERR_STATE, "dye is not enabled");
+  const int n = s->dyeW * s->dyeH;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (s->storage == ST_F32)
+    k_dye_rgba8<F32S><<<blocks, 256, 0, s->stream>>>((const float*)s->dye[s->dcur],
+                                                      (uint32_t*)out, n);
+  else if (s->storage == ST_BF16)
+    k_dye_rgba8<BF16S><<<blocks, 256, 0, s->stream>>>(
+        (const BF16S::s1*)s->dye[s->dcur], (uint32_t*)out, n);
+  else
+    k_dye_rgba8<F16S><<<blocks, 256, 0, s->stream>>>(
+        (const F16S::s1*)s->dye[s->dcur], (uint32_t*)out, n);
+  NX_HIP(hipGetLastError());
   return NX_OK;
 }
 

@@ -315,6 +315,17 @@ def advect_dye(q, u, obs, dt, speed, dissipation):
 
 
 # ---------------------------------------------------------------- F4
+def dye_rgba8(q):
+    """demo/shaders/demo.ComputeShader.comp:9-19: the dye value stored to an
+    rgba8 image in all four channels, i.e. round(clamp(v, 0, 1) * 255) in
+    float32 with ties rounded up; a NaN stores 0."""
+    f = np.float32
+    v = np.nan_to_num(q.astype(f), nan=0.0, posinf=1.0, neginf=0.0)
+    v = np.minimum(np.maximum(v, f(0.0)), f(1.0))
+    c = np.floor(v * f(255.0) + f(0.5)).astype(np.uint8)
+    return np.repeat(c[..., None], 4, axis=-1)
+
+
 def add_dye(q, position, value, radius):
     """F4: cone dye splat, clamp [0,255] inside the disk
     (demo/shaders/shader.AddParticle.comp:24-34)."""

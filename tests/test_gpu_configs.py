@@ -14,23 +14,13 @@ from oracle import mixed as M
 pytestmark = pytest.mark.gpu
 
 
-def splats_256(rng, n=3, N=256):
-    out = []
-    for _ in range(n):
-        f = rng.uniform(50, 200) / 60.0
-        a = rng.uniform(0, 2 * np.pi)
-        out.append((rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), f * np.cos(a),
-                    f * np.sin(a), 0.02 * N, 1.0))
-    return out
-
-
 def test_cpu_configuration_200_steps_bitwise(gpu):
     """256², free-slip box, vorticity 0.3, 40 sweeps from zero, 3 seeded
     Gaussian splats (radius 0.02 N, |F| ~ U(50, 200), dye 1.0) every step,
     dye on, 200 steps: GPU and oracle bitwise equal throughout."""
+    import bench
     N, steps = 256, 200
-    rng = np.random.default_rng(0)
-    splats = splats_256(rng)
+    splats = bench.config_256_splats()
     s32 = [tuple(np.float32(v) for v in s) for s in splats]
     sim = make_sim(N, N, dye=True, bc_mode="free_slip")
     sim.vorticity = 0.3

@@ -347,3 +347,28 @@ def test_native_correction_graph_replays_equal_the_eager_call(gpu):
         assert np.array_equal(outs[k], outs[0]), k
     _lib.call("nx_step_finish", sim._alive())
     sim.destroy()
+
+
+def test_set_guess_routes_update_through_the_learned_step(gpu):
+    """FluidSimulator.set_guess makes update() the learned step: the same
+    result, bitwise, as solve_step on a twin simulator, and None restores
+    the fixed-sweep step."""
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    g = LearnedGuess(net, meta.get("size", 128))
+    a, b = cylinder_sim(256, warm=3), cylinder_sim(256, warm=3)
+    a.set_guess(g, tol=1e-4)
+    for _ in range(2):
+        a.update(1 / 60)
+        st = solve_step(b, g, 1 / 60, tol=1e-4)
+    assert a.last_solve["converged"] and a.last_solve == st
+    assert np.array_equal(a.download("velocity"), b.download("velocity"))
+    a.set_guess(None)
+    b.update(1 / 60)
+    a.update(1 / 60)
+    assert np.array_equal(a.download("velocity"), b.download("velocity"))
+    with pytest.raises(ValueError):
+        a.set_guess(g, tol=0.0)
+    a.destroy()
+    b.destroy()

@@ -222,3 +222,20 @@ def test_fused_vort_div_equals_stage_chain(scene, nu):
     assert_bitwise(sim.download("divergence"), b, "fused divergence")
     p1 = F.jacobi(np.zeros_like(b), b, obs, 1)
     assert_bitwise(sim.download("pressure"), p1, "fused p1")
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_dye_rgba8_export(gpu, rng, storage):
+    """The dye display pass: bitwise equal to the oracle's rgba8 conversion
+    of the stored dye, on a dye grid of its own size."""
+    from natrix_amd import FluidSimulator
+    sim = FluidSimulator(40, 30, dye=(77, 50), storage=storage) if storage == "f32" else \
+        FluidSimulator(40, 30, dye=True, storage=storage)
+    h, w = sim.dye_shape
+    q = rng.uniform(-0.5, 1.5, (h, w)).astype(np.float32)
+    q[0, 0] = np.nan
+    sim.upload("dye", q)
+    stored = sim.download("dye")
+    ref = F.dye_rgba8(stored)
+    assert np.array_equal(sim.dye_rgba8(), ref)
+    sim.destroy()

@@ -446,3 +446,17 @@ def test_one_pass_sums_agree_with_two_pass(rng):
     one = F.relative_residual_from_sums(*F.residual_sums(p, b, obs))
     two = F.relative_residual(p, b, obs)
     assert abs(one - two) < 1e-9 * max(1.0, two)
+
+
+def test_dye_rgba8_unorm_conversion():
+    """rgba8 image store of a float: clamp to [0, 1], x255, round to nearest
+    (ties up); all four channels equal (demo.ComputeShader.comp:18)."""
+    q = np.array([[0.0, 1.0, 0.5, -3.0], [2.0, 255.0, 1.5 / 255, np.nan]], np.float32)
+    got = F.dye_rgba8(q)
+    assert got.shape == (2, 4, 4) and got.dtype == np.uint8
+    want = np.array([[0, 255, 128, 0], [255, 255, 2, 0]], np.uint8)
+    assert np.array_equal(got[..., 0], want)
+    assert all(np.array_equal(got[..., c], got[..., 0]) for c in range(4))
+    # every representable 8-bit level round-trips
+    lv = (np.arange(256, dtype=np.float32) / np.float32(255.0))
+    assert np.array_equal(F.dye_rgba8(lv)[..., 0], np.arange(256, dtype=np.uint8))
