@@ -268,6 +268,13 @@ int nx_batch_set_params(nx_batch* b, const nx_params* prm);
  * walls, inflow, dye speed/dissipation) */
 int nx_batch_set_scene_params(nx_batch* b, int scene, const nx_params* prm);
 int nx_batch_set_obstacle_map(nx_batch* b, int scene, const uint8_t* map);
+/* per-step obstacles of one scene, as nx_add_circle_obstacle /
+ * nx_add_triangle_obstacle (fluid_simulator.py:135-172): marked until the
+ * end-of-step clear of the next nx_batch_update */
+int nx_batch_add_circle_obstacle(nx_batch* b, int scene, float px, float py, float radius,
+                                 int is_static);
+int nx_batch_add_triangle_obstacle(nx_batch* b, int scene, float x1, float y1, float x2,
+                                   float y2, float x3, float y3, int is_static);
 /* Gaussian splats applied at the start of every nx_batch_update: n records
  * of 7 floats (scene, px, py, ax, ay, radius, dye) */
 int nx_batch_set_step_splats(nx_batch* b, const float* rec, int n);

@@ -122,6 +122,8 @@ SIGNATURES = {
     "nx_batch_set_params": [_P, C.POINTER(Params)],
     "nx_batch_set_scene_params": [_P, _I, C.POINTER(Params)],
     "nx_batch_set_obstacle_map": [_P, _I, C.POINTER(C.c_uint8)],
+    "nx_batch_add_circle_obstacle": [_P, _I, _F, _F, _F, _I],
+    "nx_batch_add_triangle_obstacle": [_P, _I, _F, _F, _F, _F, _F, _F, _I],
     "nx_batch_set_step_splats": [_P, C.POINTER(_F), _I],
     "nx_batch_update": [_P, _F],
     "nx_batch_set_graph": [_P, _I],

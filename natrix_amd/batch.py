@@ -113,6 +113,16 @@ class BatchedScenes:
         self._graph = bool(on)
         _lib.call("nx_batch_set_graph", self._alive(), int(self._graph))
 
+    def add_circle_obstacle(self, scene, position, radius, static=False):
+        """FluidSimulator.add_circle_obstacle for one scene (this step)."""
+        _lib.call("nx_batch_add_circle_obstacle", self._alive(), int(scene),
+                  float(position[0]), float(position[1]), float(radius), 1 if static else 0)
+
+    def add_triangle_obstacle(self, scene, p1, p2, p3, static=False):
+        """FluidSimulator.add_triangle_obstacle for one scene (this step)."""
+        _lib.call("nx_batch_add_triangle_obstacle", self._alive(), int(scene),
+                  *(float(v) for v in (*p1, *p2, *p3)), 1 if static else 0)
+
     def jacobi_blocked(self, sweeps, k, rows, waves, cols):
         _lib.call("nx_batch_jacobi_blocked", self._alive(), int(sweeps), int(k),
                   int(rows), int(waves), int(cols))

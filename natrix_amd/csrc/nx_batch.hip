@@ -507,6 +507,23 @@ int nx_batch_set_obstacle_map(nx_batch* b, int scene, const uint8_t* map) {
   return NX_OK;
 }
 
+int nx_batch_add_circle_obstacle(nx_batch* b, int scene, float px, float py, float radius,
+                                 int is_static) {
+  NXB_TRY(check_scene(b, scene));
+  (void)is_static;  // both branches of AddCircleObstacle.comp:28-35 agree
+  NXB_HIP(launch_circle(b->mask[b->mcur] + b->off[scene], b->W[scene], b->H[scene], px, py,
+                        radius, b->stream));
+  return NX_OK;
+}
+
+int nx_batch_add_triangle_obstacle(nx_batch* b, int scene, float x1, float y1, float x2,
+                                   float y2, float x3, float y3, int is_static) {
+  NXB_TRY(check_scene(b, scene));
+  NXB_HIP(launch_triangle(b->mask[b->mcur] + b->off[scene], b->W[scene], b->H[scene], x1, y1,
+                          x2, y2, x3, y3, is_static ? 1 : 0, b->stream));
+  return NX_OK;
+}
+
 int nx_batch_set_step_splats(nx_batch* b, const float* rec, int n) {
   if (!b || n < 0 || (n > 0 && !rec)) return bfail(NX_ERR_ARG, "bad splat list");
   std::vector<std::pair<int, Splat>> v;
