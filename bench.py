@@ -87,6 +87,17 @@ def configure(sim, args):
         sim.check_every = args.check_every
 
 
+def threadpools():
+    """Host thread pools (threadpoolctl): the oracle's NumPy ufuncs run on one
+    thread whatever these report."""
+    try:
+        import threadpoolctl
+        return [{"api": d.get("internal_api"), "threads": d.get("num_threads")}
+                for d in threadpoolctl.threadpool_info()]
+    except Exception as e:  # threadpoolctl missing or failing: report why
+        return [{"error": type(e).__name__}]
+
+
 def cpu_baseline(N, iters, budget_s):
     """The oracle (NumPy float32, one thread) on the same scene: whole steps
     until the time budget is spent (at least one)."""
@@ -455,6 +466,7 @@ def bench_learned(args, copy_bandwidth):
                          + (", a lower bound: capped" if plain and plain["rate_is_upper_bound"] else "")
                          + ") = the CPU's plain-Jacobi steps/s at the tolerance",
                "fixed_iteration_steps_per_s": round(sps, 4),
+               "threadpools": threadpools(),
                "config_256_200_steps": config_256()}
 
     speedup = (learned["steps_per_s"] / plain["steps_per_s"]) if plain else None
