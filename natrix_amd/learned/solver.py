@@ -97,16 +97,30 @@ class LearnedGuess:
         self.coarse = coarse
         self.coarse_iters = coarse_iters
         self.lmin = chebyshev_lmin(coarse_iters, damping) if coarse_iters else 0.0
-        self._runner = None
-        if native and isinstance(net, UNet):
-            self._runner = NativeUNet(net, coarse, coarse)
+        self._native = native and isinstance(net, UNet)
+        self._runners = {}
 
-    def _factor(self, sim):
-        W, H = sim.width, sim.height
-        if W != H or W % self.coarse:
-            raise ValueError("the learned guess needs a square grid divisible by "
-                             f"{self.coarse}")
-        return W // self.coarse
+    def coarse_grid(self, W, H):
+        """(f, Wc, Hc): the smallest factor f whose coarse grid
+        (ceil(W / f), ceil(H / f)) fits in `coarse` x `coarse` with both
+        sides multiples of 8 (the network pools three times); a square grid
+        divisible by `coarse` gets exactly `coarse` x `coarse`."""
+        c = self.coarse
+        if W == H and W % c == 0:
+            return W // c, c, c
+        for f in range(max(1, -(-max(W, H) // c)), max(W, H) + 1):
+            wc, hc = -(-W // f), -(-H // f)
+            if wc % 8 == 0 and hc % 8 == 0 and wc <= c and hc <= c:
+                return f, wc, hc
+        raise ValueError(f"no coarse grid of multiples of 8 within {c} x {c} for {W} x {H}")
+
+    def _runner(self, wc, hc):
+        if not self._native:
+            return None
+        r = self._runners.get((wc, hc))
+        if r is None:
+            r = self._runners[(wc, hc)] = NativeUNet(self.net, hc, wc)
+        return r
 
     @torch.no_grad()
     def correct(self, sim, report=False, line_search=True):
@@ -118,26 +132,26 @@ class LearnedGuess:
             return self._correct(sim, report, line_search)
 
     def _correct(self, sim, report, line_search):
-        f = self._factor(sim)
-        c = self.coarse
-        if self._runner is not None and line_search:
+        f, wc, hc = self.coarse_grid(sim.width, sim.height)
+        runner = self._runner(wc, hc)
+        if runner is not None and line_search:
             out = (C.c_double * 3)() if report else None
-            _lib.call("nx_learned_correct", sim._alive(), self._runner.handle, f,
+            _lib.call("nx_learned_correct", sim._alive(), runner.handle, f,
                       int(self.coarse_iters), float(self.lmin), out)
             return tuple(out) if report else None
-        rc = torch.empty(c * c, dtype=torch.float32, device="cuda")
-        mc = torch.empty(c * c, dtype=torch.uint8, device="cuda")
+        rc = torch.empty(hc * wc, dtype=torch.float32, device="cuda")
+        mc = torch.empty(hc * wc, dtype=torch.uint8, device="cuda")
         _lib.call("nx_restrict_residual", sim._alive(), f, C.c_void_p(rc.data_ptr()),
                   C.c_void_p(mc.data_ptr()))
         fluid = mc == 0
         n = fluid.sum().clamp_min(1)
         s = torch.sqrt((rc * rc * fluid).sum() / n).clamp_min(1e-30)
         if self.net is not None:
-            x = pack_input((rc / s).view(1, c, c), (~fluid).view(1, c, c))
+            x = pack_input((rc / s).view(1, hc, wc), (~fluid).view(1, hc, wc))
             e = (self.net(x).view(-1) * s).contiguous()
         else:
             e = torch.zeros_like(rc)
-        _lib.call("nx_coarse_smooth", sim._alive(), c, c, C.c_void_p(e.data_ptr()),
+        _lib.call("nx_coarse_smooth", sim._alive(), wc, hc, C.c_void_p(e.data_ptr()),
                   C.c_void_p(rc.data_ptr()), C.c_void_p(mc.data_ptr()),
                   int(self.coarse_iters), float(self.lmin))
         e = (e - (e * fluid).sum() / n) * fluid

@@ -372,3 +372,33 @@ def test_set_guess_routes_update_through_the_learned_step(gpu):
         a.set_guess(g, tol=0.0)
     a.destroy()
     b.destroy()
+
+
+def test_learned_step_on_a_window_shaped_grid(gpu):
+    """640 x 360 (the demo's velocity grid for a 1280 x 720 window): the
+    coarse grid is 128 x 72 (f = 5, a factor that is not a power of two);
+    the learned step reaches the tolerance with a third of plain Jacobi's
+    sweeps or fewer."""
+    from natrix_amd import FluidSimulator
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    g = LearnedGuess(net, meta.get("size", 128))
+    assert g.coarse_grid(640, 360) == (5, 128, 72)
+    res = {}
+    for name, guess in (("learned", g), ("plain", None)):
+        sim = FluidSimulator(640, 360, bc_mode="inflow")
+        sim.vorticity, sim.viscosity = 0.3, 0.0
+        X = np.arange(640)[None, :]
+        Y = np.arange(360)[:, None]
+        sim.set_obstacle_map((X - 200) ** 2 + (Y - 170) ** 2 <= 36 ** 2)
+        for _ in range(5):
+            sim.update(1 / 60)
+        st = solve_step(sim, guess, 1 / 60, tol=1e-4)
+        assert st["converged"] and st["residual"] <= 1e-4, (name, st)
+        res[name] = st["sweeps"]
+        sim.destroy()
+    print(res)
+    assert res["learned"] * 3 <= res["plain"]
+    with pytest.raises(ValueError):
+        g.coarse_grid(1000, 700)
