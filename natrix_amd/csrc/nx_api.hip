@@ -579,19 +579,26 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Coarse cells of a padded coarse grid (Wp x Hp beyond the Wv x Hv cells
+// that cover the fine grid) are solid with a zero right-hand side.
 __global__ void __launch_bounds__(256)
     k_restrict_cols(const float* __restrict__ rsum, const float* __restrict__ rcnt,
                     float* __restrict__ rc, uint8_t* __restrict__ mc, int H, int f,
-                    int Wc, int Hc) {
+                    int Wv, int Hv, int Wp, int Hp) {
   const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y * 4 + threadIdx.y;
-  if (X >= Wc || Y >= Hc) return;
+  if (X >= Wp || Y >= Hp) return;
+  if (X >= Wv || Y >= Hv) {
+    rc[Y * Wp + X] = 0.0f;
+    mc[Y * Wp + X] = 1;
+    return;
+  }
   float sum = 0.0f, cnt = 0.0f;
   for (int y = Y * f; y < min(Y * f + f, H); ++y) {
-    sum += rsum[y * Wc + X];
-    cnt += rcnt[y * Wc + X];
+    sum += rsum[y * Wv + X];
+    cnt += rcnt[y * Wv + X];
   }
-  rc[Y * Wc + X] = cnt == 0.0f ? 0.0f : (float)(f * f) * (sum / cnt);
-  mc[Y * Wc + X] = cnt == 0.0f ? 1 : 0;
+  rc[Y * Wp + X] = cnt == 0.0f ? 0.0f : (float)(f * f) * (sum / cnt);
+  mc[Y * Wp + X] = cnt == 0.0f ? 1 : 0;
 }
 
 // Prolongation: p += scale * e interpolated at the fine cell, on fluid
@@ -1529,12 +1536,14 @@ int nx_smooth(nx_sim* s, int sweeps, float omega, double* out) {
   return NX_OK;
 }
 
-int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
+// restriction onto a Wp x Hp coarse grid, Wp >= ceil(W / f), Hp >= ceil(H / f)
+static int restrict_impl(nx_sim* s, int factor, int Wp, int Hp, float* coarse_rhs,
                          uint8_t* coarse_mask) {
   if (!s || !coarse_rhs || !coarse_mask || factor < 1)
     return fail(NX_ERR_ARG, "bad restriction arguments");
   if (!s->in_step) return fail(NX_ERR_STATE, "restriction outside a step");
   const int Wc = (s->W + factor - 1) / factor, Hc = (s->H + factor - 1) / factor;
+  if (Wp < Wc || Hp < Hc) return fail(NX_ERR_ARG, "coarse grid smaller than the fine grid / f");
   const size_t need = (size_t)2 * s->H * Wc;
   if (need > s->rrow_cap) {
     if (s->rrow) NX_HIP(hipFree(s->rrow));
@@ -1552,10 +1561,17 @@ int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
     k_restrict_rows<<<dim3((Wc + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0, s->stream>>>(
         s->p[s->pcur], s->div, s->mask[s->mcur], rsum, rcnt, s->W, s->H, factor, Wc);
   NX_HIP(hipGetLastError());
-  k_restrict_cols<<<dim3((Wc + 63) / 64, (Hc + 3) / 4), dim3(64, 4), 0, s->stream>>>(
-      rsum, rcnt, coarse_rhs, coarse_mask, s->H, factor, Wc, Hc);
+  k_restrict_cols<<<dim3((Wp + 63) / 64, (Hp + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+      rsum, rcnt, coarse_rhs, coarse_mask, s->H, factor, Wc, Hc, Wp, Hp);
   NX_HIP(hipGetLastError());
   return NX_OK;
+}
+
+int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
+                         uint8_t* coarse_mask) {
+  if (!s || factor < 1) return fail(NX_ERR_ARG, "bad restriction arguments");
+  return restrict_impl(s, factor, (s->W + factor - 1) / factor, (s->H + factor - 1) / factor,
+                       coarse_rhs, coarse_mask);
 }
 
 int nx_prolong_add(nx_sim* s, int factor, const float* coarse,
@@ -1638,18 +1654,28 @@ int nx_line_search(nx_sim* s, const float* d, double* out) {
   return line_search(s, d, out);
 }
 
-int nx_prolong_line_search(nx_sim* s, int factor, const float* coarse,
+// interpolation from a Wp x Hp coarse grid (padded cells must be solid) and
+// the energy line search
+static int prolong_ls_impl(nx_sim* s, int factor, int Wp, int Hp, const float* coarse,
                            const uint8_t* coarse_mask, double* out) {
   if (!s || !coarse || !coarse_mask || factor < 1)
     return fail(NX_ERR_ARG, "bad prolongation arguments");
   if (!s->in_step) return fail(NX_ERR_STATE, "prolongation outside a step");
-  NX_TRY(ensure_line(s));
   const int Wc = (s->W + factor - 1) / factor, Hc = (s->H + factor - 1) / factor;
+  if (Wp < Wc || Hp < Hc) return fail(NX_ERR_ARG, "coarse grid smaller than the fine grid / f");
+  NX_TRY(ensure_line(s));
   const dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
   k_prolong<false><<<grid, dim3(64, 4), 0, s->stream>>>(
-      s->dcorr, s->mask[s->mcur], coarse, coarse_mask, s->W, s->H, factor, Wc, Hc, 1.0f);
+      s->dcorr, s->mask[s->mcur], coarse, coarse_mask, s->W, s->H, factor, Wp, Hp, 1.0f);
   NX_HIP(hipGetLastError());
   return line_search(s, s->dcorr, out);
+}
+
+int nx_prolong_line_search(nx_sim* s, int factor, const float* coarse,
+                           const uint8_t* coarse_mask, double* out) {
+  if (!s || factor < 1) return fail(NX_ERR_ARG, "bad prolongation arguments");
+  return prolong_ls_impl(s, factor, (s->W + factor - 1) / factor,
+                         (s->H + factor - 1) / factor, coarse, coarse_mask, out);
 }
 
 int nx_refine_begin(nx_sim* s) {
@@ -2021,3 +2047,13 @@ int nx_collect_timing(nx_sim* s, double* ms, int* launches) {
 }
 
 }  // extern "C"
+
+namespace nx {
+int restrict_residual_padded(nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc) {
+  return restrict_impl(s, factor, Wp, Hp, rc, mc);
+}
+int prolong_line_search_padded(nx_sim* s, int factor, int Wp, int Hp, const float* e,
+                               const uint8_t* mc, double* out) {
+  return prolong_ls_impl(s, factor, Wp, Hp, e, mc, out);
+}
+}  // namespace nx

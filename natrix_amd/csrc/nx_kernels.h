@@ -8,6 +8,8 @@
 
 #include "nx_cells.h"
 
+struct nx_sim;
+
 namespace nx {
 
 // record `msg` as the calling thread's nx_last_error() and return `code`
@@ -83,6 +85,12 @@ bool jacobi_cfg_supported(JacobiCfg cfg);
 // 64 x RES_ROWS block of cells (256 threads, RES_ROWS / 4 rows each):
 // red[5*blk + {sum r, sum r^2, sum b, sum b^2, fluid count}].
 constexpr int RES_ROWS = 32;
+// Restriction of the residual onto, and interpolation with the energy line
+// search from, a Wp x Hp coarse grid padded beyond ceil(W / f) x ceil(H / f)
+// (padded coarse cells are solid, with a zero right-hand side).
+int restrict_residual_padded(::nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc);
+int prolong_line_search_padded(::nx_sim* s, int factor, int Wp, int Hp, const float* e,
+                               const uint8_t* mc, double* out);
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
 // sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 8
 // Chebyshev iterations run banded over several workgroups and need

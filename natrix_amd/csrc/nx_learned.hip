@@ -3,7 +3,8 @@
 // One call of nx_learned_correct enqueues, on the simulator's stream and
 // with no host synchronisation, the whole correction of one learned cycle:
 //   1. restriction of the current residual to the coarse grid
-//      (nx_restrict_residual: rc = f^2 x block mean, coarse mask);
+//      (rc = f^2 x block mean, coarse mask; the network's grid may extend
+//      beyond ceil(W / f) x ceil(H / f), the extra cells solid);
 //   2. k_pack_input (one block): s = RMS of rc over fluid coarse cells,
 //      network input x[p] = (bf16(rc/s), bf16(solid), 0 x 6);
 //   3. the U-Net forward (4 levels, 3x3 conv + ReLU pairs, 2x2 average pool,
@@ -344,13 +345,13 @@ int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
   int W = 0, H = 0;
   rc = nx_get_size(sim, &W, &H);
   if (rc != NX_OK) return rc;
-  if ((W + factor - 1) / factor != u->W || (H + factor - 1) / factor != u->H)
-    return fail(NX_ERR_ARG, "coarse grid does not match the network's grid");
-  rc = nx_restrict_residual(sim, factor, u->rc, u->mc);
+  if ((W + factor - 1) / factor > u->W || (H + factor - 1) / factor > u->H)
+    return fail(NX_ERR_ARG, "the network's grid is smaller than the fine grid / factor");
+  rc = nx::restrict_residual_padded(sim, factor, u->W, u->H, u->rc, u->mc);
   if (rc != NX_OK) return rc;
   rc = run_pipeline(u, coarse_iters, lambda_min, s);
   if (rc != NX_OK) return rc;
-  return nx_prolong_line_search(sim, factor, u->out, u->mc, out);
+  return nx::prolong_line_search_padded(sim, factor, u->W, u->H, u->out, u->mc, out);
 }
 
 int nx_unet_coarse(nx_unet* u, float** rc, uint8_t** mc, float** e) {

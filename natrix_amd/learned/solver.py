@@ -94,6 +94,8 @@ class LearnedGuess:
 
     def __init__(self, net, coarse=128, coarse_iters=100, damping=1e-3, native=True):
         self.net = net.eval() if net is not None else None
+        if coarse < 8 or coarse % 8:
+            raise ValueError(f"coarse grid side {coarse} is not a positive multiple of 8")
         self.coarse = coarse
         self.coarse_iters = coarse_iters
         self.lmin = chebyshev_lmin(coarse_iters, damping) if coarse_iters else 0.0
@@ -101,18 +103,15 @@ class LearnedGuess:
         self._runners = {}
 
     def coarse_grid(self, W, H):
-        """(f, Wc, Hc): the smallest factor f whose coarse grid
-        (ceil(W / f), ceil(H / f)) fits in `coarse` x `coarse` with both
-        sides multiples of 8 (the network pools three times); a square grid
-        divisible by `coarse` gets exactly `coarse` x `coarse`."""
+        """(f, Wc, Hc): f = ceil(max(W, H) / coarse), the smallest factor
+        whose cells ceil(W / f) x ceil(H / f) fit in `coarse` x `coarse`;
+        the network's grid Wc x Hc rounds both sides up to multiples of 8
+        (the network pools three times).  Coarse cells beyond the fine
+        grid are solid with a zero right-hand side; only the native runner
+        takes such a padded grid."""
         c = self.coarse
-        if W == H and W % c == 0:
-            return W // c, c, c
-        for f in range(max(1, -(-max(W, H) // c)), max(W, H) + 1):
-            wc, hc = -(-W // f), -(-H // f)
-            if wc % 8 == 0 and hc % 8 == 0 and wc <= c and hc <= c:
-                return f, wc, hc
-        raise ValueError(f"no coarse grid of multiples of 8 within {c} x {c} for {W} x {H}")
+        f = max(1, -(-max(W, H) // c))
+        return f, -(-(-(-W // f)) // 8) * 8, -(-(-(-H // f)) // 8) * 8
 
     def _runner(self, wc, hc):
         if not self._native:
@@ -139,6 +138,9 @@ class LearnedGuess:
             _lib.call("nx_learned_correct", sim._alive(), runner.handle, f,
                       int(self.coarse_iters), float(self.lmin), out)
             return tuple(out) if report else None
+        if -(-sim.width // f) != wc or -(-sim.height // f) != hc:
+            raise ValueError(f"a {sim.width} x {sim.height} grid needs a padded coarse grid "
+                             f"({wc} x {hc} for f = {f}), which only the native runner takes")
         rc = torch.empty(hc * wc, dtype=torch.float32, device="cuda")
         mc = torch.empty(hc * wc, dtype=torch.uint8, device="cuda")
         _lib.call("nx_restrict_residual", sim._alive(), f, C.c_void_p(rc.data_ptr()),

@@ -374,23 +374,25 @@ def test_set_guess_routes_update_through_the_learned_step(gpu):
     b.destroy()
 
 
-def test_learned_step_on_a_window_shaped_grid(gpu):
+@pytest.mark.parametrize("W,H,grid", [(640, 360, (5, 128, 72)), (1000, 700, (8, 128, 88))])
+def test_learned_step_on_a_window_shaped_grid(gpu, W, H, grid):
     """640 x 360 (the demo's velocity grid for a 1280 x 720 window): the
-    coarse grid is 128 x 72 (f = 5, a factor that is not a power of two);
-    the learned step reaches the tolerance with a third of plain Jacobi's
-    sweeps or fewer."""
+    coarse grid is 128 x 72 (f = 5, a factor that is not a power of two).
+    1000 x 700: 125 x 88 coarse cells, padded to the network's 128 x 88
+    grid.  The learned step reaches the tolerance with a third of plain
+    Jacobi's sweeps or fewer."""
     from natrix_amd import FluidSimulator
     from natrix_amd.learned.solver import LearnedGuess, solve_step
     from natrix_amd.learned.train import load
     net, meta = load()
     g = LearnedGuess(net, meta.get("size", 128))
-    assert g.coarse_grid(640, 360) == (5, 128, 72)
+    assert g.coarse_grid(W, H) == grid
     res = {}
     for name, guess in (("learned", g), ("plain", None)):
-        sim = FluidSimulator(640, 360, bc_mode="inflow")
+        sim = FluidSimulator(W, H, bc_mode="inflow")
         sim.vorticity, sim.viscosity = 0.3, 0.0
-        X = np.arange(640)[None, :]
-        Y = np.arange(360)[:, None]
+        X = np.arange(W)[None, :]
+        Y = np.arange(H)[:, None]
         sim.set_obstacle_map((X - 200) ** 2 + (Y - 170) ** 2 <= 36 ** 2)
         for _ in range(5):
             sim.update(1 / 60)
@@ -400,5 +402,75 @@ def test_learned_step_on_a_window_shaped_grid(gpu):
         sim.destroy()
     print(res)
     assert res["learned"] * 3 <= res["plain"]
+
+
+def test_padded_coarse_grid_correction_matches_the_pytorch_statement(gpu):
+    """A 300 x 200 grid restricts onto 100 x 67 coarse cells, which the
+    native runner pads to the network's 104 x 72 grid with solid cells of
+    zero right-hand side.  The same correction stated in PyTorch from the
+    public entry points (restriction onto the 100 x 67 grid, padding, the
+    network, coarse smoothing on the padded grid, centring, interpolation
+    and line search from the 100 x 67 part) gives the same step to within
+    the float32 rounding of the input normalisation (bf16 input).  The
+    PyTorch path itself refuses a padded grid."""
+    from natrix_amd.learned.solver import LearnedGuess, pack_input
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    W, H = 300, 200
+    g = LearnedGuess(net, meta.get("size", 128))
+    f, wc, hc = g.coarse_grid(W, H)
+    assert (f, wc, hc) == (3, 104, 72)
+    wv, hv = -(-W // f), -(-H // f)
+
+    def scene():
+        sim = FluidSimulator(W, H, bc_mode="inflow")
+        sim.vorticity, sim.viscosity = 0.3, 0.0
+        X = np.arange(W)[None, :]
+        Y = np.arange(H)[:, None]
+        sim.set_obstacle_map((X - 90) ** 2 + (Y - 100) ** 2 <= 20 ** 2)
+        for _ in range(4):
+            sim.update(1 / 60)
+        sim._push_params()
+        _lib.call("nx_step_begin", sim._alive(), 1 / 60)
+        return sim, sim.download("pressure")
+
+    sim, p0 = scene()
+    a_n = g.correct(sim, report=True)
+    d_n = sim.download("pressure").astype(np.float64) - p0
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+    sim, p0 = scene()
+    with torch.no_grad(), torch.cuda.stream(torch.cuda.ExternalStream(sim.stream_handle())):
+        rv = torch.empty(hv * wv, device="cuda")
+        mv = torch.empty(hv * wv, dtype=torch.uint8, device="cuda")
+        _lib.call("nx_restrict_residual", sim._alive(), f, C.c_void_p(rv.data_ptr()),
+                  C.c_void_p(mv.data_ptr()))
+        rc = torch.zeros(hc, wc, device="cuda")
+        mc = torch.ones(hc, wc, dtype=torch.uint8, device="cuda")
+        rc[:hv, :wv] = rv.view(hv, wv)
+        mc[:hv, :wv] = mv.view(hv, wv)
+        rc, mc = rc.view(-1), mc.view(-1)
+        fluid = mc == 0
+        n = fluid.sum()
+        s = torch.sqrt((rc * rc * fluid).sum() / n)
+        x = pack_input((rc / s).view(1, hc, wc), (~fluid).view(1, hc, wc))
+        e = (net(x).view(-1) * s).contiguous()
+        _lib.call("nx_coarse_smooth", sim._alive(), wc, hc, C.c_void_p(e.data_ptr()),
+                  C.c_void_p(rc.data_ptr()), C.c_void_p(mc.data_ptr()),
+                  int(g.coarse_iters), float(g.lmin))
+        e = (e - (e * fluid).sum() / n) * fluid
+        ev = e.view(hc, wc)[:hv, :wv].contiguous()
+        mcv = mc.view(hc, wc)[:hv, :wv].contiguous()
+        out = (C.c_double * 3)()
+        _lib.call("nx_prolong_line_search", sim._alive(), f, C.c_void_p(ev.data_ptr()),
+                  C.c_void_p(mcv.data_ptr()), out)
+        a_s = tuple(out)
+    d_s = sim.download("pressure").astype(np.float64) - p0
     with pytest.raises(ValueError):
-        g.coarse_grid(1000, 700)
+        LearnedGuess(net, meta.get("size", 128), native=False).correct(sim)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+    assert a_s[0] > 0
+    assert np.linalg.norm(d_n - d_s) <= 1e-2 * np.linalg.norm(d_s)
+    assert abs(a_n[0] - a_s[0]) <= 1e-2 * abs(a_s[0])
