@@ -8,11 +8,14 @@ fixed-iteration steps (80 Jacobi sweeps each).  Inputs are resident in HBM.
 
 Default workload (`--workload learned`): steps whose pressure is solved to
 FP64 relative residual <= 1e-4.  `value` is the steps/s of the learned
-two-grid cycle + Jacobi; the same criterion with plain Jacobi correction
-solves is timed on one step (capped; its rate is then an upper bound).
-The JSON line also carries the roofline of the Jacobi kernel (HIP events on
-the simulator's stream, fixed-sweep steps), the plain fused step at fixed
-iterations, and the CPU oracle timed on a bounded sample.
+two-grid cycle (U-Net coarse guess, Chebyshev fine smoothing); the same
+criterion with plain Jacobi correction solves is timed on one step
+(capped; its rate is then an upper bound).  The JSON line also carries the
+roofline of the learned step's dominant kernel, the fine Chebyshev
+smoother (`roofline`), and of the plain path's Jacobi kernel
+(`roofline_plain_jacobi`, fixed-sweep steps), both from HIP events on the
+simulator's stream, the plain fused step at fixed iterations, and the CPU
+oracle timed on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--size N] [--tol 1e-4]
                     [--workload learned|single|batch|mixed] [--no-plain] [--no-cpu]
@@ -34,6 +37,9 @@ METRIC = ("sim steps/sec at 2048² @ rel. residual ≤1e-4 "
           "(learned+Jacobi vs plain); % HBM peak")
 HBM_PEAK_GBS = 8000.0
 JACOBI_BYTES_PER_CELL_SWEEP = 13   # read p 4 + b 4 + mask 1, write p 4
+CHEB_BYTES_PER_CELL_ITER = 25      # read x 4 + r 4 + d 4 + mask 1, write x, r, d 12
+CHEB_SMOOTH = 80                   # fine Chebyshev iterations per learned cycle
+CHEB_TILE = (8, 4, 8, 2)           # nx_smooth_chebyshev's default tile
 STEP_BYTES_PER_CELL = 104          # the other stages, FP32, dye on, nu = 0
 
 
@@ -41,11 +47,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def committed_traffic(N, iters, tile):
-    """HBM bytes per k_jacobi_tb launch from the committed rocprofv3 summary
-    of this grid, sweep count and tile (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
-    the copy-probe calibration), or None when no summary matches."""
-    path = os.path.join(ROOT, "profiles", f"k_jacobi_tb_{N}_fixed{iters}.md")
+def committed_traffic(N, iters, tile, kernel="k_jacobi_tb", suffix="fixed"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    of this grid, iteration count and tile (FETCH_SIZE x 2.0 + WRITE_SIZE x
+    1.0, the copy-probe calibration), or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{N}_{suffix}{iters}.md")
     if not os.path.exists(path):
         return None, None
     kv = {}
@@ -380,15 +386,15 @@ def bench_learned(args, copy_bandwidth):
     log(f"bench learned: {N}x{N}, warmup {args.warmup} fixed steps (I={args.iters})")
     sim = fresh()
     for _ in range(2):  # first-call costs (allocator, code objects) off the clock
-        solve_step(sim, guess, dt, tol=args.tol)
+        solve_step(sim, guess, dt, tol=args.tol, smooth=CHEB_SMOOTH)
     sim.synchronize()
     torch.cuda.synchronize()
     # no per-launch events in this timed region (they cost ~10 % of a
-    # learned step); the Jacobi roofline is taken in the fixed-sweep phase
+    # learned step); the kernels' rooflines are taken in phases of their own
     sweeps, outer, conv = [], [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        st = solve_step(sim, guess, dt, tol=args.tol)
+        st = solve_step(sim, guess, dt, tol=args.tol, smooth=CHEB_SMOOTH)
         sweeps.append(st["sweeps"])
         outer.append(st["outer"])
         conv.append(st["converged"] and st["residual"] <= args.tol)
@@ -416,6 +422,38 @@ def bench_learned(args, copy_bandwidth):
                  "residual_reached": st["residual"], "sweep_cap_per_correction_solve": args.plain_cap,
                  "rate_is_upper_bound": not done}
         log(f"bench plain: {plain}")
+
+    # the learned step's dominant kernel: the fine Chebyshev smoother, timed
+    # per launch with HIP events on the simulator's stream over calls of the
+    # step's own length (iterations per call, default tile) on this scene
+    import ctypes as C
+    from natrix_amd import _lib
+    sim = fresh()
+    sim._push_params()
+    _lib.call("nx_step_begin", sim._alive(), dt)
+    lmin = guess.fine_lmin(N, N)
+    n_iter, calls = CHEB_SMOOTH, 20
+    _lib.call("nx_smooth_chebyshev", sim._alive(), n_iter, lmin, None, None)
+    sim.set_timing(True)
+    sim.collect_timing()
+    for _ in range(calls):
+        _lib.call("nx_smooth_chebyshev", sim._alive(), n_iter, lmin, None, None)
+    tc = sim.collect_timing()
+    sim.set_timing(False)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+    ch_ms, ch_n = tc["chebyshev"]
+    ch_iters = n_iter * calls
+    ach_c = CHEB_BYTES_PER_CELL_ITER * cells * ch_iters / (ch_ms * 1e-3) / 1e9
+    traffic_c, tsrc_c = committed_traffic(N, n_iter, CHEB_TILE, "k_cheb_tb", "iters")
+    roof_cheb = {"bound": "hbm", "achieved": round(ach_c, 1), "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": round(ach_c / HBM_PEAK_GBS, 4), "traffic": traffic_c,
+                 "traffic_source": tsrc_c, "kernel": "k_cheb_tb",
+                 "avg_launch_us": round(1e3 * ch_ms / ch_n, 3),
+                 "iterations_per_launch": round(ch_iters / ch_n, 3),
+                 "algorithmic_bytes_per_launch": int(CHEB_BYTES_PER_CELL_ITER * cells * ch_iters / ch_n),
+                 "lambda_min": lmin, "tile": list(CHEB_TILE)}
+    log(f"bench chebyshev: {roof_cheb}")
 
     # fixed-sweep plain fused step: exact sweeps per Jacobi launch
     sim = fresh()
@@ -478,13 +516,13 @@ def bench_learned(args, copy_bandwidth):
                                   f"({centre[0]:.3f},{centre[1]:.3f}), left inflow 1.0, vorticity 0.3, "
                                   f"pressure to FP64 rel. residual <= {args.tol:g}",
                       "grid": N, "solver": "learned two-grid cycle (U-Net bf16 MFMA guess, "
-                                           "Chebyshev coarse smoothing) + Jacobi, FP64 refinement",
+                                           "Chebyshev coarse and fine smoothing), FP64 refinement",
                       "weights": os.path.basename(args.weights or DEFAULT_WEIGHTS)},
            "learned": learned, "plain": plain,
            "speedup_learned_vs_plain": round(speedup, 1) if speedup else None,
            "speedup_is_lower_bound": bool(plain and plain["rate_is_upper_bound"]),
            "plain_fixed_iterations": fixed, "copy_probe_GBps": round(copy_gbs, 1),
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof_cheb, "roofline_plain_jacobi": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

@@ -178,6 +178,21 @@ int nx_coarse_smooth(nx_sim* sim, int Wc, int Hc, float* e, const float* rc,
  * synchronisation.  out (host, may be NULL) = {relative residual, the last
  * line search's step a}.  Counted in the step's sweeps. */
 int nx_smooth(nx_sim* sim, int sweeps, float omega, double* out);
+/* `iterations` Chebyshev iterations of the current pressure system on the
+ * interval [lambda_min, 8] of -L (0 < lambda_min < 8; the three-term
+ * recurrence of Saad, Iterative Methods for Sparse Linear Systems, Alg.
+ * 12.1, from the current pressure): every mode of -L above lambda_min is
+ * damped by 1 / T_n((8 + lambda_min) / (8 - lambda_min)) and none grows;
+ * then the FP64 relative residual, as nx_smooth (same out).  tile (may be
+ * NULL: {8, 4, 8, 2}) = {stencil passes per launch, rows per wave, waves
+ * per block, columns per lane}; shapes (2,16,2), (4,8,2), (4,16,2),
+ * (8,8,1), (8,16,1), at most 32 passes, 64 cols - 2 passes >= 8 and
+ * rows waves - 2 passes >= 8.  Replaces the Jacobi smoothing of the
+ * learned cycle; no reference counterpart (the reference smooths only by
+ * Jacobi, shader.Poisson.comp:24-37).  Counted in the step's sweeps (one
+ * per iteration). */
+int nx_smooth_chebyshev(nx_sim* sim, int iterations, float lambda_min, const int* tile,
+                        double* out);
 /* `count` damped Jacobi sweeps p <- p + omega (J(p) - p) of the current
  * pressure system (0 < omega <= 1; omega = 1/2 removes the checkerboard
  * modes the plain sweep cannot damp); counted in the step's sweeps */
@@ -247,10 +262,12 @@ int nx_get_size(const nx_sim* sim, int* width, int* height);
 #define NX_TIME_PROJECT 3
 #define NX_TIME_DYE 4
 #define NX_TIME_SPLAT 5
-#define NX_TIME_COUNT 6
+#define NX_TIME_CHEB 6 /* fine Chebyshev smoothing (nx_smooth_chebyshev) */
+#define NX_TIME_COUNT 7
 int nx_set_timing(nx_sim* sim, int on);
 /* synchronises; per tag: total ms and number of launches since the last
- * reset; resets the accumulators */
+ * reset (ms, launches: NX_TIME_COUNT entries each); resets the
+ * accumulators */
 int nx_collect_timing(nx_sim* sim, double* ms, int* launches);
 
 /* ---- batched scenes: many independent grids of uneven sizes advanced by

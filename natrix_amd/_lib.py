@@ -33,7 +33,7 @@ BC_REFERENCE = 0
 BC_FREE_SLIP = 1
 BC_INFLOW = 2
 
-TIME_TAGS = ("advect", "vort_div", "jacobi", "project", "dye", "splat")
+TIME_TAGS = ("advect", "vort_div", "jacobi", "project", "dye", "splat", "chebyshev")
 
 
 class Params(C.Structure):
@@ -143,6 +143,7 @@ SIGNATURES = {
     "nx_line_search": [_P, _P, _P],
     "nx_damped_sweeps": [_P, _I, _F],
     "nx_smooth": [_P, _I, _F, _P],
+    "nx_smooth_chebyshev": [_P, _I, _F, _P, _P],
     "nx_coarse_smooth": [_P, _I, _I, _P, _P, _P, _I, _F],
     "nx_refine_begin": [_P],
     "nx_refine_residual": [_P, C.POINTER(_D)],

@@ -166,8 +166,8 @@ class BatchedScenes:
         _lib.call("nx_batch_set_timing", self._alive(), 1 if on else 0)
 
     def collect_timing(self):
-        ms = (C.c_double * 6)()
-        n = (C.c_int * 6)()
+        ms = (C.c_double * len(_lib.TIME_TAGS))()
+        n = (C.c_int * len(_lib.TIME_TAGS))()
         _lib.call("nx_batch_collect_timing", self._alive(), ms, n)
         return {tag: (ms[i], n[i]) for i, tag in enumerate(_lib.TIME_TAGS)}
 

@@ -481,8 +481,8 @@ class FluidSimulator:
     def collect_timing(self):
         """Per kernel family: (total ms, launches) since the last collect,
         measured with HIP events on the simulator's stream."""
-        ms = (C.c_double * 6)()
-        n = (C.c_int * 6)()
+        ms = (C.c_double * len(_lib.TIME_TAGS))()
+        n = (C.c_int * len(_lib.TIME_TAGS))()
         _lib.call("nx_collect_timing", self._alive(), ms, n)
         return {tag: (ms[i], n[i]) for i, tag in enumerate(_lib.TIME_TAGS)}
 

@@ -87,6 +87,7 @@ struct nx_sim {
   float* dcorr = nullptr;   // prolonged coarse correction (line search)
   float* rrow = nullptr;    // restriction row records (2 x H x Wc)
   float* cws = nullptr;     // banded coarse smoother: x, r, d twice
+  float* fcheb = nullptr;   // fine Chebyshev smoother: r, d twice (4 W H)
   size_t cws_cap = 0;
   size_t rrow_cap = 0;
   double* line = nullptr;   // {alpha, d.r, d.Ld, 0}
@@ -1219,6 +1220,7 @@ int nx_destroy(nx_sim* s) {
   (void)hipFree(s->dcorr);
   (void)hipFree(s->rrow);
   (void)hipFree(s->cws);
+  (void)hipFree(s->fcheb);
   (void)hipFree(s->line);
   (void)hipFree(s->curl);
   (void)hipFree(s->red5);
@@ -1515,6 +1517,60 @@ int nx_smooth(nx_sim* s, int sweeps, float omega, double* out) {
   NX_TRY(run_jacobi_tb(s, sweeps, c));
   s->step_sweeps += sweeps;
   if (omega > 0.0f) NX_TRY(nx_damped_sweeps(s, 1, omega));
+  NX_HIP(launch_sums(s->p[s->pcur], s->div, s->mask[s->mcur], s->W, s->H, s->red5,
+                     s->stream));
+  NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,
+                       nullptr, 0, 0, s->stream));
+  double rel = 0.0, alpha = 0.0;
+  NX_HIP(hipMemcpyAsync(&rel, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  if (s->line)
+    NX_HIP(hipMemcpyAsync(&alpha, s->line, sizeof(double), hipMemcpyDeviceToHost,
+                          s->stream));
+  NX_HIP(hipStreamSynchronize(s->stream));
+  s->stats.sweeps = s->step_sweeps;
+  s->stats.residual = rel;
+  s->stats.residual_sweeps = s->step_sweeps;
+  if (out) {
+    out[0] = rel;
+    out[1] = alpha;
+  }
+  return NX_OK;
+}
+
+int nx_smooth_chebyshev(nx_sim* s, int iterations, float lambda_min, const int* tile,
+                        double* out) {
+  if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (!s->in_step) return fail(NX_ERR_STATE, "nx_smooth_chebyshev outside nx_step_begin/finish");
+  if (iterations < 1 || !(lambda_min > 0.0f && lambda_min < 8.0f))
+    return fail(NX_ERR_ARG, "iterations >= 1 and 0 < lambda_min < 8 required");
+  const JacobiCfg c = tile ? JacobiCfg{tile[0], tile[1], tile[2], tile[3]}
+                          : JacobiCfg{8, 4, 8, 2};
+  if (!cheb_cfg_supported(c)) return fail(NX_ERR_ARG, "unsupported Chebyshev tile");
+  if (!s->fcheb) NX_HIP(hipMalloc(&s->fcheb, (size_t)4 * s->W * s->H * sizeof(float)));
+  // x ping-pongs between the pressure buffers, (r, d) between the halves of
+  // fcheb; the first launch spends one pass on the setup
+  const size_t n = (size_t)s->W * s->H;
+  float* rd[2][2] = {{s->fcheb, s->fcheb + n}, {s->fcheb + 2 * n, s->fcheb + 3 * n}};
+  ChebRecurrence rec(lambda_min);
+  int done = 0, cur = 0;
+  while (done < iterations) {
+    const int setup = done == 0 ? 1 : 0;
+    JacobiCfg cc = c;
+    cc.k = std::min(c.k, iterations - done + setup);
+    const bool last = done + cc.k - setup >= iterations;
+    ChebCoef cf{};
+    rec.next(setup, cc.k, cf);
+    NX_LAUNCH_TIMED(s, NX_TIME_CHEB,
+                    launch_cheb_tb(s->stream, cc, s->p[s->pcur], rd[cur][0], rd[cur][1],
+                                   s->div, s->mask[s->mcur], s->p[1 - s->pcur],
+                                   last ? nullptr : rd[1 - cur][0], rd[1 - cur][1], s->W,
+                                   s->H, setup, rec.theta, cf));
+    s->pcur ^= 1;
+    cur ^= 1;
+    done += cc.k - setup;
+  }
+  s->step_sweeps += iterations;
   NX_HIP(launch_sums(s->p[s->pcur], s->div, s->mask[s->mcur], s->W, s->H, s->red5,
                      s->stream));
   NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,

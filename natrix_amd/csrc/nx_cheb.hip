@@ -1,0 +1,276 @@
+// Temporal-blocked Chebyshev smoothing of the fine pressure system.
+//
+// Chebyshev iteration for -L x = -b on the eigenvalue interval [lmin, 8] of
+// -L (three-term recurrence of Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., Algorithm 12.1), with L the 5-point operator of the
+// Jacobi sweep (shader.Poisson.comp:26-37: a solid or out-of-grid neighbour
+// reads the centre value):
+//   setup:      r = L x - b,  d = r / theta
+//   iteration:  x += d,  r += L d,  d = (rho' rho) d + (2 rho' / delta) r
+// with theta = (8 + lmin) / 2, delta = (8 - lmin) / 2, sigma = theta /
+// delta, rho_0 = 1 / sigma, rho' = 1 / (2 sigma - rho) (the scalar
+// recurrence runs on the host; a launch takes its passes' coefficients).  -L is symmetric
+// positive semi-definite with spectrum in [0, 8], so the error polynomial is
+// at most 1 in magnitude on all of [0, 8] and at most 1 / T_n(sigma) on
+// [lmin, 8]: n iterations damp every mode above lmin, which n Jacobi sweeps
+// (factor (1 - lambda / 4)^n) leave nearly untouched near lmin.
+//
+// Only d needs neighbours, so a launch carries (x, r, d) of a register tile
+// through k stencil passes exactly as the Jacobi tile carries p: lane l of
+// wave w owns COLS columns and ROWS rows, horizontal neighbours come by DPP
+// lane shifts, the rows next to a wave's strip through a double-buffered
+// LDS exchange, and the (64 COLS - 2k) x (ROWS WAVES - 2k) interior whose
+// k-pass dependency cone stayed inside the tile is written back.  The first
+// launch of a run spends its first pass on the setup.  Every pass evaluates
+// each cell's expressions in a fixed order without contraction, so the
+// result is bitwise independent of the tile shape and of k.
+#include "../../include/natrix.h"
+#include "nx_jacobi_tile.h"
+#include "nx_kernels.h"
+
+namespace nx {
+
+namespace {
+
+using jt::lane_from_left;
+using jt::lane_from_right;
+using jt::ilane_from_left;
+using jt::ilane_from_right;
+
+template <int ROWS, int WAVES, int COLS>
+__global__ void __launch_bounds__(WAVES * 64)
+    k_cheb_tb(const float* __restrict__ xin, const float* __restrict__ rin,
+              const float* __restrict__ din, const float* __restrict__ b,
+              const uint8_t* __restrict__ mask, float* __restrict__ xout,
+              float* __restrict__ rout, float* __restrict__ dout, int W, int H, int k,
+              int first, float theta, ChebCoef cf) {
+  constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int VX = EX - 2 * k, VY = EY - 2 * k;
+  const int gx0 = blockIdx.x * VX - k + COLS * lane;
+  const int gy0 = blockIdx.y * VY - k + wave * ROWS;
+  const int tcol0 = COLS * lane, trow0 = wave * ROWS;
+
+  // float2 accesses when every lane's column pair is aligned and wholly
+  // inside or outside the grid and the written interior
+  const bool vec = COLS == 2 && (W % 2 == 0) && (k % 2 == 0);
+  float x[ROWS][COLS], r[ROWS][COLS], d[ROWS][COLS];
+  int sol[ROWS][COLS];
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    const int gy = gy0 + i;
+    if (vec) {
+      const bool in = gy >= 0 && gy < H && gx0 >= 0 && gx0 < W;
+      const int idx = in ? gy * W + gx0 : 0;
+      const float2 z = make_float2(0.0f, 0.0f);
+      const float2 xv = in ? *reinterpret_cast<const float2*>(xin + idx) : z;
+      // the first launch holds b in r until the setup pass
+      const float2 rv = in ? *reinterpret_cast<const float2*>((first ? b : rin) + idx) : z;
+      const float2 dv = (in && !first) ? *reinterpret_cast<const float2*>(din + idx) : z;
+      const uint16_t mm = in ? *reinterpret_cast<const uint16_t*>(mask + idx) : 0x0101;
+      x[i][0] = xv.x;
+      x[i][COLS - 1] = xv.y;
+      r[i][0] = rv.x;
+      r[i][COLS - 1] = rv.y;
+      d[i][0] = dv.x;
+      d[i][COLS - 1] = dv.y;
+      sol[i][0] = (mm & 0xFF) != 0;
+      sol[i][COLS - 1] = (mm >> 8) != 0;
+      continue;
+    }
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      const int gx = gx0 + c;
+      const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+      const int idx = in ? gy * W + gx : 0;
+      x[i][c] = in ? xin[idx] : 0.0f;
+      r[i][c] = in ? (first ? b[idx] : rin[idx]) : 0.0f;
+      d[i][c] = (in && !first) ? din[idx] : 0.0f;
+      sol[i][c] = in ? (mask[idx] != 0) : 1;
+    }
+  }
+
+  __shared__ float xch[2][2][WAVES][EX];
+  __shared__ int sxch[2][WAVES][EX];
+#pragma unroll
+  for (int c = 0; c < COLS; ++c) {
+    sxch[0][wave][tcol0 + c] = sol[0][c];
+    sxch[1][wave][tcol0 + c] = sol[ROWS - 1][c];
+  }
+  __syncthreads();
+  // bit0 L, bit1 R, bit2 B (row above in memory, y - 1), bit3 T (y + 1):
+  // that neighbour is solid or outside the grid and reads the centre
+  int cd[ROWS][COLS];
+  bool any_nbr = false;
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    const int sl = ilane_from_left(sol[i][COLS - 1]);
+    const int sr = ilane_from_right(sol[i][0]);
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      const int L = c > 0 ? sol[i][c - 1] : sl;
+      const int R = c < COLS - 1 ? sol[i][c + 1] : sr;
+      const int B = i > 0 ? sol[i - 1][c] : (wave > 0 ? sxch[1][wave - 1][tcol0 + c] : 1);
+      const int T = i < ROWS - 1 ? sol[i + 1][c]
+                                 : (wave < WAVES - 1 ? sxch[0][wave + 1][tcol0 + c] : 1);
+      cd[i][c] = L | (R << 1) | (B << 2) | (T << 3);
+      any_nbr |= cd[i][c] != 0;
+    }
+  }
+  const bool special = __any(any_nbr);
+
+  // out = L v = ((((x1 + x2) + y1) + y2) - 4 v) over the tile (garbage on
+  // the tile's edge cells, which lie outside the valid cone)
+  auto lap = [&](const float (&v)[ROWS][COLS], float (&out)[ROWS][COLS], int buf) {
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      xch[buf][0][wave][tcol0 + c] = v[0][c];
+      xch[buf][1][wave][tcol0 + c] = v[ROWS - 1][c];
+    }
+    __syncthreads();
+    float prev[COLS], above[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      prev[c] = wave > 0 ? xch[buf][1][wave - 1][tcol0 + c] : v[0][c];
+      above[c] = wave < WAVES - 1 ? xch[buf][0][wave + 1][tcol0 + c] : v[ROWS - 1][c];
+    }
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i) {
+      const float lft = lane_from_left(v[i][COLS - 1]);
+      const float rgt = lane_from_right(v[i][0]);
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const float vc = v[i][c];
+        float x1 = c > 0 ? v[i][c - 1] : lft;
+        float x2 = c < COLS - 1 ? v[i][c + 1] : rgt;
+        float y1 = i > 0 ? v[i - 1][c] : prev[c];
+        float y2 = i < ROWS - 1 ? v[i + 1][c] : above[c];
+        if (special) {
+          const int code = cd[i][c];
+          x1 = (code & 1) ? vc : x1;
+          x2 = (code & 2) ? vc : x2;
+          y1 = (code & 4) ? vc : y1;
+          y2 = (code & 8) ? vc : y2;
+        }
+        out[i][c] = (((x1 + x2) + y1) + y2) - 4.0f * vc;
+      }
+    }
+  };
+
+  float lv[ROWS][COLS];
+  int s = 0;
+  if (first) {
+    lap(x, lv, 0);
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        r[i][c] = lv[i][c] - r[i][c];
+        d[i][c] = r[i][c] / theta;
+      }
+    s = 1;
+  }
+  for (; s < k; ++s) {
+    lap(d, lv, s & 1);
+    const float cdd = cf.cd[s], crr = cf.cr[s];
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        x[i][c] = x[i][c] + d[i][c];
+        r[i][c] = r[i][c] + lv[i][c];
+        d[i][c] = cdd * d[i][c] + crr * r[i][c];
+      }
+  }
+
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    const int trow = trow0 + i, gy = gy0 + i;
+    if (trow < k || trow >= EY - k || gy < 0 || gy >= H) continue;
+    if (vec) {
+      if (tcol0 < k || tcol0 >= EX - k || gx0 < 0 || gx0 >= W) continue;
+      const int idx = gy * W + gx0;
+      *reinterpret_cast<float2*>(xout + idx) = make_float2(x[i][0], x[i][COLS - 1]);
+      if (rout) {
+        *reinterpret_cast<float2*>(rout + idx) = make_float2(r[i][0], r[i][COLS - 1]);
+        *reinterpret_cast<float2*>(dout + idx) = make_float2(d[i][0], d[i][COLS - 1]);
+      }
+      continue;
+    }
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+      const int tcol = tcol0 + c, gx = gx0 + c;
+      if (tcol < k || tcol >= EX - k || gx < 0 || gx >= W) continue;
+      const int idx = gy * W + gx;
+      xout[idx] = x[i][c];
+      if (rout) {
+        rout[idx] = r[i][c];
+        dout[idx] = d[i][c];
+      }
+    }
+  }
+}
+
+template <int ROWS, int WAVES, int COLS>
+hipError_t launch_cheb(const float* xin, const float* rin, const float* din, const float* b,
+                       const uint8_t* mask, float* xout, float* rout, float* dout, int W,
+                       int H, int k, int first, float theta, const ChebCoef& cf,
+                       hipStream_t st) {
+  constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
+  const dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + EY - 2 * k - 1) / (EY - 2 * k));
+  k_cheb_tb<ROWS, WAVES, COLS><<<grid, WAVES * 64, 0, st>>>(
+      xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Tile shapes (rows per wave, waves per block, columns per lane).
+#define NX_CHEB_SHAPES(X) \
+  X(2, 16, 2)             \
+  X(4, 8, 2)              \
+  X(4, 16, 2)             \
+  X(8, 8, 1)              \
+  X(8, 16, 1)
+
+bool cheb_cfg_supported(JacobiCfg c) {
+  bool shape = false;
+#define NX_CHEB_CHECK(R, WV, CL) shape |= (c.rows == R && c.waves == WV && c.cols == CL);
+  NX_CHEB_SHAPES(NX_CHEB_CHECK)
+#undef NX_CHEB_CHECK
+  return shape && c.k >= 1 && c.k <= CHEB_MAX_PASSES && 64 * c.cols - 2 * c.k >= 8 &&
+         c.rows * c.waves - 2 * c.k >= 8;
+}
+
+hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const float* rin,
+                          const float* din, const float* b, const uint8_t* mask, float* xout,
+                          float* rout, float* dout, int W, int H, int setup, float theta,
+                          const ChebCoef& cf) {
+  if (!cheb_cfg_supported(c)) return hipErrorInvalidValue;
+#define NX_CHEB_LAUNCH(R, WV, CL)                                                     \
+  if (c.rows == R && c.waves == WV && c.cols == CL)                                  \
+    return launch_cheb<R, WV, CL>(xin, rin, din, b, mask, xout, rout, dout, W, H, c.k, \
+                                  setup, theta, cf, st);
+  NX_CHEB_SHAPES(NX_CHEB_LAUNCH)
+#undef NX_CHEB_LAUNCH
+  return hipErrorInvalidValue;
+}
+
+ChebRecurrence::ChebRecurrence(float lambda_min) {
+  theta = 0.5f * (8.0f + lambda_min);
+  delta = 0.5f * (8.0f - lambda_min);
+  sigma = theta / delta;
+  rho = 1.0f / sigma;
+}
+
+void ChebRecurrence::next(int first, int passes, ChebCoef& cf) {
+  for (int i = first; i < passes; ++i) {
+    const float rho1 = 1.0f / (2.0f * sigma - rho);
+    cf.cd[i] = rho1 * rho;
+    cf.cr[i] = 2.0f * rho1 / delta;
+    rho = rho1;
+  }
+}
+
+}  // namespace nx

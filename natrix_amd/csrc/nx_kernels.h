@@ -81,22 +81,48 @@ hipError_t launch_jacobi_tb(const float* pin, const float* b,
                             hipStream_t s);
 bool jacobi_cfg_supported(JacobiCfg cfg);
 
-// Exact FP64 residual sums of `p` (of a zero guess when p == nullptr) per
-// 64 x RES_ROWS block of cells (256 threads, RES_ROWS / 4 rows each):
-// red[5*blk + {sum r, sum r^2, sum b, sum b^2, fluid count}].
-constexpr int RES_ROWS = 32;
 // Restriction of the residual onto, and interpolation with the energy line
 // search from, a Wp x Hp coarse grid padded beyond ceil(W / f) x ceil(H / f)
 // (padded coarse cells are solid, with a zero right-hand side).
 int restrict_residual_padded(::nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc);
 int prolong_line_search_padded(::nx_sim* s, int factor, int Wp, int Hp, const float* e,
                                const uint8_t* mc, double* out);
+// Chebyshev smoothing of the fine system L x = b (mask != 0: solid) on the
+// interval [lambda_min, 8] of -L.  One launch runs c.k stencil passes on
+// a (64 c.cols) x (c.rows c.waves) register tile per block: with setup != 0
+// the first pass computes r = L x - b, d = r / theta from x and b (rin, din
+// unread), every other pass s is an iteration x += d, r += L d,
+// d = cf.cd[s] d + cf.cr[s] r.  rout == nullptr: only x is written.
+constexpr int CHEB_MAX_PASSES = 32;
+struct ChebCoef {
+  float cd[CHEB_MAX_PASSES];
+  float cr[CHEB_MAX_PASSES];
+};
+// the scalar recurrence (float32, on the host): theta, delta, sigma of the
+// interval and rho of the next iteration; next() fills passes [first,
+// passes) of a launch's coefficients and advances rho
+struct ChebRecurrence {
+  float theta, delta, sigma, rho;
+  explicit ChebRecurrence(float lambda_min);
+  void next(int first, int passes, ChebCoef& cf);
+};
+bool cheb_cfg_supported(JacobiCfg c);
+hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const float* rin,
+                          const float* din, const float* b, const uint8_t* mask, float* xout,
+                          float* rout, float* dout, int W, int H, int setup, float theta,
+                          const ChebCoef& cf);
+
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
 // sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 8
 // Chebyshev iterations run banded over several workgroups and need
 // ws = 6 Wc Hc floats.
 int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* rc,
                          const uint8_t* mc, int iterations, float lambda_min, float* ws);
+
+// Exact FP64 residual sums of `p` (of a zero guess when p == nullptr) per
+// 64 x RES_ROWS block of cells (256 threads, RES_ROWS / 4 rows each):
+// red[5*blk + {sum r, sum r^2, sum b, sum b^2, fluid count}].
+constexpr int RES_ROWS = 32;
 int residual_blocks(int W, int H);
 hipError_t launch_sums(const float* p, const float* b, const uint8_t* mask,
                        int W, int H, double* red, hipStream_t s);

@@ -113,6 +113,13 @@ class LearnedGuess:
         f = max(1, -(-max(W, H) // c))
         return f, -(-(-(-W // f)) // 8) * 8, -(-(-(-H // f)) // 8) * 8
 
+    def fine_lmin(self, W, H):
+        """Lower end of the fine Chebyshev smoother's interval, 1.28 / f^2
+        (0.005 at f = 16): the modes of -L below it (wavelengths beyond
+        about 4 f cells) are left to the coarse correction."""
+        f = self.coarse_grid(W, H)[0]
+        return 1.28 / (f * f)
+
     def _runner(self, wc, hc):
         if not self._native:
             return None
@@ -173,12 +180,20 @@ def _solve(sim, tol, max_sweeps):
     return {"sweeps": st.sweeps, "converged": bool(st.converged), "residual": st.residual}
 
 
-def _smooth(sim, target, sweeps, early_exit=False):
-    """`sweeps` fine sweeps of the current system ending in a damped one
-    (omega = 1/2, which removes the near-checkerboard modes the plain sweep
-    keeps) and the residual, in one synchronisation; with `early_exit` the
-    plain sweeps run as a tolerance solve that stops at relative residual
-    <= target.  Also returns the last line search's step."""
+def _smooth(sim, target, sweeps, early_exit=False, lmin=0.0, tile=None):
+    """`sweeps` fine smoothing iterations of the current system and the
+    residual, in one synchronisation: Chebyshev iterations on [lmin, 8]
+    (lmin > 0), or Jacobi sweeps ending in a damped one (omega = 1/2, which
+    removes the near-checkerboard modes the plain sweep keeps); with
+    `early_exit` plain Jacobi sweeps run as a tolerance solve that stops at
+    relative residual <= target.  Also returns the last line search's
+    step."""
+    out = (C.c_double * 2)()
+    if lmin > 0 and not early_exit:
+        t = None if tile is None else (C.c_int * 4)(*(int(v) for v in tile))
+        _lib.call("nx_smooth_chebyshev", sim._alive(), int(sweeps), float(lmin), t, out)
+        return {"sweeps": sim.stats()["sweeps"], "converged": out[0] <= target,
+                "residual": out[0], "alpha": out[1]}
     n = max(0, sweeps - 1)
     if early_exit:
         st = _solve(sim, target, max(1, n))
@@ -186,14 +201,13 @@ def _smooth(sim, target, sweeps, early_exit=False):
             st["alpha"] = None
             return st
         n = 0
-    out = (C.c_double * 2)()
     _lib.call("nx_smooth", sim._alive(), n, 0.5, out)
     return {"sweeps": sim.stats()["sweeps"], "converged": out[0] <= target,
             "residual": out[0], "alpha": out[1]}
 
 
 def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
-           plain_stall=0.9, trace=None):
+           plain_stall=0.9, trace=None, lmin=0.0, tile=None):
     """Solve the correction equation (current right-hand side, iterate
     from its first sweep) to relative residual `target`: learned cycles
     while each reduces the residual below `stall` x the previous one, then
@@ -206,7 +220,7 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
     start = sim.stats()["sweeps"]
     for _ in range(max_cycles if guess is not None else 0):
         guess.correct(sim)
-        st = _smooth(sim, target, smooth)
+        st = _smooth(sim, target, smooth, lmin=lmin, tile=tile)
         a = st["alpha"]
         if trace is not None:
             trace.append(("cycle", round(a, 4), st["sweeps"], st["residual"]))
@@ -234,18 +248,27 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
     return kept, abandoned
 
 
-def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=96, max_cycles=6,
-               max_outer=12, chunk=65536, max_inner_sweeps=4_000_000, trace=None):
+def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=80, max_cycles=6,
+               max_outer=12, chunk=65536, max_inner_sweeps=4_000_000, trace=None,
+               fine_lmin=None, fine_tile=None):
     """One fluid step whose pressure is solved to FP64 relative residual
     <= tol by iterative refinement (pressure held as a float32 pair, outer
     residual in FP64, float32 correction solves); with `guess` (a
     LearnedGuess) each correction solve starts with learned cycles, without
-    it the correction solves are plain Jacobi from zero.  Returns the
-    step's statistics: sweeps (fine Jacobi sweeps), converged, residual
+    it the correction solves are plain Jacobi from zero.  A learned cycle
+    is the correction followed by `smooth` fine smoothing iterations:
+    Chebyshev on [fine_lmin, 8] (None: the guess's 1.28 / f^2; `fine_tile`
+    the kernel's tile, None for the library's default) or, with
+    fine_lmin = 0, Jacobi sweeps ending in a damped one.  Returns the
+    step's statistics: sweeps (fine sweeps and Chebyshev iterations), converged, residual
     (FP64 relative), outer (refinement iterations), learned_corrections
     (cycles that moved the pressure), fallback (learned cycles abandoned
     for plain Jacobi in some correction solve).  `trace` (a list) receives
     one record per outer iteration, learned cycle and plain chunk."""
+    if fine_lmin is None:
+        lmin = guess.fine_lmin(sim.width, sim.height) if guess is not None else 0.0
+    else:
+        lmin = float(fine_lmin)
     sim._push_params()
     _lib.call("nx_step_begin", sim._alive(), float(dt))
     _lib.call("nx_refine_begin", sim._alive())
@@ -267,7 +290,7 @@ def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=96, max_cycles=6,
                 break
             target = max(inner_tol, 0.5 * tol / rel.value)
             k, ab = _inner(sim, guess, target, smooth, max_cycles, chunk,
-                           max_inner_sweeps, trace=trace)
+                           max_inner_sweeps, trace=trace, lmin=lmin, tile=fine_tile)
             kept += k
             fallback |= ab
             _lib.call("nx_refine_accumulate", sim._alive())

@@ -31,6 +31,17 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.SIGNATURES) == names
 
 
+def test_timing_tags_match_c_header():
+    """The timing arrays the Python side passes to nx_collect_timing hold
+    NX_TIME_COUNT entries, one per tag, in the header's order."""
+    txt = open(HEADER).read()
+    tags = dict((m[0], int(m[1])) for m in re.findall(r"^#define (NX_TIME_\w+) (\d+)", txt, re.M))
+    assert tags.pop("NX_TIME_COUNT") == len(_lib.TIME_TAGS)
+    assert sorted(tags.values()) == list(range(len(_lib.TIME_TAGS)))
+    assert tags["NX_TIME_JACOBI"] == _lib.TIME_TAGS.index("jacobi")
+    assert tags["NX_TIME_CHEB"] == _lib.TIME_TAGS.index("chebyshev")
+
+
 def test_struct_layout_matches_c_header():
     src = r'''
 #include <stdio.h>

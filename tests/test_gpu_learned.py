@@ -22,9 +22,11 @@ from smoothers import (cheb32, cheb_ref, jacobi32, jacobi_ref, lap, prolong_ref,
 pytestmark = pytest.mark.gpu
 
 
-def in_step_sim(rng, H, W):
+def in_step_sim(rng, H, W, tile=None):
     sim = FluidSimulator(W, H)
     sim.viscosity = 0.0
+    if tile is not None:
+        sim.jacobi_tile = tile
     Y, X = np.mgrid[0:H, 0:W]
     sim.set_obstacle_map((X - 0.4 * W) ** 2 + (Y - 0.5 * H) ** 2 <= (0.15 * H) ** 2)
     sim._push_params()
@@ -202,6 +204,83 @@ def test_damped_sweep_matches_float32_statement(gpu, rng):
     assert np.array_equal(got, q)
     with pytest.raises(ValueError):
         _lib.call("nx_damped_sweeps", sim._alive(), 1, 0.0)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+def _tile(t):
+    return None if t is None else (C.c_int * 4)(*t)
+
+
+@pytest.mark.parametrize("H,W", [(72, 100), (131, 257)])
+@pytest.mark.parametrize("tile,iters", [((8, 2, 16, 2), 1), ((8, 2, 16, 2), 7),
+                                        ((8, 2, 16, 2), 8), ((8, 2, 16, 2), 40),
+                                        ((5, 4, 8, 2), 40), ((3, 8, 8, 1), 17),
+                                        ((12, 2, 16, 2), 30), (None, 40),
+                                        ((10, 8, 16, 1), 25), ((6, 4, 16, 2), 33)])
+def test_fine_chebyshev_matches_float32_statement(gpu, rng, H, W, tile, iters):
+    """nx_smooth_chebyshev equals the float32 statement of the Chebyshev
+    iteration (cheb32: the kernels' operation order, no contraction)
+    bitwise, for every tile shape, odd and even passes per launch (float2
+    and scalar accesses) and the default tile (None); the reported residual
+    is the FP64 relative residual of the result."""
+    lmin = 0.01
+    sim = in_step_sim(rng, H, W)
+    solid = sim.download("mask") != 0
+    b = rng.normal(size=(H, W)).astype(np.float32)
+    p = rng.normal(size=(H, W)).astype(np.float32)
+    dev(sim, "divergence").copy_(torch.from_numpy(b))
+    dev(sim, "pressure").copy_(torch.from_numpy(p))
+    torch.cuda.synchronize()
+    out = (C.c_double * 2)()
+    _lib.call("nx_smooth_chebyshev", sim._alive(), iters, lmin, _tile(tile), out)
+    got = sim.download("pressure")
+    want = cheb32(p, b, solid, iters, lmin)
+    assert np.array_equal(got, want)
+    r = (b.astype(np.float64) - lap(got.astype(np.float64), solid))[~solid]
+    bf = b.astype(np.float64)[~solid]
+    n = r.size
+    rel = np.sqrt((np.sum(r * r) - np.sum(r) ** 2 / n) / (np.sum(bf * bf) - np.sum(bf) ** 2 / n))
+    assert abs(out[0] - rel) <= 1e-9 * rel
+    for bad in ((0, lmin, None), (4, 0.0, None), (4, 8.0, None), (4, float("nan"), None),
+                (4, lmin, _tile((4, 2, 8, 2))), (4, lmin, _tile((13, 2, 16, 2))),
+                (4, lmin, _tile((0, 4, 8, 2)))):
+        with pytest.raises(ValueError):
+            _lib.call("nx_smooth_chebyshev", sim._alive(), *bad, None)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+def test_fine_chebyshev_damps_the_interval(gpu):
+    """On a grid without obstacles the eigenvectors of -L are cosine modes
+    with eigenvalue 4 sin^2(pi i / 2W) + 4 sin^2(pi j / 2H); n iterations
+    scale a mode inside [lmin, 8] by at most 1 / T_n(sigma) and one below
+    lmin by at most 1 (closed forms of the Chebyshev polynomial)."""
+    H, W = 64, 96
+    lmin, n = 0.05, 24
+    sim = FluidSimulator(W, H)
+    sim._push_params()
+    _lib.call("nx_step_begin", sim._alive(), 1 / 60)
+    sim.synchronize()
+    Y, X = np.mgrid[0:H, 0:W]
+    theta, delta = 0.5 * (8 + lmin), 0.5 * (8 - lmin)
+    T = np.cosh(n * np.arccosh(theta / delta))
+    for i, j in ((1, 0), (3, 2), (20, 10), (47, 31), (95, 63)):
+        mode = np.cos(np.pi * i * (X + 0.5) / W) * np.cos(np.pi * j * (Y + 0.5) / H)
+        lam = 4 * np.sin(np.pi * i / (2 * W)) ** 2 + 4 * np.sin(np.pi * j / (2 * H)) ** 2
+        dev(sim, "divergence").zero_()
+        dev(sim, "pressure").copy_(torch.from_numpy(mode.astype(np.float32)))
+        torch.cuda.synchronize()
+        _lib.call("nx_smooth_chebyshev", sim._alive(), n, lmin, None, None)
+        got = sim.download("pressure").astype(np.float64)
+        gain = np.sum(got * mode) / np.sum(mode * mode)
+        # the closed form of the mode's factor: T_n((8 + lmin - 2 lam) / (8 - lmin)) / T_n(sigma)
+        z = (8 + lmin - 2 * lam) / (8 - lmin)
+        tz = np.cos(n * np.arccos(z)) if abs(z) <= 1 else np.sign(z) ** n * np.cosh(n * np.arccosh(abs(z)))
+        assert abs(gain - tz / T) <= 1e-4 + 1e-3 * abs(tz / T), (i, j, gain, tz / T)
+        if lam >= lmin:
+            assert abs(gain) <= 1 / T + 1e-4
+        assert abs(gain) <= 1 + 1e-4
     _lib.call("nx_step_finish", sim._alive())
     sim.destroy()
 
