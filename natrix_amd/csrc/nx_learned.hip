@@ -75,7 +75,7 @@ int conv_splits(long long m, int cin, int cout) {
   if (cin % 16) return 1;
   const long long base = ((m + 255) / 256) * (cout / (cout % 64 == 0 ? 64 : 32));
   if (base >= 256) return 1;
-  const long long a = (9 * cin / 16) / 4, b = (512 + base - 1) / base;
+  const long long a = (9 * cin / 16) / 8, b = (256 + base - 1) / base;
   const long long s = a < b ? a : b;
   return s < 1 ? 1 : (int)s;
 }

@@ -40,15 +40,17 @@ def _call(name, *args):
 
 
 def conv_splits(m, cin, cout):
-    """K splits of a forward convolution over m pixels: enough workgroups
-    (~512) for small images with many channels, at least 4 K steps of 16
-    channels each; 1 (no split) when the image alone fills the chip."""
+    """K splits of a forward convolution over m pixels: about one workgroup
+    per compute unit (~256) for small images with many channels, at least 8
+    K steps of 16 channels each (fewer, longer splits: every split writes
+    and the finish pass re-reads a float32 partial of the whole output); 1
+    (no split) when the image alone fills the chip."""
     if cin % 16:
         return 1
     base = -(-m // 256) * (cout // (64 if cout % 64 == 0 else 32))
     if base >= 256:
         return 1
-    return max(1, min((9 * cin // 16) // 4, -(-512 // base)))
+    return max(1, min((9 * cin // 16) // 8, -(-256 // base)))
 
 
 class Conv3x3Fn(torch.autograd.Function):
