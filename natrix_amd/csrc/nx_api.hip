@@ -88,6 +88,8 @@ struct nx_sim {
   float* rrow = nullptr;    // restriction row records (2 x H x Wc)
   float* cws = nullptr;     // banded coarse smoother: x, r, d twice
   float* fcheb = nullptr;   // fine Chebyshev smoother: r, d twice (4 W H)
+  double* cheb_red = nullptr;  // its residual records, 5 per block
+  size_t cheb_red_cap = 0;
   size_t cws_cap = 0;
   size_t rrow_cap = 0;
   double* line = nullptr;   // {alpha, d.r, d.Ld, 0}
@@ -1221,6 +1223,7 @@ int nx_destroy(nx_sim* s) {
   (void)hipFree(s->rrow);
   (void)hipFree(s->cws);
   (void)hipFree(s->fcheb);
+  (void)hipFree(s->cheb_red);
   (void)hipFree(s->line);
   (void)hipFree(s->curl);
   (void)hipFree(s->red5);
@@ -1549,32 +1552,41 @@ int nx_smooth_chebyshev(nx_sim* s, int iterations, float lambda_min, const int* 
   if (!cheb_cfg_supported(c)) return fail(NX_ERR_ARG, "unsupported Chebyshev tile");
   if (!s->fcheb) NX_HIP(hipMalloc(&s->fcheb, (size_t)4 * s->W * s->H * sizeof(float)));
   // x ping-pongs between the pressure buffers, (r, d) between the halves of
-  // fcheb; the first launch spends one pass on the setup
+  // fcheb; the first launch spends one pass on the setup, the last one pass
+  // on the FP64 residual of the result (records in cheb_red, one per block)
   const size_t n = (size_t)s->W * s->H;
   float* rd[2][2] = {{s->fcheb, s->fcheb + n}, {s->fcheb + 2 * n, s->fcheb + 3 * n}};
+  const size_t red_need = (size_t)jacobi_launch_blocks(s->W, s->H, c) * 5;
+  if (red_need > s->cheb_red_cap) {
+    if (s->cheb_red) NX_HIP(hipFree(s->cheb_red));
+    s->cheb_red = nullptr;
+    NX_HIP(hipMalloc(&s->cheb_red, red_need * sizeof(double)));
+    s->cheb_red_cap = red_need;
+  }
   ChebRecurrence rec(lambda_min);
-  int done = 0, cur = 0;
-  while (done < iterations) {
+  int done = 0, cur = 0, last_blocks = 0;
+  for (bool last = false; !last;) {
     const int setup = done == 0 ? 1 : 0;
     JacobiCfg cc = c;
-    cc.k = std::min(c.k, iterations - done + setup);
-    const bool last = done + cc.k - setup >= iterations;
+    last = setup + (iterations - done) + 1 <= c.k;
+    cc.k = last ? setup + (iterations - done) + 1 : c.k;
+    const int its = cc.k - setup - (last ? 1 : 0);
     ChebCoef cf{};
-    rec.next(setup, cc.k, cf);
+    rec.next(setup, setup + its, cf);
     NX_LAUNCH_TIMED(s, NX_TIME_CHEB,
                     launch_cheb_tb(s->stream, cc, s->p[s->pcur], rd[cur][0], rd[cur][1],
                                    s->div, s->mask[s->mcur], s->p[1 - s->pcur],
                                    last ? nullptr : rd[1 - cur][0], rd[1 - cur][1], s->W,
-                                   s->H, setup, rec.theta, cf));
+                                   s->H, setup, rec.theta, cf,
+                                   last ? s->cheb_red : nullptr));
+    if (last) last_blocks = jacobi_launch_blocks(s->W, s->H, cc);
     s->pcur ^= 1;
     cur ^= 1;
-    done += cc.k - setup;
+    done += its;
   }
   s->step_sweeps += iterations;
-  NX_HIP(launch_sums(s->p[s->pcur], s->div, s->mask[s->mcur], s->W, s->H, s->red5,
-                     s->stream));
-  NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,
-                       nullptr, 0, 0, s->stream));
+  NX_HIP(launch_reduce(s->cheb_red, last_blocks, 5, 0, s->sums, -1.0, nullptr, 0, 0,
+                       s->stream));
   double rel = 0.0, alpha = 0.0;
   NX_HIP(hipMemcpyAsync(&rel, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
                         s->stream));

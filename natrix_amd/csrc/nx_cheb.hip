@@ -37,13 +37,17 @@ using jt::lane_from_right;
 using jt::ilane_from_left;
 using jt::ilane_from_right;
 
-template <int ROWS, int WAVES, int COLS>
+// RES: the launch's last pass evaluates the exact FP64 residual
+// r = b - L x of the final iterate on the written interior (the expression
+// of the residual sums, r = b - ((((x1 + x2) + y1) + y2) - 4 x) in double)
+// and writes red[5 blk + {sum r, sum r^2, sum b, sum b^2, fluid cells}].
+template <int ROWS, int WAVES, int COLS, bool RES>
 __global__ void __launch_bounds__(WAVES * 64)
     k_cheb_tb(const float* __restrict__ xin, const float* __restrict__ rin,
               const float* __restrict__ din, const float* __restrict__ b,
               const uint8_t* __restrict__ mask, float* __restrict__ xout,
               float* __restrict__ rout, float* __restrict__ dout, int W, int H, int k,
-              int first, float theta, ChebCoef cf) {
+              int first, float theta, ChebCoef cf, double* __restrict__ red) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -56,6 +60,7 @@ __global__ void __launch_bounds__(WAVES * 64)
   // inside or outside the grid and the written interior
   const bool vec = COLS == 2 && (W % 2 == 0) && (k % 2 == 0);
   float x[ROWS][COLS], r[ROWS][COLS], d[ROWS][COLS];
+  float bb[RES ? ROWS : 1][RES ? COLS : 1];  // b for the residual pass
   int sol[ROWS][COLS];
 #pragma unroll
   for (int i = 0; i < ROWS; ++i) {
@@ -77,6 +82,11 @@ __global__ void __launch_bounds__(WAVES * 64)
       d[i][COLS - 1] = dv.y;
       sol[i][0] = (mm & 0xFF) != 0;
       sol[i][COLS - 1] = (mm >> 8) != 0;
+      if (RES) {
+        const float2 bv = in ? *reinterpret_cast<const float2*>(b + idx) : z;
+        bb[RES ? i : 0][0] = bv.x;
+        bb[RES ? i : 0][RES ? COLS - 1 : 0] = bv.y;
+      }
       continue;
     }
 #pragma unroll
@@ -88,6 +98,7 @@ __global__ void __launch_bounds__(WAVES * 64)
       r[i][c] = in ? (first ? b[idx] : rin[idx]) : 0.0f;
       d[i][c] = (in && !first) ? din[idx] : 0.0f;
       sol[i][c] = in ? (mask[idx] != 0) : 1;
+      if (RES) bb[RES ? i : 0][RES ? c : 0] = in ? b[idx] : 0.0f;
     }
   }
 
@@ -120,9 +131,9 @@ __global__ void __launch_bounds__(WAVES * 64)
   }
   const bool special = __any(any_nbr);
 
-  // out = L v = ((((x1 + x2) + y1) + y2) - 4 v) over the tile (garbage on
-  // the tile's edge cells, which lie outside the valid cone)
-  auto lap = [&](const float (&v)[ROWS][COLS], float (&out)[ROWS][COLS], int buf) {
+  // f(i, c, centre, x1, x2, y1, y2) for every tile cell with the neighbours
+  // of v (garbage on the tile's edge cells, which lie outside the valid cone)
+  auto stencil = [&](const float (&v)[ROWS][COLS], int buf, auto&& f) {
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       xch[buf][0][wave][tcol0 + c] = v[0][c];
@@ -153,15 +164,21 @@ __global__ void __launch_bounds__(WAVES * 64)
           y1 = (code & 4) ? vc : y1;
           y2 = (code & 8) ? vc : y2;
         }
-        out[i][c] = (((x1 + x2) + y1) + y2) - 4.0f * vc;
+        f(i, c, vc, x1, x2, y1, y2);
       }
     }
   };
-
+  // out = L v = ((((x1 + x2) + y1) + y2) - 4 v)
   float lv[ROWS][COLS];
+  auto lap = [&](const float (&v)[ROWS][COLS], int buf) {
+    stencil(v, buf, [&](int i, int c, float vc, float x1, float x2, float y1, float y2) {
+      lv[i][c] = (((x1 + x2) + y1) + y2) - 4.0f * vc;
+    });
+  };
+
   int s = 0;
   if (first) {
-    lap(x, lv, 0);
+    lap(x, 0);
 #pragma unroll
     for (int i = 0; i < ROWS; ++i)
 #pragma unroll
@@ -171,8 +188,9 @@ __global__ void __launch_bounds__(WAVES * 64)
       }
     s = 1;
   }
-  for (; s < k; ++s) {
-    lap(d, lv, s & 1);
+  const int iter_end = RES ? k - 1 : k;
+  for (; s < iter_end; ++s) {
+    lap(d, s & 1);
     const float cdd = cf.cd[s], crr = cf.cr[s];
 #pragma unroll
     for (int i = 0; i < ROWS; ++i)
@@ -182,6 +200,39 @@ __global__ void __launch_bounds__(WAVES * 64)
         r[i][c] = r[i][c] + lv[i][c];
         d[i][c] = cdd * d[i][c] + crr * r[i][c];
       }
+  }
+
+  if (RES) {
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    stencil(x, s & 1, [&](int i, int c, float vc, float x1, float x2, float y1, float y2) {
+      const int trow = trow0 + i, tcol = tcol0 + c, gy = gy0 + i, gx = gx0 + c;
+      if (trow < k || trow >= EY - k || tcol < k || tcol >= EX - k || gy < 0 || gy >= H ||
+          gx < 0 || gx >= W || sol[i][c])
+        return;
+      const double bd = (double)bb[RES ? i : 0][RES ? c : 0];
+      const double rr = bd - (((((double)x1 + (double)x2) + (double)y1) + (double)y2) -
+                              4.0 * (double)vc);
+      acc[0] += rr;
+      acc[1] += rr * rr;
+      acc[2] += bd;
+      acc[3] += bd * bd;
+      acc[4] += 1.0;
+    });
+    __shared__ double part[WAVES][5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[j] = jt::wave_sum(acc[j]);
+    if (lane == 0)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) part[wave][j] = acc[j];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+      for (int j = 0; j < 5; ++j) {
+        double t = 0.0;
+        for (int w = 0; w < WAVES; ++w) t += part[w][j];
+        red[5 * bid + j] = t;
+      }
+    }
   }
 
 #pragma unroll
@@ -215,12 +266,16 @@ __global__ void __launch_bounds__(WAVES * 64)
 template <int ROWS, int WAVES, int COLS>
 hipError_t launch_cheb(const float* xin, const float* rin, const float* din, const float* b,
                        const uint8_t* mask, float* xout, float* rout, float* dout, int W,
-                       int H, int k, int first, float theta, const ChebCoef& cf,
+                       int H, int k, int first, float theta, const ChebCoef& cf, double* red,
                        hipStream_t st) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + EY - 2 * k - 1) / (EY - 2 * k));
-  k_cheb_tb<ROWS, WAVES, COLS><<<grid, WAVES * 64, 0, st>>>(
-      xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf);
+  if (red)
+    k_cheb_tb<ROWS, WAVES, COLS, true><<<grid, WAVES * 64, 0, st>>>(
+        xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf, red);
+  else
+    k_cheb_tb<ROWS, WAVES, COLS, false><<<grid, WAVES * 64, 0, st>>>(
+        xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf, nullptr);
   return hipGetLastError();
 }
 
@@ -246,12 +301,12 @@ bool cheb_cfg_supported(JacobiCfg c) {
 hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const float* rin,
                           const float* din, const float* b, const uint8_t* mask, float* xout,
                           float* rout, float* dout, int W, int H, int setup, float theta,
-                          const ChebCoef& cf) {
+                          const ChebCoef& cf, double* red) {
   if (!cheb_cfg_supported(c)) return hipErrorInvalidValue;
 #define NX_CHEB_LAUNCH(R, WV, CL)                                                     \
   if (c.rows == R && c.waves == WV && c.cols == CL)                                  \
     return launch_cheb<R, WV, CL>(xin, rin, din, b, mask, xout, rout, dout, W, H, c.k, \
-                                  setup, theta, cf, st);
+                                  setup, theta, cf, red, st);
   NX_CHEB_SHAPES(NX_CHEB_LAUNCH)
 #undef NX_CHEB_LAUNCH
   return hipErrorInvalidValue;

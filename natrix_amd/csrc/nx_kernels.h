@@ -92,7 +92,11 @@ int prolong_line_search_padded(::nx_sim* s, int factor, int Wp, int Hp, const fl
 // a (64 c.cols) x (c.rows c.waves) register tile per block: with setup != 0
 // the first pass computes r = L x - b, d = r / theta from x and b (rin, din
 // unread), every other pass s is an iteration x += d, r += L d,
-// d = cf.cd[s] d + cf.cr[s] r.  rout == nullptr: only x is written.
+// d = cf.cd[s] d + cf.cr[s] r.  rout == nullptr: only x is written.  With
+// red, the last pass instead evaluates the exact FP64 residual b - L x of
+// the final iterate on the written interior and each block writes
+// red[5 blk + {sum r, sum r^2, sum b, sum b^2, fluid cells}] (blocks:
+// jacobi_launch_blocks of the launch's shape and passes).
 constexpr int CHEB_MAX_PASSES = 32;
 struct ChebCoef {
   float cd[CHEB_MAX_PASSES];
@@ -110,7 +114,7 @@ bool cheb_cfg_supported(JacobiCfg c);
 hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const float* rin,
                           const float* din, const float* b, const uint8_t* mask, float* xout,
                           float* rout, float* dout, int W, int H, int setup, float theta,
-                          const ChebCoef& cf);
+                          const ChebCoef& cf, double* red);
 
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
 // sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 8

@@ -92,26 +92,41 @@ __device__ __forceinline__ float block_sum_1024(float v, float* sh) {
 }
 
 // x[p] = (bf16(rc[p] / s), bf16(mask), 0, ...), s = sqrt(sum_fluid rc^2 / n)
+// The coarse grid has at most 128 x 128 cells: each of the 1024 threads
+// holds its (at most) 16 cells i = t, t + 1024, ... in registers, so the
+// loads of a pass are issued together; sums run over them in that order.
+constexpr int CELLS_PER_THREAD = 16;
+
+// x[p] = (bf16(rc[p] / s), bf16(mask), 0, ...), s = sqrt(sum_fluid rc^2 / n)
 __global__ void __launch_bounds__(1024)
     k_pack_input(const float* __restrict__ rc, const uint8_t* __restrict__ mc,
                  uint16_t* __restrict__ x, float* __restrict__ scal, int n) {
   __shared__ float sh[16];
+  float v[CELLS_PER_THREAD];
+  bool solid[CELLS_PER_THREAD];
+#pragma unroll
+  for (int k = 0; k < CELLS_PER_THREAD; ++k) {
+    const int i = threadIdx.x + 1024 * k;
+    v[k] = i < n ? rc[i] : 0.0f;
+    solid[k] = i < n ? mc[i] != 0 : true;
+  }
   float ss = 0.0f, cnt = 0.0f;
-  for (int i = threadIdx.x; i < n; i += 1024) {
-    if (mc[i] == 0) {
-      ss += rc[i] * rc[i];
+#pragma unroll
+  for (int k = 0; k < CELLS_PER_THREAD; ++k)
+    if (!solid[k]) {
+      ss += v[k] * v[k];
       cnt += 1.0f;
     }
-  }
   ss = block_sum_1024(ss, sh);
   cnt = block_sum_1024(cnt, sh);
   const float s = fmaxf(sqrtf(ss / fmaxf(cnt, 1.0f)), 1e-30f);
   if (threadIdx.x == 0) scal[0] = s;
-  for (int i = threadIdx.x; i < n; i += 1024) {
-    uint16_t* px = x + 8 * (size_t)i;
-    px[0] = (uint16_t)nx::f32_to_bf16(rc[i] / s);
-    px[1] = (uint16_t)nx::f32_to_bf16(mc[i] ? 1.0f : 0.0f);
-    for (int c = 2; c < 8; ++c) px[c] = 0;
+#pragma unroll
+  for (int k = 0; k < CELLS_PER_THREAD; ++k) {
+    const int i = threadIdx.x + 1024 * k;
+    if (i >= n) break;
+    const uint32_t lo = nx::f32_to_bf16(v[k] / s) | (nx::f32_to_bf16(solid[k] ? 1.0f : 0.0f) << 16);
+    reinterpret_cast<uint4*>(x)[i] = make_uint4(lo, 0u, 0u, 0u);
   }
 }
 
@@ -124,17 +139,29 @@ __global__ void k_scale_output(float* __restrict__ e, const float* __restrict__ 
 __global__ void __launch_bounds__(1024)
     k_center_output(float* __restrict__ e, const uint8_t* __restrict__ mc, int n) {
   __shared__ float sh[16];
+  float v[CELLS_PER_THREAD];
+  bool solid[CELLS_PER_THREAD];
+#pragma unroll
+  for (int k = 0; k < CELLS_PER_THREAD; ++k) {
+    const int i = threadIdx.x + 1024 * k;
+    v[k] = i < n ? e[i] : 0.0f;
+    solid[k] = i < n ? mc[i] != 0 : true;
+  }
   float sum = 0.0f, cnt = 0.0f;
-  for (int i = threadIdx.x; i < n; i += 1024) {
-    if (mc[i] == 0) {
-      sum += e[i];
+#pragma unroll
+  for (int k = 0; k < CELLS_PER_THREAD; ++k)
+    if (!solid[k]) {
+      sum += v[k];
       cnt += 1.0f;
     }
-  }
   sum = block_sum_1024(sum, sh);
   cnt = block_sum_1024(cnt, sh);
   const float m = sum / fmaxf(cnt, 1.0f);
-  for (int i = threadIdx.x; i < n; i += 1024) e[i] = mc[i] ? 0.0f : e[i] - m;
+#pragma unroll
+  for (int k = 0; k < CELLS_PER_THREAD; ++k) {
+    const int i = threadIdx.x + 1024 * k;
+    if (i < n) e[i] = solid[k] ? 0.0f : v[k] - m;
+  }
 }
 
 int conv(nx_unet* u, int i, const void* x, void* y, int h, int w, hipStream_t s) {
