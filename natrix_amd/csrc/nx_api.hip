@@ -545,40 +545,81 @@ __global__ void __launch_bounds__(256)
   rcnt[y * Wc + X] = (float)fluid;
 }
 
+// Row streaming for the residual-type kernels below: wave w of a 64 x
+// RES_ROWS block walks rows y0 .. y0 + RES_ROWS/4 - 1 of column x = 64 bx +
+// lane, holding the rows above and below in registers; the left and right
+// neighbours come from the adjacent lanes by DPP, lanes 0 and 63 load theirs
+// (x - 1, x + 1) and the grid's edges use the cell itself, as the clamped
+// index does.  Lanes past the last column read column W - 1 and write
+// nothing, so every lane runs the same code.
+struct RowNbr {
+  float l, r;
+  int ml, mr;
+};
+
+__device__ __forceinline__ RowNbr row_nbr(const float* __restrict__ p,
+                                          const uint8_t* __restrict__ mask, int row, int x,
+                                          int W, int lane, float pc, int mcv) {
+  RowNbr n;
+  n.l = jt::lane_from_left(pc);
+  n.r = jt::lane_from_right(pc);
+  n.ml = jt::ilane_from_left(mcv);
+  n.mr = jt::ilane_from_right(mcv);
+  if (lane == 0) {
+    n.l = x > 0 ? p[row + x - 1] : pc;
+    n.ml = x > 0 ? mask[row + x - 1] : mcv;
+  }
+  if (lane == 63 || x + 1 >= W) {
+    n.r = x + 1 < W ? p[row + x + 1] : pc;
+    n.mr = x + 1 < W ? mask[row + x + 1] : mcv;
+  }
+  return n;
+}
+
 // The same row records when f is a power of two dividing 64: lane l owns
-// fine column 64 b + l, so every load is one contiguous row segment, and
-// the f lanes of a coarse column add their values with xor shuffles inside
-// their aligned group (lane order differs from the loop above only in the
-// float32 summation order).
+// fine column 64 b + l, and the f lanes of a coarse column add their values
+// with xor shuffles inside their aligned group (lane order differs from the
+// loop above only in the float32 summation order); rows stream as above.
 __global__ void __launch_bounds__(256)
     k_restrict_rows_seg(const float* __restrict__ p, const float* __restrict__ b,
                         const uint8_t* __restrict__ mask, float* __restrict__ rsum,
                         float* __restrict__ rcnt, int W, int H, int f, int Wc) {
-  const int lane = threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
-  const int x = blockIdx.x * 64 + lane;
-  if (y >= H) return;  // whole waves: y is uniform across a wave
-  float v = 0.0f, c = 0.0f;
-  if (x < W) {
-    const int row = y * W, pos = row + x;
-    if (mask[pos] == 0) {
-      const int iL = row + max(x - 1, 0), iR = row + min(x + 1, W - 1);
-      const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
-      const float pc = p[pos];
-      const float xl = mask[iL] ? pc : p[iL];
-      const float xr = mask[iR] ? pc : p[iR];
-      const float yb = mask[iB] ? pc : p[iB];
-      const float yt = mask[iT] ? pc : p[iT];
-      v = b[pos] - ((((xl + xr) + yb) + yt) - 4.0f * pc);
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  const int x = blockIdx.x * 64 + lane, xc = min(x, W - 1);
+  const int y0 = blockIdx.y * RES_ROWS + wave * (RES_ROWS / 4);
+  if (y0 >= H) return;  // wave-uniform
+  const int y1 = min(y0 + RES_ROWS / 4, H);
+  float pc = p[y0 * W + xc];
+  int mcv = mask[y0 * W + xc];
+  float pu = y0 > 0 ? p[(y0 - 1) * W + xc] : pc;
+  int mu = y0 > 0 ? mask[(y0 - 1) * W + xc] : mcv;
+  for (int y = y0; y < y1; ++y) {
+    const int row = y * W;
+    const bool dn = y + 1 < H;
+    const float pd = dn ? p[row + W + xc] : pc;
+    const int md = dn ? mask[row + W + xc] : mcv;
+    const RowNbr n = row_nbr(p, mask, row, x, W, lane, pc, mcv);
+    float v = 0.0f, c = 0.0f;
+    if (x < W && mcv == 0) {
+      const float xl = n.ml ? pc : n.l;
+      const float xr = n.mr ? pc : n.r;
+      const float yb = mu ? pc : pu;
+      const float yt = md ? pc : pd;
+      v = b[row + x] - ((((xl + xr) + yb) + yt) - 4.0f * pc);
       c = 1.0f;
     }
-  }
-  for (int off = f >> 1; off > 0; off >>= 1) {
-    v += __shfl_xor(v, off, 64);
-    c += __shfl_xor(c, off, 64);
-  }
-  if ((lane & (f - 1)) == 0 && x < W) {
-    rsum[y * Wc + x / f] = v;
-    rcnt[y * Wc + x / f] = c;
+    for (int off = f >> 1; off > 0; off >>= 1) {
+      v += __shfl_xor(v, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    if ((lane & (f - 1)) == 0 && x < W) {
+      rsum[y * Wc + x / f] = v;
+      rcnt[y * Wc + x / f] = c;
+    }
+    pu = pc;
+    mu = mcv;
+    pc = pd;
+    mcv = md;
   }
 }
 
@@ -948,27 +989,50 @@ __global__ void __launch_bounds__(256)
     k_line_terms(const float* __restrict__ p, const float* __restrict__ b,
                  const uint8_t* __restrict__ mask, const float* __restrict__ d,
                  double* __restrict__ red, int W, int H) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
+  // rows stream as in k_restrict_rows_seg; a solid neighbour is left out
+  // (its difference to the centre is 0), a clamped one is the cell itself
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  const int x = blockIdx.x * 64 + lane, xc = min(x, W - 1);
+  const int y0 = blockIdx.y * RES_ROWS + wave * (RES_ROWS / 4);
+  const int y1 = min(y0 + RES_ROWS / 4, H);
   double v0 = 0.0, v1 = 0.0;
-  for (int r = 0; r < RES_ROWS / 4; ++r) {
-    const int y = blockIdx.y * RES_ROWS + threadIdx.y + 4 * r;
-    if (x >= W || y >= H) continue;
-    const int pos = y * W + x;
-    if (mask[pos] != 0) continue;
-    const int nb[4] = {y * W + max(x - 1, 0), y * W + min(x + 1, W - 1),
-                       max(y - 1, 0) * W + x, min(y + 1, H - 1) * W + x};
-    const double pc = (double)p[pos], dc = (double)d[pos];
-    double lp = 0.0, ld = 0.0;
-    for (int j = 0; j < 4; ++j) {
-      if (mask[nb[j]]) continue;
-      lp += (double)p[nb[j]] - pc;
-      ld += (double)d[nb[j]] - dc;
+  if (y0 < H) {
+    float pc = p[y0 * W + xc], dc = d[y0 * W + xc];
+    int mcv = mask[y0 * W + xc];
+    float pu = y0 > 0 ? p[(y0 - 1) * W + xc] : pc;
+    float du = y0 > 0 ? d[(y0 - 1) * W + xc] : dc;
+    int mu = y0 > 0 ? mask[(y0 - 1) * W + xc] : mcv;
+    for (int y = y0; y < y1; ++y) {
+      const int row = y * W;
+      const bool dn = y + 1 < H;
+      const float pd = dn ? p[row + W + xc] : pc;
+      const float dd = dn ? d[row + W + xc] : dc;
+      const int md = dn ? mask[row + W + xc] : mcv;
+      const RowNbr np = row_nbr(p, mask, row, x, W, lane, pc, mcv);
+      const RowNbr nd = row_nbr(d, mask, row, x, W, lane, dc, mcv);
+      if (x < W && mcv == 0) {
+        const double pcd = (double)pc, dcd = (double)dc;
+        const float pv[4] = {np.l, np.r, pu, pd}, dv[4] = {nd.l, nd.r, du, dd};
+        const int sv[4] = {np.ml, np.mr, mu, md};
+        double lp = 0.0, ld = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (sv[j]) continue;
+          lp += (double)pv[j] - pcd;
+          ld += (double)dv[j] - dcd;
+        }
+        v0 += dcd * ((double)b[row + x] - lp);
+        v1 += dcd * ld;
+      }
+      pu = pc;
+      du = dc;
+      mu = mcv;
+      pc = pd;
+      dc = dd;
+      mcv = md;
     }
-    v0 += dc * ((double)b[pos] - lp);
-    v1 += dc * ld;
   }
   __shared__ double part[4][2];
-  const int lane = threadIdx.x, wave = threadIdx.y;
   for (int off = 32; off > 0; off >>= 1) {
     v0 += __shfl_xor(v0, off, 64);
     v1 += __shfl_xor(v1, off, 64);
@@ -1042,34 +1106,55 @@ __global__ void __launch_bounds__(256)
                       const float* __restrict__ b0, const uint8_t* __restrict__ mask,
                       float* __restrict__ r_out, float* __restrict__ p1,
                       double* __restrict__ red, int W, int H) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
+  // rows stream as in k_restrict_rows_seg, with the pair (ph, pl) per cell
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  const int x = blockIdx.x * 64 + lane, xc = min(x, W - 1);
+  const int y0 = blockIdx.y * RES_ROWS + wave * (RES_ROWS / 4);
+  const int y1 = min(y0 + RES_ROWS / 4, H);
   double v[5] = {0, 0, 0, 0, 0};
-  for (int rr_ = 0; rr_ < RES_ROWS / 4; ++rr_) {
-    const int y = blockIdx.y * RES_ROWS + threadIdx.y + 4 * rr_;
-    if (x >= W || y >= H) continue;
-    const int pos = y * W + x;
-    const int iL = y * W + max(x - 1, 0), iR = y * W + min(x + 1, W - 1);
-    const int iB = max(y - 1, 0) * W + x, iT = min(y + 1, H - 1) * W + x;
-    const double pc = (double)ph[pos] + (double)pl[pos];
-    const double x1 = mask[iL] ? pc : (double)ph[iL] + (double)pl[iL];
-    const double x2 = mask[iR] ? pc : (double)ph[iR] + (double)pl[iR];
-    const double y1 = mask[iB] ? pc : (double)ph[iB] + (double)pl[iB];
-    const double y2 = mask[iT] ? pc : (double)ph[iT] + (double)pl[iT];
-    const double bd = (double)b0[pos];
-    const double rr = bd - ((((x1 + x2) + y1) + y2) - 4.0 * pc);
-    const float rf = mask[pos] ? 0.0f : (float)rr;
-    r_out[pos] = rf;
-    p1[pos] = (0.0f - rf) * 0.25f;
-    if (mask[pos] == 0) {
-      v[0] += rr;
-      v[1] += rr * rr;
-      v[2] += bd;
-      v[3] += bd * bd;
-      v[4] += 1.0;
+  if (y0 < H) {
+    float hc = ph[y0 * W + xc], lc = pl[y0 * W + xc];
+    int mcv = mask[y0 * W + xc];
+    float hu = y0 > 0 ? ph[(y0 - 1) * W + xc] : hc;
+    float lu = y0 > 0 ? pl[(y0 - 1) * W + xc] : lc;
+    int mu = y0 > 0 ? mask[(y0 - 1) * W + xc] : mcv;
+    for (int y = y0; y < y1; ++y) {
+      const int row = y * W;
+      const bool dn = y + 1 < H;
+      const float hd = dn ? ph[row + W + xc] : hc;
+      const float ld = dn ? pl[row + W + xc] : lc;
+      const int md = dn ? mask[row + W + xc] : mcv;
+      const RowNbr nh = row_nbr(ph, mask, row, x, W, lane, hc, mcv);
+      const RowNbr nl = row_nbr(pl, mask, row, x, W, lane, lc, mcv);
+      if (x < W) {
+        const int pos = row + x;
+        const double pc = (double)hc + (double)lc;
+        const double x1 = nh.ml ? pc : (double)nh.l + (double)nl.l;
+        const double x2 = nh.mr ? pc : (double)nh.r + (double)nl.r;
+        const double y1v = mu ? pc : (double)hu + (double)lu;
+        const double y2v = md ? pc : (double)hd + (double)ld;
+        const double bd = (double)b0[pos];
+        const double rr = bd - ((((x1 + x2) + y1v) + y2v) - 4.0 * pc);
+        const float rf = mcv ? 0.0f : (float)rr;
+        r_out[pos] = rf;
+        p1[pos] = (0.0f - rf) * 0.25f;
+        if (mcv == 0) {
+          v[0] += rr;
+          v[1] += rr * rr;
+          v[2] += bd;
+          v[3] += bd * bd;
+          v[4] += 1.0;
+        }
+      }
+      hu = hc;
+      lu = lc;
+      mu = mcv;
+      hc = hd;
+      lc = ld;
+      mcv = md;
     }
   }
   __shared__ double part[4][5];
-  const int lane = threadIdx.x, wave = threadIdx.y;
   for (int j = 0; j < 5; ++j)
     for (int off = 32; off > 0; off >>= 1) v[j] += __shfl_xor(v[j], off, 64);
   if (lane == 0)
@@ -1622,7 +1707,7 @@ static int restrict_impl(nx_sim* s, int factor, int Wp, int Hp, float* coarse_rh
   float* rsum = s->rrow;
   float* rcnt = s->rrow + (size_t)s->H * Wc;
   if (factor >= 2 && factor <= 64 && (factor & (factor - 1)) == 0)
-    k_restrict_rows_seg<<<dim3((s->W + 63) / 64, (s->H + 3) / 4), dim3(64, 4), 0,
+    k_restrict_rows_seg<<<dim3((s->W + 63) / 64, (s->H + RES_ROWS - 1) / RES_ROWS), dim3(64, 4), 0,
                           s->stream>>>(s->p[s->pcur], s->div, s->mask[s->mcur], rsum, rcnt,
                                        s->W, s->H, factor, Wc);
   else

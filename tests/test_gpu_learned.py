@@ -135,6 +135,39 @@ def test_prolongation_matches_float64_statement(gpu, rng, N, f):
     sim.destroy()
 
 
+@pytest.mark.parametrize("H,W", [(72, 100), (131, 257)])
+def test_refine_residual_matches_float64_statement(gpu, rng, H, W):
+    """After nx_refine_begin and one nx_refine_accumulate of e the pressure
+    pair is (e, 0) exactly; nx_refine_residual then makes the float32 of the
+    FP64 residual b - L e (the operator's summation order) the right-hand
+    side on fluid cells, 0 on solid ones, restarts the pressure at its first
+    Jacobi sweep (0 - r) / 4 and returns the FP64 relative residual."""
+    sim = in_step_sim(rng, H, W)
+    solid = sim.download("mask") != 0
+    b = rng.normal(size=(H, W)).astype(np.float32)
+    e = rng.normal(size=(H, W)).astype(np.float32)
+    dev(sim, "divergence").copy_(torch.from_numpy(b))
+    torch.cuda.synchronize()
+    _lib.call("nx_refine_begin", sim._alive())
+    dev(sim, "pressure").copy_(torch.from_numpy(e))
+    torch.cuda.synchronize()
+    _lib.call("nx_refine_accumulate", sim._alive())
+    rel = C.c_double()
+    _lib.call("nx_refine_residual", sim._alive(), C.byref(rel))
+    r64 = b.astype(np.float64) - lap(e.astype(np.float64), solid)
+    want = np.where(solid, np.float32(0), r64.astype(np.float32))
+    assert np.array_equal(sim.download("divergence"), want)
+    assert np.array_equal(sim.download("pressure"), (np.float32(0) - want) * np.float32(0.25))
+    rf, bf = r64[~solid], b.astype(np.float64)[~solid]
+    n = rf.size
+    want_rel = np.sqrt((np.sum(rf * rf) - np.sum(rf) ** 2 / n) /
+                       (np.sum(bf * bf) - np.sum(bf) ** 2 / n))
+    assert abs(rel.value - want_rel) <= 1e-12 * want_rel
+    _lib.call("nx_refine_end", sim._alive())
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
 def test_line_search_is_the_energy_minimiser(gpu, rng):
     H, W = 80, 96
     sim = in_step_sim(rng, H, W)
