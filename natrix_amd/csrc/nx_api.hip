@@ -845,8 +845,9 @@ __global__ void __launch_bounds__(256)
 // CB_T + 1 rows on each side, whose outer rows go stale one row per stencil,
 // and stores the band's x, r, d.  Launch boundaries are the only global
 // synchronisation.  Every cell sees the operations of the single-workgroup
-// kernel in the same order; the recurrence coefficient after s0 iterations
-// (rho_s0) comes from the host, computed by the same float32 recurrence.
+// kernel in the same order; theta and the coefficients of the launch's
+// iterations (cf) come from the host, computed by the same float32
+// operations (IEEE single on both sides).
 constexpr int CB_ROWS = 2, CB_EY = CB_ROWS * CJ_WAVES, CB_T = 8;
 constexpr int CB_BAND = CB_EY - 2 * (CB_T + 1);
 
@@ -855,7 +856,7 @@ __global__ void __launch_bounds__(1024)
                        const float* __restrict__ din, float* __restrict__ xout,
                        float* __restrict__ rout, float* __restrict__ dout,
                        const float* __restrict__ rhs, const uint8_t* __restrict__ mc,
-                       int Wc, int Hc, int s0, int T, float lmin, float rho_s0) {
+                       int Wc, int Hc, int s0, int T, float theta, ChebCoef cf) {
   __shared__ float top[2][CJ_WAVES][CJ_MAX], bot[2][CJ_WAVES][CJ_MAX];
   CoarseTile<CB_ROWS> t;
   t.top = top;
@@ -884,9 +885,6 @@ __global__ void __launch_bounds__(1024)
       t.code[i][c] = (solid(gx - 1, gy) ? 1 : 0) | (solid(gx + 1, gy) ? 2 : 0) |
                      (solid(gx, gy - 1) ? 4 : 0) | (solid(gx, gy + 1) ? 8 : 0);
     }
-  const float theta = 0.5f * (8.0f + lmin), delta = 0.5f * (8.0f - lmin);
-  const float sigma = theta / delta;
-  float rho = rho_s0;
   if (first) {
     t.lap(x, lv);
 #pragma unroll
@@ -899,8 +897,7 @@ __global__ void __launch_bounds__(1024)
   }
   for (int it = 0; it < T; ++it) {
     t.lap(d, lv);
-    const float rho1 = 1.0f / (2.0f * sigma - rho);
-    const float cd = rho1 * rho, cr = 2.0f * rho1 / delta;
+    const float cd = cf.cd[it], cr = cf.cr[it];
 #pragma unroll
     for (int i = 0; i < CB_ROWS; ++i)
 #pragma unroll
@@ -909,7 +906,6 @@ __global__ void __launch_bounds__(1024)
         r[i][c] = r[i][c] + lv[i][c];
         d[i][c] = cd * d[i][c] + cr * r[i][c];
       }
-    rho = rho1;
   }
 #pragma unroll
   for (int i = 0; i < CB_ROWS; ++i)
@@ -936,21 +932,20 @@ int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* 
     const size_t n = (size_t)Wc * Hc;
     float* set[2][3] = {{ws, ws + n, ws + 2 * n}, {ws + 3 * n, ws + 4 * n, ws + 5 * n}};
     const int nb = (Hc + CB_BAND - 1) / CB_BAND;
-    const float theta = 0.5f * (8.0f + lambda_min), delta = 0.5f * (8.0f - lambda_min);
-    const float sigma = theta / delta;
-    float rho = 1.0f / sigma;
+    ChebRecurrence rec(lambda_min);
     int done = 0, cur = 0;
     while (done < iterations) {
       const int T = iterations - done < CB_T ? iterations - done : CB_T;
       const bool last = done + T >= iterations;
       const float* xin = done == 0 ? e : set[cur][0];
       float** out = set[done == 0 ? 0 : 1 - cur];
+      ChebCoef cf{};
+      rec.next(0, T, cf);
       k_coarse_cheb_band<<<nb, 1024, 0, st>>>(xin, set[cur][1], set[cur][2],
                                               last ? e : out[0], out[1], out[2], rc, mc,
-                                              Wc, Hc, done, T, lambda_min, rho);
+                                              Wc, Hc, done, T, rec.theta, cf);
       if (hipGetLastError() != hipSuccess)
         return set_error(NX_ERR_HIP, "coarse smoother launch failed");
-      for (int k = 0; k < T; ++k) rho = 1.0f / (2.0f * sigma - rho);
       cur = done == 0 ? 0 : 1 - cur;
       done += T;
     }
