@@ -84,41 +84,33 @@ __global__ void __launch_bounds__(256)
 
   const int K = 9 * Cin;
   if ((Cin & 15) == 0) {
-    // K steps of 16 channels of one tap, in order; the fragments of step
-    // ks + 1 load while the MFMAs of step ks run (two register stages)
     const int cps = Cin >> 4;  // K steps per tap
     const int ks0 = part ? blockIdx.z * steps_per_split : 0;
     const int ks1 = part ? min(ks0 + steps_per_split, 9 * cps) : 9 * cps;
-    auto load = [&](int ks, bf16x8 (&a)[MT], bf16x8 (&b)[NT]) {
-      const int tap = ks / cps;
-      const int ci = (ks - tap * cps) * 16 + kb * 8;
+    for (int tap = ks0 / cps; tap < 9 && tap * cps < ks1; ++tap) {
       const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      long long src[MT];
 #pragma unroll
       for (int i = 0; i < MT; ++i) {
         const int yy = py[i] + dy, xx = px[i] + dx;
-        a[i] = (valid[i] && yy >= 0 && yy < H && xx >= 0 && xx < W)
-                   ? ld_frag(x + (pimg[i] + (long long)yy * W + xx) * Cin + ci)
-                   : zero_frag();
+        src[i] = (valid[i] && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                     ? (pimg[i] + (long long)yy * W + xx) * Cin
+                     : -1;
       }
+      const int cb = max(ks0 - tap * cps, 0) * 16, ce = min(ks1 - tap * cps, cps) * 16;
+      for (int c0 = cb; c0 < ce; c0 += 16) {
+        const int ci = c0 + kb * 8;
+        bf16x8 a[MT], b[NT];
 #pragma unroll
-      for (int j = 0; j < NT; ++j)
-        b[j] = ld_frag(w + ((long long)(n_base + 32 * j + r) * 9 + tap) * Cin + ci);
-    };
-    auto mma = [&](const bf16x8 (&a)[MT], const bf16x8 (&b)[NT]) {
-#pragma unroll
-      for (int i = 0; i < MT; ++i)
+        for (int i = 0; i < MT; ++i) a[i] = src[i] >= 0 ? ld_frag(x + src[i] + ci) : zero_frag();
 #pragma unroll
         for (int j = 0; j < NT; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    };
-    bf16x8 a0[MT], b0[NT], a1[MT], b1[NT];
-    if (ks0 < ks1) load(ks0, a0, b0);
-    for (int ks = ks0; ks < ks1; ks += 2) {
-      if (ks + 1 < ks1) load(ks + 1, a1, b1);
-      mma(a0, b0);
-      if (ks + 1 < ks1) {
-        if (ks + 2 < ks1) load(ks + 2, a0, b0);
-        mma(a1, b1);
+          b[j] = ld_frag(w + ((long long)(n_base + 32 * j + r) * 9 + tap) * Cin + ci);
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
       }
     }
   } else {
