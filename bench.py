@@ -316,9 +316,9 @@ def bench_mixed(args):
     err32 = None if args.no_errors else mixed_error_vs_f32(N, args.storage, iters)
     ms = 1e3 * el / args.steps
     cells = N * N
-    # bf16/f16 bytes per cell: 54 + 7*I + splats 12 + the
-    # correction launch (p 2 + b 2 + mask 1 in, p32 4 out) + projection's
-    # float32 p (2 extra)
+    # bf16/f16 bytes per cell: 54 for the other stages + 7 per Jacobi sweep
+    # (p 2 + b 2 + mask 1 in, p 2 out) + splats 12 + the correction launch
+    # (p 2 + b 2 + mask 1 in, p32 4 out) + projection's float32 p (2 extra)
     step_bytes = cells * (54 + 7 * (iters - 1) + 12 + 9 + 2)
     jac_ms, jac_n = timing["jacobi"]
     roof = None

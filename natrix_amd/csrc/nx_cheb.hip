@@ -9,11 +9,12 @@
 //   iteration:  x += d,  r += L d,  d = (rho' rho) d + (2 rho' / delta) r
 // with theta = (8 + lmin) / 2, delta = (8 - lmin) / 2, sigma = theta /
 // delta, rho_0 = 1 / sigma, rho' = 1 / (2 sigma - rho) (the scalar
-// recurrence runs on the host; a launch takes its passes' coefficients).  -L is symmetric
-// positive semi-definite with spectrum in [0, 8], so the error polynomial is
-// at most 1 in magnitude on all of [0, 8] and at most 1 / T_n(sigma) on
-// [lmin, 8]: n iterations damp every mode above lmin, which n Jacobi sweeps
-// (factor (1 - lambda / 4)^n) leave nearly untouched near lmin.
+// recurrence runs on the host; a launch takes its passes' coefficients).
+// -L is symmetric positive semi-definite with spectrum in [0, 8], so the
+// error polynomial is at most 1 in magnitude on all of [0, 8] and at most
+// 1 / T_n(sigma) on [lmin, 8]: n iterations damp every mode above lmin,
+// which n Jacobi sweeps (factor (1 - lambda / 4)^n) leave nearly untouched
+// near lmin.
 //
 // Only d needs neighbours, so a launch carries (x, r, d) of a register tile
 // through k stencil passes exactly as the Jacobi tile carries p: lane l of
@@ -21,7 +22,8 @@
 // lane shifts, the rows next to a wave's strip through a double-buffered
 // LDS exchange, and the (64 COLS - 2k) x (ROWS WAVES - 2k) interior whose
 // k-pass dependency cone stayed inside the tile is written back.  The first
-// launch of a run spends its first pass on the setup.  Every pass evaluates
+// launch of a run spends its first pass on the setup, the last one (RES)
+// its last pass on the FP64 residual of the result.  Every pass evaluates
 // each cell's expressions in a fixed order without contraction, so the
 // result is bitwise independent of the tile shape and of k.
 #include "../../include/natrix.h"
