@@ -123,6 +123,14 @@ int nx_add_dye(nx_sim* sim, float px, float py, float amount, float radius);
  * value stored to an rgba8 image in every channel: clamp(v, 0, 1) * 255
  * rounded to nearest (ties up), NaN -> 0; on the simulator's stream */
 int nx_dye_rgba8(nx_sim* sim, void* out);
+/* the demo's velocity arrows (demo/shaders/demo.QuiverFragmentShader.frag:
+ * 14-70, uniforms as simulation_demo.py:251-273: WindowSize = the
+ * framebuffer, VelocitySize = the simulator's grid, ArrowTileSize):
+ * out[window_h * window_w] (device, one uint32 per pixel, bytes R,G,B,A =
+ * 255, 255, 255, alpha) with row y at gl_FragCoord.y = y + 0.5; alpha =
+ * 1 - clamp(distance to the pixel's tile arrow, 0, 1) rounded to 8 bits;
+ * on the simulator's stream */
+int nx_quiver_rgba8(nx_sim* sim, int window_w, int window_h, float arrow_tile, void* out);
 /* Gaussian splats (no reference counterpart): n records of 6 floats
  * (px, py, ax, ay, radius, dye); velocity += (ax, ay) * w, dye += dye * w,
  * w = exp(-d^2 / r^2).  Amplitudes are final (already scaled by dt). */

@@ -91,6 +91,7 @@ SIGNATURES = {
     "nx_add_triangle_obstacle": [_P, _F, _F, _F, _F, _F, _F, _I],
     "nx_add_dye": [_P, _F, _F, _F, _F],
     "nx_dye_rgba8": [_P, _P],
+    "nx_quiver_rgba8": [_P, _I, _I, _F, _P],
     "nx_add_gaussian_splats": [_P, C.POINTER(_F), _I],
     "nx_set_obstacle_map": [_P, C.POINTER(C.c_uint8)],
     "nx_update": [_P, _F],

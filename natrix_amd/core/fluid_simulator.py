@@ -399,6 +399,20 @@ class FluidSimulator:
         self.synchronize()
         return out if device else out.cpu().numpy()
 
+    def quiver_rgba8(self, window_w, window_h, arrow_tile, device=False):
+        """The demo's velocity arrows over a window_w x window_h framebuffer
+        (demo/shaders/demo.QuiverFragmentShader.frag, one arrow per
+        arrow_tile-pixel tile, row y at gl_FragCoord.y = y + 0.5): a
+        (window_h, window_w, 4) uint8 image, white with the arrows in alpha,
+        as numpy or with `device` a torch tensor on the GPU."""
+        import torch
+        out = torch.empty((int(window_h), int(window_w), 4), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        _lib.call("nx_quiver_rgba8", self._alive(), int(window_w), int(window_h),
+                  float(arrow_tile), C.c_void_p(out.data_ptr()))
+        self.synchronize()
+        return out if device else out.cpu().numpy()
+
     def download(self, name):
         fid = _FIELD_NAMES[name] if isinstance(name, str) else int(name)
         ptr, nbytes, _ = self.field_buffer_raw(fid)

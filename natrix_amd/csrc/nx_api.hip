@@ -973,6 +973,72 @@ __global__ void k_dye_rgba8(const typename S::s1* __restrict__ q, uint32_t* __re
   out[i] = c | (c << 8) | (c << 16) | (c << 24);
 }
 
+// The demo's velocity arrows, demo/shaders/demo.QuiverFragmentShader.frag:
+// 14-70, for every pixel of a Ww x Hw framebuffer (row y holds
+// gl_FragCoord.y = y + 0.5), as an rgba8 image (1, 1, 1, alpha).  Each
+// pixel samples the velocity at its arrow tile's centre exactly as _field
+// does: fPos = (1 - pos / WindowSize) * VelocitySize, corners clamped to
+// the grid, mix(a, b, (t, 0)) (which interpolates only the x component;
+// the y component is the bottom-left corner's), scaled by -WindowSize /
+// VelocitySize and by tile * 0.4; then _vector's distance to the arrow
+// (shaft and two head strokes) and alpha = 1 - clamp(dist, 0, 1), rounded
+// to 8 bits like the dye image.  float32 throughout, in the shader's order.
+__device__ __forceinline__ float qmix(float a, float b, float t) {
+  return a * (1.0f - t) + b * t;
+}
+
+__device__ __forceinline__ float qlen(float x, float y) { return sqrtf(x * x + y * y); }
+
+__device__ __forceinline__ float qline(float px, float py, float p1x, float p1y, float p2x,
+                                       float p2y) {
+  const float cx = (p1x + p2x) * 0.5f, cy = (p1y + p2y) * 0.5f;
+  const float len = qlen(p2x - p1x, p2y - p1y);
+  const float dx = (p2x - p1x) / len, dy = (p2y - p1y) / len;
+  const float rx = px - cx, ry = py - cy;
+  const float d1 = fabsf(rx * dy + ry * (-dx));
+  const float d2 = fabsf(rx * dx + ry * dy) - 0.5f * len;
+  return fmaxf(d1, d2);
+}
+
+template <class S>
+__global__ void k_quiver_rgba8(const typename S::s2* __restrict__ u, int Wv, int Hv, int Ww,
+                               int Hw, float T, uint32_t* __restrict__ out) {
+  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  if (x >= Ww || y >= Hw) return;
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  const float cx = (floorf(px / T) + 0.5f) * T, cy = (floorf(py / T) + 0.5f) * T;
+  const float Vw = (float)Wv, Vh = (float)Hv, Fw = (float)Ww, Fh = (float)Hw;
+  const float fx = (1.0f - cx / Fw) * Vw, fy = (1.0f - cy / Fh) * Vh;
+  const int trx = (int)fminf(fmaxf(ceilf(fx), 0.0f), Vw - 1.0f);
+  const int tr_y = (int)fminf(fmaxf(ceilf(fy), 0.0f), Vh - 1.0f);
+  const int blx = (int)fminf(fmaxf(floorf(fx), 0.0f), Vw - 1.0f);
+  const int bly = (int)fminf(fmaxf(floorf(fy), 0.0f), Vh - 1.0f);
+  const float dx = fx - (float)blx, dy = fy - (float)bly;
+  const float2 lt = S::ld2(u, tr_y * Wv + blx), rt = S::ld2(u, tr_y * Wv + trx);
+  const float2 lb = S::ld2(u, bly * Wv + blx), rb = S::ld2(u, bly * Wv + trx);
+  const float h1x = qmix(lt.x, rt.x, dx), h1y = qmix(lt.y, rt.y, 0.0f);
+  const float h2x = qmix(lb.x, rb.x, dx), h2y = qmix(lb.y, rb.y, 0.0f);
+  const float mx = qmix(h2x, h1x, dy), my = qmix(h2y, h1y, 0.0f);
+  float vx = (((-1.0f * mx) * (Fw / Vw)) * T) * 0.4f;
+  float vy = (((-1.0f * my) * (Fh / Vh)) * T) * 0.4f;
+  const float rx = px - cx, ry = py - cy;
+  float dist = 1.0f;
+  const float mag = qlen(vx, vy);
+  if (mag > 0.001f) {
+    const float ux = vx / mag, uy = vy / mag;
+    const float mc = fminf(fmaxf(mag, 0.0f), T * 0.5f);
+    vx = ux * mc;
+    vy = uy * mc;
+    const float shaft = qline(rx, ry, vx, vy, -vx, -vy);
+    const float head = fminf(qline(rx, ry, vx, vy, 0.4f * vx + 0.2f * (-vy), 0.4f * vy + 0.2f * vx),
+                             qline(rx, ry, vx, vy, 0.4f * vx + 0.2f * vy, 0.4f * vy + 0.2f * (-vx)));
+    dist = fminf(shaft, head);
+  }
+  const float alpha = 1.0f - fminf(fmaxf(dist, 0.0f), 1.0f);
+  const uint32_t a8 = (uint32_t)floorf(alpha * 255.0f + 0.5f);
+  out[y * Ww + x] = 0x00FFFFFFu | (a8 << 24);
+}
+
 // Energy line search along a correction d.  With L the 5-point operator of
 // the Jacobi sweep (a solid neighbour reads the centre value, an edge
 // neighbour is the cell itself), -L is symmetric positive semi-definite on
@@ -1408,6 +1474,25 @@ int nx_dye_rgba8(nx_sim* s, void* out) {
   else
     k_dye_rgba8<F16S><<<blocks, 256, 0, s->stream>>>(
         (const F16S::s1*)s->dye[s->dcur], (uint32_t*)out, n);
+  NX_HIP(hipGetLastError());
+  return NX_OK;
+}
+
+int nx_quiver_rgba8(nx_sim* s, int window_w, int window_h, float arrow_tile, void* out) {
+  if (!s || !out) return fail(NX_ERR_ARG, "null argument");
+  if (window_w < 1 || window_h < 1 || !(arrow_tile > 0.0f) || !(arrow_tile < 1e30f))
+    return fail(NX_ERR_ARG, "window size >= 1 and a finite arrow tile > 0 required");
+  const dim3 grid((window_w + 63) / 64, (window_h + 3) / 4);
+  const void* u = s->vel[s->vcur];
+  if (s->storage == ST_F32)
+    k_quiver_rgba8<F32S><<<grid, dim3(64, 4), 0, s->stream>>>(
+        (const F32S::s2*)u, s->W, s->H, window_w, window_h, arrow_tile, (uint32_t*)out);
+  else if (s->storage == ST_BF16)
+    k_quiver_rgba8<BF16S><<<grid, dim3(64, 4), 0, s->stream>>>(
+        (const BF16S::s2*)u, s->W, s->H, window_w, window_h, arrow_tile, (uint32_t*)out);
+  else
+    k_quiver_rgba8<F16S><<<grid, dim3(64, 4), 0, s->stream>>>(
+        (const F16S::s2*)u, s->W, s->H, window_w, window_h, arrow_tile, (uint32_t*)out);
   NX_HIP(hipGetLastError());
   return NX_OK;
 }

@@ -326,6 +326,79 @@ def dye_rgba8(q):
     return np.repeat(c[..., None], 4, axis=-1)
 
 
+def quiver_rgba8(u, window_w, window_h, tile):
+    """demo/shaders/demo.QuiverFragmentShader.frag:14-70 for every pixel of
+    a window_w x window_h framebuffer (row y holds gl_FragCoord.y = y + 0.5),
+    in float32 in the shader's order: the velocity at the pixel's arrow-tile
+    centre (_field: fPos = (1 - pos / WindowSize) * VelocitySize, corners
+    clamped to the grid, mix(a, b, (t, 0)), which interpolates only the x
+    component, scaled by -WindowSize / VelocitySize), times tile * 0.4, then
+    the distance to that arrow (_vector: shaft and two head strokes, each the
+    max of the distances across and along a segment) and colour (1, 1, 1,
+    1 - clamp(dist, 0, 1)) stored to an rgba8 image (round to nearest, ties
+    up).  u: [Hv, Wv, 2] float32.  Returns [window_h, window_w, 4] uint8."""
+    f = np.float32
+    Hv, Wv = u.shape[:2]
+    T = f(tile)
+    X = np.arange(window_w, dtype=f)[None, :] + f(0.5)
+    Y = np.arange(window_h, dtype=f)[:, None] + f(0.5)
+    X, Y = np.broadcast_arrays(X, Y)
+
+    def centre(p):
+        return (np.floor(p / T) + f(0.5)) * T
+
+    def mix(a, b, t):
+        return a * (f(1.0) - t) + b * t
+
+    cx, cy = centre(X), centre(Y)
+    Vw, Vh, Ww, Wh = f(Wv), f(Hv), f(window_w), f(window_h)
+    fx = (f(1.0) - cx / Ww) * Vw
+    fy = (f(1.0) - cy / Wh) * Vh
+    trx = np.clip(np.ceil(fx), f(0), f(Vw - f(1))).astype(np.int64)
+    tr_y = np.clip(np.ceil(fy), f(0), f(Vh - f(1))).astype(np.int64)
+    blx = np.clip(np.floor(fx), f(0), f(Vw - f(1))).astype(np.int64)
+    bly = np.clip(np.floor(fy), f(0), f(Vh - f(1))).astype(np.int64)
+    dx = fx - blx.astype(f)
+    dy = fy - bly.astype(f)
+    lt, rt = u[tr_y, blx], u[tr_y, trx]
+    lb, rb = u[bly, blx], u[bly, trx]
+    z = f(0.0)
+    h1 = (mix(lt[..., 0], rt[..., 0], dx), mix(lt[..., 1], rt[..., 1], z))
+    h2 = (mix(lb[..., 0], rb[..., 0], dx), mix(lb[..., 1], rb[..., 1], z))
+    m = (mix(h2[0], h1[0], dy), mix(h2[1], h1[1], z))
+    vx = ((f(-1.0) * m[0]) * (Ww / Vw)) * T * f(0.4)
+    vy = ((f(-1.0) * m[1]) * (Wh / Vh)) * T * f(0.4)
+    px, py = X - cx, Y - cy
+
+    def length(ax, ay):
+        return np.sqrt(ax * ax + ay * ay)
+
+    def line(p1x, p1y, p2x, p2y):
+        ccx, ccy = (p1x + p2x) * f(0.5), (p1y + p2y) * f(0.5)
+        ln = length(p2x - p1x, p2y - p1y)
+        dirx, diry = (p2x - p1x) / ln, (p2y - p1y) / ln
+        rx, ry = px - ccx, py - ccy
+        d1 = np.abs(rx * diry + ry * (-dirx))
+        d2 = np.abs(rx * dirx + ry * diry) - f(0.5) * ln
+        return np.maximum(d1, d2)
+
+    mag = length(vx, vy)
+    arrow = mag > f(0.001)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ux, uy = vx / mag, vy / mag
+        mc = np.clip(mag, f(0.0), T * f(0.5))
+        ax, ay = ux * mc, uy * mc
+        shaft = line(ax, ay, -ax, -ay)
+        head = np.minimum(line(ax, ay, f(0.4) * ax + f(0.2) * (-ay), f(0.4) * ay + f(0.2) * ax),
+                          line(ax, ay, f(0.4) * ax + f(0.2) * ay, f(0.4) * ay + f(0.2) * (-ax)))
+    dist = np.where(arrow, np.minimum(shaft, head), f(1.0))
+    alpha = f(1.0) - np.minimum(np.maximum(dist, f(0.0)), f(1.0))
+    a8 = np.floor(alpha * f(255.0) + f(0.5)).astype(np.uint8)
+    out = np.full((window_h, window_w, 4), 255, np.uint8)
+    out[..., 3] = a8
+    return out
+
+
 def add_dye(q, position, value, radius):
     """F4: cone dye splat, clamp [0,255] inside the disk
     (demo/shaders/shader.AddParticle.comp:24-34)."""

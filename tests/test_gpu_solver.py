@@ -126,6 +126,45 @@ def test_rollout_10_steps_bitwise(gpu, fused, bc_mode, nu):
     sim.destroy()
 
 
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("vel,dye", [((60, 80), (120, 160)), ((37, 50), (97, 131))])
+def test_rollout_with_dye_at_its_own_resolution(gpu, fused, vel, dye):
+    """The demo's use of the simulator: velocity at half the window
+    (V = window / 2), dye at the framebuffer (P), including a ratio that is
+    not an integer; velocity cone splats on V, dye cone splats on P, a
+    dynamic obstacle every step; 10 update() steps bitwise equal to the
+    oracle (simulation_demo.py:100-109, 220-237)."""
+    H, W = vel
+    Hp, Wp = dye
+    u, m = _scene(H, W, 5)
+    sim = make_sim(H, W, dye=(Wp, Hp), bc_mode=1, fused=fused)
+    sim.vorticity = 0.3
+    sim.iterations = 20
+    sim.upload("velocity", u)
+    sim.set_obstacle_map(m)
+    q = np.zeros((Hp, Wp), f32)
+    prm = F.Params(vorticity=0.3, iterations=20, bc_mode=1)
+    dt = np.float32(1 / 60)
+    pers = m != 0
+    obs = np.zeros((H, W, 2), f32)
+    p = np.zeros((H, W), f32)
+    for step in range(10):
+        pos = (0.2 + 0.04 * step, 0.55)
+        sim.add_velocity(pos, (0.8, -0.2), 6.0)
+        u = F.add_velocity(u, tuple(np.float32(v) for v in pos),
+                           (np.float32(0.8), np.float32(-0.2)), np.float32(6.0))
+        sim.add_dye(pos, 9.0, 120.0)
+        q = F.add_dye(q, tuple(np.float32(v) for v in pos), np.float32(120.0), np.float32(9.0))
+        sim.add_circle_obstacle((0.7, 0.35), 4.0)
+        obs = F.add_circle_obstacle(obs, (0.7, 0.35), 4.0)
+        sim.update(dt)
+        u, p, obs, q = F.step(u, p, obs, dt, prm, dye=q, persistent=pers)
+    assert_bitwise(sim.download("velocity"), u, "rollout velocity")
+    assert_bitwise(sim.download("dye"), q, "rollout dye")
+    assert np.abs(q).max() > 1.0  # the dye moved and accumulated
+    sim.destroy()
+
+
 def test_graph_replay_matches_eager(gpu):
     H, W = 80, 112
     u, m = _scene(H, W, 33)

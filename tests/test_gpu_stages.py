@@ -239,3 +239,30 @@ def test_dye_rgba8_export(gpu, rng, storage):
     ref = F.dye_rgba8(stored)
     assert np.array_equal(sim.dye_rgba8(), ref)
     sim.destroy()
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("vel,window,tile", [((37, 50), (148, 200), 16.0),
+                                             ((60, 80), (120, 160), 12.0),
+                                             ((64, 64), (97, 131), 20.0)])
+def test_quiver_rgba8(gpu, rng, storage, vel, window, tile):
+    """The demo's velocity arrows (demo.QuiverFragmentShader.frag): bitwise
+    equal to the oracle's statement of the stored velocity, for grids whose
+    window scale is and is not an integer and partial tiles at the edges."""
+    from natrix_amd import FluidSimulator
+    H, W = vel
+    Hw, Ww = window
+    sim = FluidSimulator(W, H, storage=storage)
+    u = rng.uniform(-0.3, 0.3, (H, W, 2)).astype(np.float32)
+    u[: H // 3] *= np.float32(30.0)  # arrows long enough to be clamped
+    u[H // 2, :] = 0.0               # and tiles with no arrow
+    sim.upload("velocity", u)
+    stored = sim.download("velocity").astype(np.float32)
+    img = sim.quiver_rgba8(Ww, Hw, tile)
+    assert img.shape == (Hw, Ww, 4)
+    assert np.array_equal(img, F.quiver_rgba8(stored, Ww, Hw, tile))
+    assert (img[..., 3] > 0).any() and (img[..., 3] == 0).any()
+    for bad in ((0, Hw, tile), (Ww, Hw, 0.0), (Ww, Hw, float("inf"))):
+        with pytest.raises(ValueError):
+            sim.quiver_rgba8(*bad)
+    sim.destroy()

@@ -460,3 +460,47 @@ def test_dye_rgba8_unorm_conversion():
     # every representable 8-bit level round-trips
     lv = (np.arange(256, dtype=np.float32) / np.float32(255.0))
     assert np.array_equal(F.dye_rgba8(lv)[..., 0], np.arange(256, dtype=np.uint8))
+
+
+def test_quiver_statement_closed_forms():
+    """The quiver statement (demo.QuiverFragmentShader.frag) on an 8 x 8
+    velocity grid under a 128 x 128 window with 16-pixel tiles: tile centres
+    sit at 8, 24, .., 120, so tile (i, j) samples fPos = (7.5 - i, 7.5 - j)
+    (the window flip), and the arrow of velocity u is v = -u * 16 * 16 *
+    0.4 pixels (the -WindowSize / VelocitySize scale), clamped to 8."""
+    f = np.float32
+    u = np.zeros((8, 8, 2), f)
+    img = F.quiver_rgba8(u, 128, 128, 16.0)
+    assert img.shape == (128, 128, 4)
+    assert (img[..., :3] == 255).all() and (img[..., 3] == 0).all()  # no arrow
+
+    # uniform +x flow: v = (-5.12, 0), tip (with the head) on the -x side.
+    # Pixel offsets from the tile centre are odd multiples of 0.5.
+    u[..., 0] = f(0.05)
+    a = F.quiver_rgba8(u, 128, 128, 16.0)[..., 3].reshape(8, 16, 8, 16)
+    # rows 0.5 from the axis on the tail side: shaft distance max(0.5,
+    # |rx| - 5.12) = 0.5 -> alpha 0.5 -> 128 for rx = 0.5 .. 4.5
+    assert (a[:, 7:9, :, 8:13] == 128).all()
+    # rows 1.5 from the axis: the shaft is 1.5 away, only the head strokes
+    # (from the tip to (-2.048, +-1.024)) can ink them, on the tip's side
+    assert (a[:, [6, 9], :, 8:] == 0).all()
+    assert (a[:, [6, 9], :, :8] > 0).any()
+
+    # the y component is the bottom-left sample's (mix with (t, 0)): rows of
+    # the grid alternate 0 and 5, tile row i samples grid row 7 - i exactly
+    # at fy = 7.5 - i, so only tile rows with 7 - i odd carry arrows
+    u[:] = 0
+    u[1::2, :, 1] = f(5.0)
+    a = F.quiver_rgba8(u, 128, 128, 16.0)[..., 3].reshape(8, 16, 8, 16)
+    inked = a.reshape(8, -1).max(axis=1) > 0
+    assert list(inked) == [(7 - i) % 2 == 1 for i in range(8)]
+
+    # magnitudes beyond the clamp (tile / 2) give the same arrow, whose shaft
+    # then spans the tile: alpha 128 at 0.5 from the axis, |rx| <= 7.5
+    u[:] = 0
+    u[..., 0] = f(50.0)
+    big = F.quiver_rgba8(u, 128, 128, 16.0)
+    u[..., 0] = f(500.0)
+    assert np.array_equal(big, F.quiver_rgba8(u, 128, 128, 16.0))
+    a = big[..., 3].reshape(8, 16, 8, 16)
+    assert (a[:, 7:9, :, 9:] == 128).all()
