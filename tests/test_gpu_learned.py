@@ -586,3 +586,31 @@ def test_padded_coarse_grid_correction_matches_the_pytorch_statement(gpu):
     assert a_s[0] > 0
     assert np.linalg.norm(d_n - d_s) <= 1e-2 * np.linalg.norm(d_s)
     assert abs(a_n[0] - a_s[0]) <= 1e-2 * abs(a_s[0])
+
+
+@pytest.mark.parametrize("W,H", [(8, 8), (40, 24), (130, 70), (1024, 64), (333, 517)])
+def test_learned_step_on_small_odd_and_narrow_grids(gpu, W, H):
+    """The learned step reaches FP64 relative residual <= 1e-4 on grids the
+    coarse-grid rule handles at its edges: the network's smallest grid
+    (8 x 8, f = 1), odd sizes padded to multiples of 8, a 16:1 aspect ratio
+    (a 128 x 8 coarse grid) and a portrait grid; the returned residual is
+    the FP64 residual the simulator reports for its pressure."""
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    g = LearnedGuess(net, meta.get("size", 128))
+    f, wc, hc = g.coarse_grid(W, H)
+    assert wc % 8 == 0 and hc % 8 == 0 and wc <= 128 and hc <= 128
+    assert -(-W // f) <= wc and -(-H // f) <= hc
+    sim = FluidSimulator(W, H, bc_mode="inflow")
+    sim.vorticity, sim.viscosity = 0.3, 0.0
+    X = np.arange(W)[None, :]
+    Y = np.arange(H)[:, None]
+    r = max(1.0, 0.12 * min(W, H))
+    sim.set_obstacle_map((X - 0.35 * W) ** 2 + (Y - 0.5 * H) ** 2 <= r * r)
+    for _ in range(3):
+        sim.update(1 / 60)
+    for _ in range(2):
+        st = solve_step(sim, g, 1 / 60, tol=1e-4)
+        assert st["converged"] and st["residual"] <= 1e-4, st
+    sim.destroy()
