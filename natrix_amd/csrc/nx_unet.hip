@@ -84,35 +84,56 @@ __global__ void __launch_bounds__(256)
 
   const int K = 9 * Cin;
   if ((Cin & 15) == 0) {
+    // K steps of 16 channels of one tap, in order, unrolled by two so the
+    // fragments of the next step load while the MFMAs of the current one
+    // run.  Loads are unconditional: an out-of-image pixel reads x[0] and
+    // its fragment is zeroed by a mask; the prefetch past the last step
+    // re-reads that step (clamped) and is never used.
     const int cps = Cin >> 4;  // K steps per tap
     const int ks0 = part ? blockIdx.z * steps_per_split : 0;
     const int ks1 = part ? min(ks0 + steps_per_split, 9 * cps) : 9 * cps;
-    for (int tap = ks0 / cps; tap < 9 && tap * cps < ks1; ++tap) {
-      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-      long long src[MT];
-#pragma unroll
-      for (int i = 0; i < MT; ++i) {
-        const int yy = py[i] + dy, xx = px[i] + dx;
-        src[i] = (valid[i] && yy >= 0 && yy < H && xx >= 0 && xx < W)
-                     ? (pimg[i] + (long long)yy * W + xx) * Cin
-                     : -1;
+    uint4 aA[MT], bA[NT], aB[MT], bB[NT];
+    uint32_t mA[MT], mB[MT];
+#define NX_CONV_LOAD(A_, B_, M_, KS_)                                                  \
+  {                                                                                    \
+    const int ks_ = (KS_);                                                             \
+    const int tap_ = ks_ / cps;                                                        \
+    const int ci_ = (ks_ - tap_ * cps) * 16 + kb * 8;                                  \
+    const int dy_ = tap_ / 3 - 1, dx_ = tap_ % 3 - 1;                                  \
+    _Pragma("unroll") for (int i = 0; i < MT; ++i) {                                   \
+      const int yy = py[i] + dy_, xx = px[i] + dx_;                                    \
+      const bool ok = valid[i] && yy >= 0 && yy < H && xx >= 0 && xx < W;              \
+      const long long off = ok ? (pimg[i] + (long long)yy * W + xx) * Cin + ci_ : 0;   \
+      A_[i] = *reinterpret_cast<const uint4*>(x + off);                                \
+      M_[i] = ok ? 0xFFFFFFFFu : 0u;                                                   \
+    }                                                                                  \
+    _Pragma("unroll") for (int j = 0; j < NT; ++j) B_[j] =                             \
+        *reinterpret_cast<const uint4*>(w + ((long long)(n_base + 32 * j + r) * 9 + tap_) * Cin + ci_); \
+  }
+#define NX_CONV_MMA(A_, B_, M_)                                                        \
+  {                                                                                    \
+    _Pragma("unroll") for (int i = 0; i < MT; ++i) {                                   \
+      const bf16x8 af = __builtin_bit_cast(                                            \
+          bf16x8, make_uint4(A_[i].x & M_[i], A_[i].y & M_[i], A_[i].z & M_[i],        \
+                             A_[i].w & M_[i]));                                        \
+      _Pragma("unroll") for (int j = 0; j < NT; ++j) acc[i][j] =                       \
+          __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, __builtin_bit_cast(bf16x8, B_[j]), \
+                                                  acc[i][j], 0, 0, 0);                 \
+    }                                                                                  \
+  }
+    int ks = ks0;
+    if (ks < ks1) {
+      NX_CONV_LOAD(aA, bA, mA, ks);
+      for (; ks + 1 < ks1; ks += 2) {
+        NX_CONV_LOAD(aB, bB, mB, ks + 1);
+        NX_CONV_MMA(aA, bA, mA);
+        NX_CONV_LOAD(aA, bA, mA, min(ks + 2, ks1 - 1));
+        NX_CONV_MMA(aB, bB, mB);
       }
-      const int cb = max(ks0 - tap * cps, 0) * 16, ce = min(ks1 - tap * cps, cps) * 16;
-      for (int c0 = cb; c0 < ce; c0 += 16) {
-        const int ci = c0 + kb * 8;
-        bf16x8 a[MT], b[NT];
-#pragma unroll
-        for (int i = 0; i < MT; ++i) a[i] = src[i] >= 0 ? ld_frag(x + src[i] + ci) : zero_frag();
-#pragma unroll
-        for (int j = 0; j < NT; ++j)
-          b[j] = ld_frag(w + ((long long)(n_base + 32 * j + r) * 9 + tap) * Cin + ci);
-#pragma unroll
-        for (int i = 0; i < MT; ++i)
-#pragma unroll
-          for (int j = 0; j < NT; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-      }
+      if (ks < ks1) NX_CONV_MMA(aA, bA, mA);
     }
+#undef NX_CONV_LOAD
+#undef NX_CONV_MMA
   } else {
     // Cin a multiple of 8: one 8-wide fragment never crosses a tap
     for (int k0 = 0; k0 < K; k0 += 16) {
