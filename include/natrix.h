@@ -367,7 +367,10 @@ int nx_head_bwd(const void* x, const float* w, const float* dout, void* dx, floa
  *      forward order (e0 a, e0 b, e1 a, e1 b, e2 a, e2 b, e3 a, e3 b, d2 a,
  *      d2 b, d1 a, d1 b, d0 a, d0 b) as bfloat16 KRSC weights and float32
  *      biases, head_w[base] / head_b[1] the 1x1 head, all device pointers
- *      the caller keeps alive; the runner owns its activation buffers. ---- */
+ *      the caller keeps alive (biases and head are read at every forward);
+ *      the runner copies the convolution weights at creation (tiled for
+ *      coalesced loads where cin % 16 == 0) and owns its activation
+ *      buffers. ---- */
 typedef struct nx_unet nx_unet;
 int nx_unet_create(int H, int W, int base, const void* const* conv_w,
                    const float* const* conv_b, const float* head_w, const float* head_b,

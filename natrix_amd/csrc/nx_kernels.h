@@ -116,6 +116,13 @@ hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const f
                           float* rout, float* dout, int W, int H, int setup, float theta,
                           const ChebCoef& cf, double* red);
 
+// U-Net convolutions on weights tiled as wt[tap][ci / 16][co][16] (one
+// contiguous block per K step and output-channel group); the same results
+// as nx_conv3x3_fwd / nx_conv3x3_fwd_split with n = 1.
+int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t s);
+int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
+                      int splits, int h, int wd, int cin, int cout, int relu, hipStream_t s);
+
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
 // sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 8
 // Chebyshev iterations run banded over several workgroups and need

@@ -37,6 +37,7 @@ struct nx_unet {
   // 14 convolutions in forward order: e0 a/b, e1 a/b, e2 a/b, e3 a/b,
   // d2 a/b, d1 a/b, d0 a/b
   const void* w[14] = {};
+  void* wt[14] = {};  // tiled copies (cin % 16 == 0), owned by the runner
   const float* b[14] = {};
   int cin[14] = {}, cout[14] = {};
   const float* head_w = nullptr;
@@ -167,6 +168,9 @@ __global__ void __launch_bounds__(1024)
 int conv(nx_unet* u, int i, const void* x, void* y, int h, int w, hipStream_t s) {
   const long long m = (long long)h * w;
   const int sp = conv_splits(m, u->cin[i], u->cout[i]);
+  if (u->wt[i])
+    return nx::conv3x3_fwd_tiled(x, u->wt[i], u->b[i], y, u->work, sp, h, w, u->cin[i],
+                                 u->cout[i], 1, s);
   if (sp > 1)
     return nx_conv3x3_fwd_split(x, u->w[i], u->b[i], y, u->work, sp, 1, h, w, u->cin[i],
                                 u->cout[i], 1, s);
@@ -340,9 +344,38 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
   u->work = (float*)alloc((u->work_elems ? u->work_elems : 1) * sizeof(float));
   u->scal = (float*)alloc(4 * sizeof(float));
   u->cws = (float*)alloc(6 * P0 * sizeof(float));
+  void* wcopy[14] = {};
+  for (int i = 0; i < 14 && ok; ++i) {
+    if (ci[i] % 16 == 0)
+      u->wt[i] = alloc((size_t)9 * ci[i] * co[i] * B);
+    else
+      wcopy[i] = alloc((size_t)9 * ci[i] * co[i] * B);
+  }
   if (!ok) {
     nx_unet_destroy(u);
     return fail(NX_ERR_HIP, "U-Net workspace allocation failed");
+  }
+  // the runner's copies of the weights, made once on the null stream:
+  // tiled where the convolution takes them, as given otherwise
+  for (int i = 0; i < 14; ++i) {
+    int rc = NX_OK;
+    if (u->wt[i]) {
+      rc = nx::conv3x3_tile_weights(u->w[i], u->wt[i], ci[i], co[i], nullptr);
+      u->w[i] = nullptr;  // the tiled copy replaces the caller's weights
+    } else {
+      if (hipMemcpyAsync(wcopy[i], u->w[i], (size_t)9 * ci[i] * co[i] * B,
+                         hipMemcpyDeviceToDevice, nullptr) != hipSuccess)
+        rc = fail(NX_ERR_HIP, "U-Net weight copy failed");
+      u->w[i] = wcopy[i];
+    }
+    if (rc != NX_OK) {
+      nx_unet_destroy(u);
+      return rc;
+    }
+  }
+  if (hipStreamSynchronize(nullptr) != hipSuccess) {
+    nx_unet_destroy(u);
+    return fail(NX_ERR_HIP, "U-Net weight tiling failed");
   }
   *out = u;
   return NX_OK;

@@ -44,12 +44,16 @@ __device__ __forceinline__ float bf(uint16_t h) { return nx::bf16_to_f32(h); }
 __device__ __forceinline__ uint16_t tobf(float f) { return (uint16_t)nx::f32_to_bf16(f); }
 
 // ---------------------------------------------------------------- forward
+// WT (Cin % 16 == 0 only): the weights are tiled as wt[tap][ci / 16][co][16]
+// (k_tile_weights), so the B fragments of one K step, 8 channels of 32
+// output channels per half-wave, are one contiguous 1 KiB block instead of
+// 16-byte pieces of 32 rows 9 Cin apart; the arithmetic is the same.
 // grid: (ceil(M / (4*32*MT)), Cout / (32*NT), splits); block 256 = 4 waves
 // along M.  With part == nullptr (one split) the block runs the whole K loop
 // and writes bias + ReLU in bf16; otherwise (Cin % 16 == 0) split z runs
 // K steps [z * steps_per_split, ...) of 16 channels of one tap each and
 // writes its float32 partial sums to part[z][m][co] for k_conv_finish.
-template <int MT, int NT>
+template <int MT, int NT, bool WT>
 __global__ void __launch_bounds__(256)
     k_conv3x3(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
               const float* __restrict__ bias, uint16_t* __restrict__ y, int N,
@@ -107,8 +111,10 @@ __global__ void __launch_bounds__(256)
       A_[i] = *reinterpret_cast<const uint4*>(x + off);                                \
       M_[i] = ok ? 0xFFFFFFFFu : 0u;                                                   \
     }                                                                                  \
-    _Pragma("unroll") for (int j = 0; j < NT; ++j) B_[j] =                             \
-        *reinterpret_cast<const uint4*>(w + ((long long)(n_base + 32 * j + r) * 9 + tap_) * Cin + ci_); \
+    const int s_ = ks_ - tap_ * cps;                                                   \
+    _Pragma("unroll") for (int j = 0; j < NT; ++j) B_[j] = *reinterpret_cast<const uint4*>( \
+        WT ? w + ((long long)(tap_ * cps + s_) * Cout + n_base + 32 * j + r) * 16 + kb * 8 \
+           : w + ((long long)(n_base + 32 * j + r) * 9 + tap_) * Cin + ci_);            \
   }
 #define NX_CONV_MMA(A_, B_, M_)                                                        \
   {                                                                                    \
@@ -436,19 +442,30 @@ extern "C" {
 
 static int conv_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                     int h, int wd, int cin, int cout, int relu, float* part, int splits,
-                    hipStream_t s) {
+                    hipStream_t s, bool tiled = false) {
   const long long M = (long long)n * h * wd;
   const int steps = 9 * cin / 16;
   const int per = part ? (steps + splits - 1) / splits : 0;
   const int used = part ? (steps + per - 1) / per : 1;
+  const uint16_t* xs = (const uint16_t*)x;
+  const uint16_t* ws = (const uint16_t*)w;
+  uint16_t* ys = (uint16_t*)y;
   if (cout % 64 == 0) {
     dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64, used);
-    k_conv3x3<2, 2><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)w, bias,
-                                          (uint16_t*)y, n, h, wd, cin, cout, relu, part, per);
+    if (tiled)
+      k_conv3x3<2, 2, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
+                                                 part, per);
+    else
+      k_conv3x3<2, 2, false><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
+                                                  part, per);
   } else {
     dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 32, used);
-    k_conv3x3<2, 1><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)w, bias,
-                                          (uint16_t*)y, n, h, wd, cin, cout, relu, part, per);
+    if (tiled)
+      k_conv3x3<2, 1, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
+                                                 part, per);
+    else
+      k_conv3x3<2, 1, false><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
+                                                  part, per);
   }
   if (part) {
     const long long mc = M * cout;
@@ -456,6 +473,19 @@ static int conv_fwd(const void* x, const void* w, const float* bias, void* y, in
                                                                cout, used, relu);
   }
   return 0;
+}
+
+// w[co][tap][ci] (KRSC) -> wt[tap][ci / 16][co][16] for the WT convolution
+__global__ void k_tile_weights(const uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
+                               int cin, int cout) {
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= 9LL * cin * cout) return;
+  const int e = (int)(i % 16);
+  long long t = i / 16;
+  const int co = (int)(t % cout);
+  t /= cout;
+  const int s = (int)(t % (cin / 16)), tap = (int)(t / (cin / 16));
+  wt[i] = w[((long long)co * 9 + tap) * cin + 16 * s + e];
 }
 
 int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int n,
@@ -578,3 +608,26 @@ int nx_head_bwd(const void* x, const float* w, const float* dout, void* dx, floa
 }
 
 }  // extern "C"
+
+namespace nx {
+// the native U-Net runner's convolutions on tiled weights (k_tile_weights);
+// same arithmetic, results and arguments as nx_conv3x3_fwd(_split) at n = 1
+int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t s) {
+  if (!w || !wt || cin % 16 || cout % 32)
+    return set_error(NX_ERR_ARG, "weight tiling: cin % 16 == 0, cout % 32 == 0");
+  const long long n = 9LL * cin * cout;
+  k_tile_weights<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((const uint16_t*)w,
+                                                             (uint16_t*)wt, cin, cout);
+  NX_UNET_RET();
+}
+
+int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
+                      int splits, int h, int wd, int cin, int cout, int relu, hipStream_t s) {
+  if (!x || !wt || !y || h < 1 || wd < 1 || cin % 16 || cout % 32 || splits < 1 ||
+      splits > 9 * cin / 16 || (splits > 1 && !work))
+    return set_error(NX_ERR_ARG, "tiled conv3x3: cin % 16 == 0, cout % 32 == 0, valid splits");
+  conv_fwd(x, wt, bias, y, 1, h, wd, cin, cout, relu, splits > 1 ? work : nullptr, splits, s,
+           true);
+  NX_UNET_RET();
+}
+}  // namespace nx
