@@ -71,6 +71,7 @@ struct nx_sim {
   double* sums = nullptr;  // 8 doubles
   int* state = nullptr;    // 4 ints
   int* host_state = nullptr;  // pinned, 2 slots x 4 ints
+  double* host_out = nullptr;  // pinned, 8 doubles: per-call scalar read-backs
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
 
   nx_params prm;
@@ -1324,6 +1325,7 @@ int nx_create_ex(int width, int height, int storage, nx_sim** out) {
   ok &= hipMalloc(&s->sums, 8 * sizeof(double)) == hipSuccess;
   ok &= hipMalloc(&s->state, 4 * sizeof(int)) == hipSuccess;
   ok &= hipHostMalloc(&s->host_state, 8 * sizeof(int), 0) == hipSuccess;
+  ok &= hipHostMalloc(&s->host_out, 8 * sizeof(double), 0) == hipSuccess;
   ok &= hipEventCreateWithFlags(&s->poll_ev[0], hipEventDisableTiming) == hipSuccess;
   ok &= hipEventCreateWithFlags(&s->poll_ev[1], hipEventDisableTiming) == hipSuccess;
   if (!ok) return cleanup(fail(NX_ERR_HIP, "device allocation failed"));
@@ -1378,6 +1380,7 @@ int nx_destroy(nx_sim* s) {
   (void)hipFree(s->state);
   (void)hipFree(s->map_staging);
   if (s->host_state) (void)hipHostFree(s->host_state);
+  if (s->host_out) (void)hipHostFree(s->host_out);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return NX_OK;
@@ -1689,13 +1692,14 @@ int nx_smooth(nx_sim* s, int sweeps, float omega, double* out) {
                      s->stream));
   NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,
                        nullptr, 0, 0, s->stream));
-  double rel = 0.0, alpha = 0.0;
-  NX_HIP(hipMemcpyAsync(&rel, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
+  s->host_out[1] = 0.0;
+  NX_HIP(hipMemcpyAsync(s->host_out, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
                         s->stream));
   if (s->line)
-    NX_HIP(hipMemcpyAsync(&alpha, s->line, sizeof(double), hipMemcpyDeviceToHost,
+    NX_HIP(hipMemcpyAsync(s->host_out + 1, s->line, sizeof(double), hipMemcpyDeviceToHost,
                           s->stream));
   NX_HIP(hipStreamSynchronize(s->stream));
+  const double rel = s->host_out[0], alpha = s->host_out[1];
   s->stats.sweeps = s->step_sweeps;
   s->stats.residual = rel;
   s->stats.residual_sweeps = s->step_sweeps;
@@ -1752,13 +1756,14 @@ int nx_smooth_chebyshev(nx_sim* s, int iterations, float lambda_min, const int* 
   s->step_sweeps += iterations;
   NX_HIP(launch_reduce(s->cheb_red, last_blocks, 5, 0, s->sums, -1.0, nullptr, 0, 0,
                        s->stream));
-  double rel = 0.0, alpha = 0.0;
-  NX_HIP(hipMemcpyAsync(&rel, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
+  s->host_out[1] = 0.0;
+  NX_HIP(hipMemcpyAsync(s->host_out, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
                         s->stream));
   if (s->line)
-    NX_HIP(hipMemcpyAsync(&alpha, s->line, sizeof(double), hipMemcpyDeviceToHost,
+    NX_HIP(hipMemcpyAsync(s->host_out + 1, s->line, sizeof(double), hipMemcpyDeviceToHost,
                           s->stream));
   NX_HIP(hipStreamSynchronize(s->stream));
+  const double rel = s->host_out[0], alpha = s->host_out[1];
   s->stats.sweeps = s->step_sweeps;
   s->stats.residual = rel;
   s->stats.residual_sweeps = s->step_sweeps;
@@ -1937,11 +1942,11 @@ int nx_refine_residual(nx_sim* s, double* rel) {
   NX_HIP(hipGetLastError());
   NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,
                        nullptr, 0, 0, s->stream));
-  double sums[6];
-  NX_HIP(hipMemcpyAsync(sums, s->sums, sizeof(sums), hipMemcpyDeviceToHost, s->stream));
+  NX_HIP(hipMemcpyAsync(s->host_out, s->sums, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
   NX_HIP(hipStreamSynchronize(s->stream));
   s->pcur = 0;
-  *rel = sums[5];
+  *rel = s->host_out[5];
   return NX_OK;
 }
 
