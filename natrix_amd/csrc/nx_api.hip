@@ -303,11 +303,10 @@ static int exact_residual(nx_sim* s, double* rel) {
                      s->red5, s->stream));
   NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums,
                        -1.0, nullptr, 0, 0, s->stream));
-  double sums[6];
-  NX_HIP(hipMemcpyAsync(sums, s->sums, sizeof(sums), hipMemcpyDeviceToHost,
+  NX_HIP(hipMemcpyAsync(s->host_out, s->sums, 6 * sizeof(double), hipMemcpyDeviceToHost,
                         s->stream));
   NX_HIP(hipStreamSynchronize(s->stream));
-  *rel = sums[5];
+  *rel = s->host_out[5];
   return NX_OK;
 }
 
