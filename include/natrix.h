@@ -368,9 +368,10 @@ int nx_head_bwd(const void* x, const float* w, const float* dout, void* dx, floa
  *      d2 b, d1 a, d1 b, d0 a, d0 b) as bfloat16 KRSC weights and float32
  *      biases, head_w[base] / head_b[1] the 1x1 head, all device pointers
  *      the caller keeps alive (biases and head are read at every forward);
- *      the runner copies the convolution weights at creation (tiled for
- *      coalesced loads where cin % 16 == 0) and owns its activation
- *      buffers. ---- */
+ *      the runner copies the convolution weights during nx_unet_create (on
+ *      the null stream, which it synchronises; tiled for coalesced loads
+ *      where cin % 16 == 0), so they must be complete when it is called,
+ *      and owns its activation buffers. ---- */
 typedef struct nx_unet nx_unet;
 int nx_unet_create(int H, int W, int base, const void* const* conv_w,
                    const float* const* conv_b, const float* head_w, const float* head_b,

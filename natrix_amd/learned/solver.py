@@ -51,6 +51,9 @@ class NativeUNet:
         self._hw = net.hw.detach().float().contiguous()
         self._hb = net.hb.detach().float().contiguous()
         base = convs[0].w.shape[0]
+        # nx_unet_create copies the weights on the null stream: they must be
+        # written before it runs, whatever stream produced them
+        torch.cuda.current_stream().synchronize()
         h = C.c_void_p()
         wp = (C.c_void_p * 14)(*[t.data_ptr() for t in self._w])
         bp = (C.c_void_p * 14)(*[t.data_ptr() for t in self._b])
@@ -134,17 +137,18 @@ class LearnedGuess:
         simulator's stream), scaled by the energy line search (or by 1
         without `line_search`).  With `report` returns (a, d.r, d.Ld)
         (synchronises); otherwise None."""
-        with torch.cuda.stream(torch.cuda.ExternalStream(sim.stream_handle())):
-            return self._correct(sim, report, line_search)
-
-    def _correct(self, sim, report, line_search):
         f, wc, hc = self.coarse_grid(sim.width, sim.height)
         runner = self._runner(wc, hc)
         if runner is not None and line_search:
+            # the native runner enqueues everything on the simulator's stream
             out = (C.c_double * 3)() if report else None
             _lib.call("nx_learned_correct", sim._alive(), runner.handle, f,
                       int(self.coarse_iters), float(self.lmin), out)
             return tuple(out) if report else None
+        with torch.cuda.stream(torch.cuda.ExternalStream(sim.stream_handle())):
+            return self._correct(sim, f, wc, hc, report, line_search)
+
+    def _correct(self, sim, f, wc, hc, report, line_search):
         if -(-sim.width // f) != wc or -(-sim.height // f) != hc:
             raise ValueError(f"a {sim.width} x {sim.height} grid needs a padded coarse grid "
                              f"({wc} x {hc} for f = {f}), which only the native runner takes")
