@@ -389,6 +389,22 @@ int nx_unet_forward(nx_unet* net, const void* x, float* out, void* stream);
  * stream and smoothing parameters and replayed from then on. */
 int nx_learned_correct(nx_sim* sim, nx_unet* net, int factor, int coarse_iters,
                        float lambda_min, double* out);
+/* the learned cycles of one correction solve, with one host call: up to
+ * max_cycles times nx_learned_correct (factor, coarse_iters, lambda_min)
+ * followed by smooth_iters fine iterations (nx_smooth_chebyshev on
+ * [fine_lambda_min, 8] with `tile`, or nx_smooth with smooth_iters - 1
+ * Jacobi sweeps and one damped sweep when fine_lambda_min <= 0), stopping
+ * when the FP64 relative residual is <= target (converged), when a line
+ * search takes step 0 or a cycle leaves more than stall x the previous
+ * cycle's residual (abandoned), or after max_cycles.  result[4] (host) =
+ * {cycles run, cycles whose step moved the pressure, converged, abandoned};
+ * records (host, may be NULL) [3 * max_cycles] = per cycle {line-search
+ * step, the step's sweeps so far, residual}.  One synchronisation per
+ * cycle.  No reference counterpart (the solver loop of the learned step). */
+int nx_learned_cycles(nx_sim* sim, nx_unet* net, int factor, int coarse_iters,
+                      float lambda_min, int smooth_iters, float fine_lambda_min,
+                      const int* tile, double target, double stall, int max_cycles,
+                      double* records, int* result);
 /* the runner's coarse buffers (rhs, mask, correction), for inspection */
 int nx_unet_coarse(nx_unet* net, float** rc, uint8_t** mc, float** e);
 

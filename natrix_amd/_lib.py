@@ -157,6 +157,7 @@ SIGNATURES = {
     "nx_unet_destroy": [_P],
     "nx_unet_forward": [_P, _P, _P, _P],
     "nx_learned_correct": [_P, _P, _I, _I, _F, _P],
+    "nx_learned_cycles": [_P, _P, _I, _I, _F, _I, _F, _P, _D, _D, _I, _P, _P],
     "nx_unet_coarse": [_P, _P, _P, _P],
     "nx_get_size": [_P, _P, _P],
     "nx_conv3x3_dgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],

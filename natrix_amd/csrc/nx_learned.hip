@@ -414,6 +414,52 @@ int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
   return nx::prolong_line_search_padded(sim, factor, u->W, u->H, u->out, u->mc, out);
 }
 
+int nx_learned_cycles(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
+                      float lambda_min, int smooth_iters, float fine_lambda_min,
+                      const int* tile, double target, double stall, int max_cycles,
+                      double* records, int* result) {
+  if (!sim || !u || !result || factor < 1 || smooth_iters < 1 || max_cycles < 0 ||
+      !(stall > 0.0))
+    return fail(NX_ERR_ARG, "bad learned cycle arguments");
+  int n = 0, kept = 0, converged = 0, abandoned = 0;
+  double before = -1.0;
+  for (int c = 0; c < max_cycles; ++c) {
+    int rc = nx_learned_correct(sim, u, factor, coarse_iters, lambda_min, nullptr);
+    if (rc != NX_OK) return rc;
+    double out[2] = {0.0, 0.0};
+    rc = fine_lambda_min > 0.0f
+             ? nx_smooth_chebyshev(sim, smooth_iters, fine_lambda_min, tile, out)
+             : nx_smooth(sim, smooth_iters - 1, 0.5f, out);
+    if (rc != NX_OK) return rc;
+    ++n;
+    const double rel = out[0], a = out[1];
+    if (records) {
+      nx_stats st{};
+      rc = nx_get_stats(sim, &st);
+      if (rc != NX_OK) return rc;
+      records[3 * c] = a;
+      records[3 * c + 1] = (double)st.sweeps;
+      records[3 * c + 2] = rel;
+    }
+    if (rel <= target) {
+      converged = 1;
+      kept += a != 0.0;
+      break;
+    }
+    if (a == 0.0 || (before >= 0.0 && rel > stall * before)) {
+      abandoned = 1;
+      break;
+    }
+    ++kept;
+    before = rel;
+  }
+  result[0] = n;
+  result[1] = kept;
+  result[2] = converged;
+  result[3] = abandoned;
+  return NX_OK;
+}
+
 int nx_unet_coarse(nx_unet* u, float** rc, uint8_t** mc, float** e) {
   if (!u) return fail(NX_ERR_ARG, "null argument");
   if (rc) *rc = u->rc;

@@ -132,6 +132,12 @@ class LearnedGuess:
         return r
 
     @torch.no_grad()
+    def native(self, sim):
+        """(f, runner) when corrections of `sim` run natively, else None."""
+        f, wc, hc = self.coarse_grid(sim.width, sim.height)
+        runner = self._runner(wc, hc)
+        return None if runner is None else (f, runner)
+
     def correct(self, sim, report=False, line_search=True):
         """Add one learned correction to the simulator's pressure (on the
         simulator's stream), scaled by the energy line search (or by 1
@@ -210,8 +216,26 @@ def _smooth(sim, target, sweeps, early_exit=False, lmin=0.0, tile=None):
             "residual": out[0], "alpha": out[1]}
 
 
+def _native_cycles(sim, guess, nat, target, smooth, max_cycles, stall, trace, lmin, tile):
+    """The learned cycles in one nx_learned_cycles call (the same GPU work
+    and decisions as the loop in _inner).  Returns (kept, converged,
+    abandoned, residual after the last cycle or None)."""
+    f, runner = nat
+    rec = (C.c_double * (3 * max(1, max_cycles)))()
+    res = (C.c_int * 4)()
+    t = None if tile is None else (C.c_int * 4)(*(int(v) for v in tile))
+    _lib.call("nx_learned_cycles", sim._alive(), runner.handle, f, int(guess.coarse_iters),
+              float(guess.lmin), int(smooth), float(lmin), t, float(target), float(stall),
+              int(max_cycles), rec, res)
+    n, kept, conv, abandoned = res[0], res[1], bool(res[2]), bool(res[3])
+    if trace is not None:
+        for c in range(n):
+            trace.append(("cycle", round(rec[3 * c], 4), int(rec[3 * c + 1]), rec[3 * c + 2]))
+    return kept, conv, abandoned, (rec[3 * (n - 1) + 2] if n else None)
+
+
 def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
-           plain_stall=0.9, trace=None, lmin=0.0, tile=None):
+           plain_stall=0.9, trace=None, lmin=0.0, tile=None, native_loop=True):
     """Solve the correction equation (current right-hand side, iterate
     from its first sweep) to relative residual `target`: learned cycles
     while each reduces the residual below `stall` x the previous one, then
@@ -222,7 +246,13 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
     kept, abandoned = 0, False
     before = None
     start = sim.stats()["sweeps"]
-    for _ in range(max_cycles if guess is not None else 0):
+    nat = guess.native(sim) if (guess is not None and native_loop) else None
+    if nat is not None:
+        kept, conv, abandoned, before = _native_cycles(sim, guess, nat, target, smooth,
+                                                       max_cycles, stall, trace, lmin, tile)
+        if conv:
+            return kept, False
+    for _ in range(max_cycles if (guess is not None and nat is None) else 0):
         guess.correct(sim)
         st = _smooth(sim, target, smooth, lmin=lmin, tile=tile)
         a = st["alpha"]
@@ -254,7 +284,7 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
 
 def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=80, max_cycles=6,
                max_outer=12, chunk=65536, max_inner_sweeps=4_000_000, trace=None,
-               fine_lmin=None, fine_tile=None):
+               fine_lmin=None, fine_tile=None, native_loop=True):
     """One fluid step whose pressure is solved to FP64 relative residual
     <= tol by iterative refinement (pressure held as a float32 pair, outer
     residual in FP64, float32 correction solves); with `guess` (a
@@ -263,7 +293,9 @@ def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=80, max_cycles=6,
     is the correction followed by `smooth` fine smoothing iterations:
     Chebyshev on [fine_lmin, 8] (None: the guess's 1.28 / f^2; `fine_tile`
     the kernel's tile, None for the library's default) or, with
-    fine_lmin = 0, Jacobi sweeps ending in a damped one.  Returns the
+    fine_lmin = 0, Jacobi sweeps ending in a damped one; with a native
+    runner the cycles of a correction solve run in one nx_learned_cycles
+    call (native_loop=False: the same cycles from Python).  Returns the
     step's statistics: sweeps (fine sweeps and Chebyshev iterations), converged, residual
     (FP64 relative), outer (refinement iterations), learned_corrections
     (cycles that moved the pressure), fallback (learned cycles abandoned
@@ -294,7 +326,8 @@ def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=80, max_cycles=6,
                 break
             target = max(inner_tol, 0.5 * tol / rel.value)
             k, ab = _inner(sim, guess, target, smooth, max_cycles, chunk,
-                           max_inner_sweeps, trace=trace, lmin=lmin, tile=fine_tile)
+                           max_inner_sweeps, trace=trace, lmin=lmin, tile=fine_tile,
+                           native_loop=native_loop)
             kept += k
             fallback |= ab
             _lib.call("nx_refine_accumulate", sim._alive())

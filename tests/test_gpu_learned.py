@@ -614,3 +614,29 @@ def test_learned_step_on_small_odd_and_narrow_grids(gpu, W, H):
         st = solve_step(sim, g, 1 / 60, tol=1e-4)
         assert st["converged"] and st["residual"] <= 1e-4, st
     sim.destroy()
+
+
+@pytest.mark.parametrize("fine_lmin", [None, 0.0])
+def test_native_cycle_loop_equals_the_python_loop(gpu, fine_lmin):
+    """nx_learned_cycles enqueues the same corrections and smoothing calls
+    and takes the same decisions as the Python loop it replaces: the step's
+    pressure, statistics and per-cycle trace are identical, with Chebyshev
+    and with Jacobi fine smoothing."""
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    res = {}
+    for native_loop in (True, False):
+        sim = cylinder_sim(256, warm=4)
+        g = LearnedGuess(net, meta.get("size", 128))
+        tr = []
+        st = solve_step(sim, g, 1 / 60, tol=1e-4, trace=tr, fine_lmin=fine_lmin,
+                        native_loop=native_loop)
+        assert st["converged"] and st["residual"] <= 1e-4, st
+        res[native_loop] = (sim.download("pressure"), st, tr)
+        sim.destroy()
+    (p1, s1, t1), (p2, s2, t2) = res[True], res[False]
+    assert np.array_equal(p1, p2)
+    assert s1 == s2
+    assert t1 == t2
+    assert sum(1 for t in t1 if t[0] == "cycle") >= 2
