@@ -640,3 +640,35 @@ def test_native_cycle_loop_equals_the_python_loop(gpu, fine_lmin):
     assert s1 == s2
     assert t1 == t2
     assert sum(1 for t in t1 if t[0] == "cycle") >= 2
+
+
+def test_native_cycle_loop_arguments(gpu):
+    """nx_learned_cycles rejects bad arguments before enqueueing anything
+    and returns at once for zero cycles."""
+    from natrix_amd.learned.solver import LearnedGuess
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    sim = cylinder_sim(128, warm=2)
+    g = LearnedGuess(net, meta.get("size", 128))
+    f, runner = g.native(sim)
+    sim._push_params()
+    _lib.call("nx_step_begin", sim._alive(), 1 / 60)
+    res = (C.c_int * 4)()
+    rec = (C.c_double * 3)()
+    args = [f, 100, float(g.lmin), 80, 0.3, None, 0.1, 0.5, 1, rec, res]
+    # factor, smoothing iterations, stall factor, cycle count
+    for i, bad in ((0, 0), (3, 0), (7, 0.0), (8, -1)):
+        a = list(args)
+        a[i] = bad
+        with pytest.raises(ValueError):
+            _lib.call("nx_learned_cycles", sim._alive(), runner.handle, *a)
+    with pytest.raises(ValueError):
+        _lib.call("nx_learned_cycles", sim._alive(), runner.handle, *args[:-1], None)
+    before = sim.download("pressure")
+    a = list(args)
+    a[8] = 0
+    _lib.call("nx_learned_cycles", sim._alive(), runner.handle, *a)
+    assert list(res) == [0, 0, 0, 0]
+    assert np.array_equal(sim.download("pressure"), before)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
