@@ -1709,9 +1709,11 @@ int nx_smooth(nx_sim* s, int sweeps, float omega, double* out) {
   return NX_OK;
 }
 
-int nx_smooth_chebyshev(nx_sim* s, int iterations, float lambda_min, const int* tile,
-                        double* out) {
+static int smooth_cheb_impl(nx_sim* s, int iterations, float lambda_min, const int* tile,
+                            double* out, bool pending) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
+  if (pending && (!s->dcorr || !s->line))
+    return fail(NX_ERR_STATE, "no pending line-search step");
   if (!s->in_step) return fail(NX_ERR_STATE, "nx_smooth_chebyshev outside nx_step_begin/finish");
   if (iterations < 1 || !(lambda_min > 0.0f && lambda_min < 8.0f))
     return fail(NX_ERR_ARG, "iterations >= 1 and 0 < lambda_min < 8 required");
@@ -1746,7 +1748,9 @@ int nx_smooth_chebyshev(nx_sim* s, int iterations, float lambda_min, const int* 
                                    s->div, s->mask[s->mcur], s->p[1 - s->pcur],
                                    last ? nullptr : rd[1 - cur][0], rd[1 - cur][1], s->W,
                                    s->H, setup, rec.theta, cf,
-                                   last ? s->cheb_red : nullptr));
+                                   last ? s->cheb_red : nullptr,
+                                   pending && setup ? s->dcorr : nullptr,
+                                   pending && setup ? s->line : nullptr));
     if (last) last_blocks = jacobi_launch_blocks(s->W, s->H, cc);
     s->pcur ^= 1;
     cur ^= 1;
@@ -1771,6 +1775,11 @@ int nx_smooth_chebyshev(nx_sim* s, int iterations, float lambda_min, const int* 
     out[1] = alpha;
   }
   return NX_OK;
+}
+
+int nx_smooth_chebyshev(nx_sim* s, int iterations, float lambda_min, const int* tile,
+                        double* out) {
+  return smooth_cheb_impl(s, iterations, lambda_min, tile, out, false);
 }
 
 // restriction onto a Wp x Hp coarse grid, Wp >= ceil(W / f), Hp >= ceil(H / f)
@@ -1864,7 +1873,7 @@ static int ensure_line(nx_sim* s) {
   return NX_OK;
 }
 
-static int line_search(nx_sim* s, const float* d, double* out) {
+static int line_search(nx_sim* s, const float* d, double* out, bool apply = true) {
   const size_t n = (size_t)s->W * s->H;
   const dim3 grid((s->W + 63) / 64, (s->H + RES_ROWS - 1) / RES_ROWS);
   // red5 holds 5 doubles per block of this grid; the line terms use 2
@@ -1873,9 +1882,11 @@ static int line_search(nx_sim* s, const float* d, double* out) {
   NX_HIP(hipGetLastError());
   k_line_alpha<<<1, 1024, 0, s->stream>>>(s->red5, residual_blocks(s->W, s->H), s->line);
   NX_HIP(hipGetLastError());
-  k_line_apply<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(
-      s->p[s->pcur], d, s->mask[s->mcur], s->line, (int)n);
-  NX_HIP(hipGetLastError());
+  if (apply) {
+    k_line_apply<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(
+        s->p[s->pcur], d, s->mask[s->mcur], s->line, (int)n);
+    NX_HIP(hipGetLastError());
+  }
   if (out) {
     NX_HIP(hipMemcpyAsync(out, s->line, 3 * sizeof(double), hipMemcpyDeviceToHost,
                           s->stream));
@@ -1894,7 +1905,7 @@ int nx_line_search(nx_sim* s, const float* d, double* out) {
 // interpolation from a Wp x Hp coarse grid (padded cells must be solid) and
 // the energy line search
 static int prolong_ls_impl(nx_sim* s, int factor, int Wp, int Hp, const float* coarse,
-                           const uint8_t* coarse_mask, double* out) {
+                           const uint8_t* coarse_mask, double* out, bool apply = true) {
   if (!s || !coarse || !coarse_mask || factor < 1)
     return fail(NX_ERR_ARG, "bad prolongation arguments");
   if (!s->in_step) return fail(NX_ERR_STATE, "prolongation outside a step");
@@ -1905,7 +1916,7 @@ static int prolong_ls_impl(nx_sim* s, int factor, int Wp, int Hp, const float* c
   k_prolong<false><<<grid, dim3(64, 4), 0, s->stream>>>(
       s->dcorr, s->mask[s->mcur], coarse, coarse_mask, s->W, s->H, factor, Wp, Hp, 1.0f);
   NX_HIP(hipGetLastError());
-  return line_search(s, s->dcorr, out);
+  return line_search(s, s->dcorr, out, apply);
 }
 
 int nx_prolong_line_search(nx_sim* s, int factor, const float* coarse,
@@ -2290,7 +2301,11 @@ int restrict_residual_padded(nx_sim* s, int factor, int Wp, int Hp, float* rc, u
   return restrict_impl(s, factor, Wp, Hp, rc, mc);
 }
 int prolong_line_search_padded(nx_sim* s, int factor, int Wp, int Hp, const float* e,
-                               const uint8_t* mc, double* out) {
-  return prolong_ls_impl(s, factor, Wp, Hp, e, mc, out);
+                               const uint8_t* mc, double* out, bool apply) {
+  return prolong_ls_impl(s, factor, Wp, Hp, e, mc, out, apply);
+}
+int smooth_chebyshev_pending(nx_sim* s, int iterations, float lambda_min, const int* tile,
+                             double* out) {
+  return smooth_cheb_impl(s, iterations, lambda_min, tile, out, true);
 }
 }  // namespace nx

@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(WAVES * 64)
               const float* __restrict__ din, const float* __restrict__ b,
               const uint8_t* __restrict__ mask, float* __restrict__ xout,
               float* __restrict__ rout, float* __restrict__ dout, int W, int H, int k,
-              int first, float theta, ChebCoef cf, double* __restrict__ red) {
+              int first, float theta, ChebCoef cf, double* __restrict__ red,
+              const float* __restrict__ dapp, const double* __restrict__ line) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -101,6 +102,24 @@ __global__ void __launch_bounds__(WAVES * 64)
       d[i][c] = (in && !first) ? din[idx] : 0.0f;
       sol[i][c] = in ? (mask[idx] != 0) : 1;
       if (RES) bb[RES ? i : 0][RES ? c : 0] = in ? b[idx] : 0.0f;
+    }
+  }
+
+  // a pending line-search step x += a d on the fluid cells, as the apply
+  // of the line search would have stored it (first launch only)
+  if (dapp != nullptr) {
+    const float a = (float)line[0];
+    if (a != 0.0f) {
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i) {
+        const int gy = gy0 + i;
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) {
+          const int gx = gx0 + c;
+          if (gy >= 0 && gy < H && gx >= 0 && gx < W && !sol[i][c])
+            x[i][c] = x[i][c] + a * dapp[gy * W + gx];
+        }
+      }
     }
   }
 
@@ -269,15 +288,16 @@ template <int ROWS, int WAVES, int COLS>
 hipError_t launch_cheb(const float* xin, const float* rin, const float* din, const float* b,
                        const uint8_t* mask, float* xout, float* rout, float* dout, int W,
                        int H, int k, int first, float theta, const ChebCoef& cf, double* red,
-                       hipStream_t st) {
+                       const float* dapp, const double* line, hipStream_t st) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + EY - 2 * k - 1) / (EY - 2 * k));
   if (red)
     k_cheb_tb<ROWS, WAVES, COLS, true><<<grid, WAVES * 64, 0, st>>>(
-        xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf, red);
+        xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf, red, dapp, line);
   else
     k_cheb_tb<ROWS, WAVES, COLS, false><<<grid, WAVES * 64, 0, st>>>(
-        xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf, nullptr);
+        xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf, nullptr, dapp,
+        line);
   return hipGetLastError();
 }
 
@@ -303,12 +323,13 @@ bool cheb_cfg_supported(JacobiCfg c) {
 hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const float* rin,
                           const float* din, const float* b, const uint8_t* mask, float* xout,
                           float* rout, float* dout, int W, int H, int setup, float theta,
-                          const ChebCoef& cf, double* red) {
+                          const ChebCoef& cf, double* red, const float* dapp,
+                          const double* line) {
   if (!cheb_cfg_supported(c)) return hipErrorInvalidValue;
 #define NX_CHEB_LAUNCH(R, WV, CL)                                                     \
   if (c.rows == R && c.waves == WV && c.cols == CL)                                  \
     return launch_cheb<R, WV, CL>(xin, rin, din, b, mask, xout, rout, dout, W, H, c.k, \
-                                  setup, theta, cf, red, st);
+                                  setup, theta, cf, red, dapp, line, st);
   NX_CHEB_SHAPES(NX_CHEB_LAUNCH)
 #undef NX_CHEB_LAUNCH
   return hipErrorInvalidValue;

@@ -86,7 +86,11 @@ bool jacobi_cfg_supported(JacobiCfg cfg);
 // (padded coarse cells are solid, with a zero right-hand side).
 int restrict_residual_padded(::nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc);
 int prolong_line_search_padded(::nx_sim* s, int factor, int Wp, int Hp, const float* e,
-                               const uint8_t* mc, double* out);
+                               const uint8_t* mc, double* out, bool apply = true);
+// nx_smooth_chebyshev taking first the line-search step that a
+// prolong_line_search_padded(..., apply = false) left pending
+int smooth_chebyshev_pending(::nx_sim* s, int iterations, float lambda_min, const int* tile,
+                             double* out);
 // Chebyshev smoothing of the fine system L x = b (mask != 0: solid) on the
 // interval [lambda_min, 8] of -L.  One launch runs c.k stencil passes on
 // a (64 c.cols) x (c.rows c.waves) register tile per block: with setup != 0
@@ -96,7 +100,9 @@ int prolong_line_search_padded(::nx_sim* s, int factor, int Wp, int Hp, const fl
 // red, the last pass instead evaluates the exact FP64 residual b - L x of
 // the final iterate on the written interior and each block writes
 // red[5 blk + {sum r, sum r^2, sum b, sum b^2, fluid cells}] (blocks:
-// jacobi_launch_blocks of the launch's shape and passes).
+// jacobi_launch_blocks of the launch's shape and passes).  With dapp (first
+// launch), the loaded x first takes a pending line-search step: x += a dapp
+// on fluid cells, a = (float)line[0], nothing when a == 0.
 constexpr int CHEB_MAX_PASSES = 32;
 struct ChebCoef {
   float cd[CHEB_MAX_PASSES];
@@ -114,7 +120,8 @@ bool cheb_cfg_supported(JacobiCfg c);
 hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const float* rin,
                           const float* din, const float* b, const uint8_t* mask, float* xout,
                           float* rout, float* dout, int W, int H, int setup, float theta,
-                          const ChebCoef& cf, double* red);
+                          const ChebCoef& cf, double* red, const float* dapp = nullptr,
+                          const double* line = nullptr);
 
 // U-Net convolutions on weights tiled as wt[tap][ci / 16][co][16] (one
 // contiguous block per K step and output-channel group); the same results

@@ -394,8 +394,12 @@ int nx_unet_forward(nx_unet* u, const void* x, float* out, void* stream) {
   return forward(u, (const uint16_t*)x, out, (hipStream_t)stream);
 }
 
-int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
-                       float lambda_min, double* out) {
+}  // extern "C"
+
+// apply == false leaves the line-search step pending for
+// nx::smooth_chebyshev_pending, which takes it as it loads the pressure
+static int learned_correct_impl(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
+                                float lambda_min, double* out, bool apply) {
   if (!sim || !u || factor < 1) return fail(NX_ERR_ARG, "bad learned correction arguments");
   void* sv = nullptr;
   int rc = nx_get_stream(sim, &sv);
@@ -411,7 +415,14 @@ int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
   if (rc != NX_OK) return rc;
   rc = run_pipeline(u, coarse_iters, lambda_min, s);
   if (rc != NX_OK) return rc;
-  return nx::prolong_line_search_padded(sim, factor, u->W, u->H, u->out, u->mc, out);
+  return nx::prolong_line_search_padded(sim, factor, u->W, u->H, u->out, u->mc, out, apply);
+}
+
+extern "C" {
+
+int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
+                       float lambda_min, double* out) {
+  return learned_correct_impl(sim, u, factor, coarse_iters, lambda_min, out, true);
 }
 
 int nx_learned_cycles(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
@@ -424,12 +435,14 @@ int nx_learned_cycles(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
   int n = 0, kept = 0, converged = 0, abandoned = 0;
   double before = -1.0;
   for (int c = 0; c < max_cycles; ++c) {
-    int rc = nx_learned_correct(sim, u, factor, coarse_iters, lambda_min, nullptr);
+    // with Chebyshev smoothing the correction's step is taken by the first
+    // smoothing launch as it loads the pressure (one pass less)
+    const bool cheb = fine_lambda_min > 0.0f;
+    int rc = learned_correct_impl(sim, u, factor, coarse_iters, lambda_min, nullptr, !cheb);
     if (rc != NX_OK) return rc;
     double out[2] = {0.0, 0.0};
-    rc = fine_lambda_min > 0.0f
-             ? nx_smooth_chebyshev(sim, smooth_iters, fine_lambda_min, tile, out)
-             : nx_smooth(sim, smooth_iters - 1, 0.5f, out);
+    rc = cheb ? nx::smooth_chebyshev_pending(sim, smooth_iters, fine_lambda_min, tile, out)
+              : nx_smooth(sim, smooth_iters - 1, 0.5f, out);
     if (rc != NX_OK) return rc;
     ++n;
     const double rel = out[0], a = out[1];
