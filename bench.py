@@ -38,7 +38,7 @@ METRIC = ("sim steps/sec at 2048² @ rel. residual ≤1e-4 "
 HBM_PEAK_GBS = 8000.0
 JACOBI_BYTES_PER_CELL_SWEEP = 13   # read p 4 + b 4 + mask 1, write p 4
 CHEB_BYTES_PER_CELL_ITER = 25      # read x 4 + r 4 + d 4 + mask 1, write x, r, d 12
-CHEB_SMOOTH = 80                   # fine Chebyshev iterations per learned cycle
+CHEB_SMOOTH = 72                   # fine Chebyshev iterations per learned cycle
 CHEB_TILE = (8, 4, 8, 2)           # nx_smooth_chebyshev's default tile
 STEP_BYTES_PER_CELL = 104          # the other stages, FP32, dye on, nu = 0
 

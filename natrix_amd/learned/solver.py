@@ -117,11 +117,11 @@ class LearnedGuess:
         return f, -(-(-(-W // f)) // 8) * 8, -(-(-(-H // f)) // 8) * 8
 
     def fine_lmin(self, W, H):
-        """Lower end of the fine Chebyshev smoother's interval, 1.28 / f^2
-        (0.005 at f = 16): the modes of -L below it (wavelengths beyond
+        """Lower end of the fine Chebyshev smoother's interval, 1.536 / f^2
+        (0.006 at f = 16): the modes of -L below it (wavelengths beyond
         about 4 f cells) are left to the coarse correction."""
         f = self.coarse_grid(W, H)[0]
-        return 1.28 / (f * f)
+        return 1.536 / (f * f)
 
     def _runner(self, wc, hc):
         if not self._native:
@@ -282,7 +282,7 @@ def _inner(sim, guess, target, smooth, max_cycles, chunk, max_sweeps, stall=0.5,
     return kept, abandoned
 
 
-def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=80, max_cycles=6,
+def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=72, max_cycles=6,
                max_outer=12, chunk=65536, max_inner_sweeps=4_000_000, trace=None,
                fine_lmin=None, fine_tile=None, native_loop=True):
     """One fluid step whose pressure is solved to FP64 relative residual
@@ -291,7 +291,7 @@ def solve_step(sim, guess, dt, tol=1e-4, inner_tol=0.1, smooth=80, max_cycles=6,
     LearnedGuess) each correction solve starts with learned cycles, without
     it the correction solves are plain Jacobi from zero.  A learned cycle
     is the correction followed by `smooth` fine smoothing iterations:
-    Chebyshev on [fine_lmin, 8] (None: the guess's 1.28 / f^2; `fine_tile`
+    Chebyshev on [fine_lmin, 8] (None: the guess's 1.536 / f^2; `fine_tile`
     the kernel's tile, None for the library's default) or, with
     fine_lmin = 0, Jacobi sweeps ending in a damped one; with a native
     runner the cycles of a correction solve run in one nx_learned_cycles
