@@ -398,13 +398,16 @@ int nx_learned_correct(nx_sim* sim, nx_unet* net, int factor, int coarse_iters,
  * search takes step 0 or a cycle leaves more than stall x the previous
  * cycle's residual (abandoned), or after max_cycles.  result[4] (host) =
  * {cycles run, cycles whose step moved the pressure, converged, abandoned};
- * records (host, may be NULL) [3 * max_cycles] = per cycle {line-search
- * step, the step's sweeps so far, residual}.  One synchronisation per
+ * records (host, may be NULL) holds records_capacity doubles, at least
+ * 3 * max_cycles (else NX_ERR_ARG): per cycle {line-search step, the
+ * step's sweeps so far, residual}.  All arguments, the smoothing tile
+ * (cheb tiles as nx_smooth_chebyshev) and 0 < fine_lambda_min < 8 included,
+ * are checked before anything is enqueued.  One synchronisation per
  * cycle.  No reference counterpart (the solver loop of the learned step). */
 int nx_learned_cycles(nx_sim* sim, nx_unet* net, int factor, int coarse_iters,
                       float lambda_min, int smooth_iters, float fine_lambda_min,
                       const int* tile, double target, double stall, int max_cycles,
-                      double* records, int* result);
+                      double* records, int records_capacity, int* result);
 /* the runner's coarse buffers (rhs, mask, correction), for inspection */
 int nx_unet_coarse(nx_unet* net, float** rc, uint8_t** mc, float** e);
 
@@ -419,8 +422,14 @@ int nx_fp64_compare(const float* p, const double* x, const uint8_t* mask, int W,
                     double* rel, void* stream);
 
 /* attainable bandwidth: a plain device copy of `bytes` (multiple of 16),
- * `iters` times; *gbps = (read + write bytes) / time */
+ * `iters` times; *gbps = (read + write bytes) / time.  nx_copy_bandwidth
+ * runs the counter-calibration copy (one float4 per thread per grid-stride
+ * iteration); nx_copy_bandwidth_variant runs variant 0-4 (0 = that copy,
+ * 1/2 = 4/8 float4 in flight per thread, 3 = 8 with non-temporal loads and
+ * stores, 4 = one contiguous chunk per workgroup) or, with variant -1, all
+ * of them and reports the best.  No reference counterpart (measurement). */
 int nx_copy_bandwidth(size_t bytes, int iters, double* gbps);
+int nx_copy_bandwidth_variant(size_t bytes, int iters, int variant, double* gbps);
 
 const char* nx_last_error(void);
 

@@ -157,7 +157,10 @@ SIGNATURES = {
     "nx_unet_destroy": [_P],
     "nx_unet_forward": [_P, _P, _P, _P],
     "nx_learned_correct": [_P, _P, _I, _I, _F, _P],
-    "nx_learned_cycles": [_P, _P, _I, _I, _F, _I, _F, _P, _D, _D, _I, _P, _P],
+    # typed pointers: ctypes turns a Python int given for a c_void_p into an
+    # address, so a misplaced integer would reach the library as a pointer
+    "nx_learned_cycles": [_P, _P, _I, _I, _F, _I, _F, C.POINTER(_I), _D, _D, _I,
+                          C.POINTER(_D), _I, C.POINTER(_I)],
     "nx_unet_coarse": [_P, _P, _P, _P],
     "nx_get_size": [_P, _P, _P],
     "nx_conv3x3_dgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
@@ -172,6 +175,7 @@ SIGNATURES = {
     "nx_fp64_solve": [_P, _P, _I, _I, _P, _D, _I, C.POINTER(_D), _P],
     "nx_fp64_compare": [_P, _P, _P, _I, _I, C.POINTER(_D), _P],
     "nx_copy_bandwidth": [_SZ, _I, C.POINTER(_D)],
+    "nx_copy_bandwidth_variant": [_SZ, _I, _I, C.POINTER(_D)],
     "nx_last_error": [],
 }
 _RESTYPE = {"nx_last_error": C.c_char_p}

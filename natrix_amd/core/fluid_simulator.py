@@ -15,6 +15,7 @@ per-stage entry points.
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -293,10 +294,20 @@ class FluidSimulator:
     def set_guess(self, guess, tol=1e-4):
         """Solve the pressure of every update() with `guess` (a
         natrix_amd.learned.solver.LearnedGuess) to FP64 relative residual
-        <= tol, by iterative refinement with learned cycles and Jacobi;
-        None restores the fixed-sweep (or `tol`-driven) Jacobi step."""
-        if tol is not None and not (tol > 0):
-            raise ValueError("tol must be positive")
+        <= tol (a positive number), by iterative refinement with learned
+        cycles and Jacobi; guess=None restores the fixed-sweep (or
+        `tol`-driven) Jacobi step and ignores `tol`.  The learned step needs
+        float32 storage: a guess on a bf16/f16 simulator is refused here
+        rather than by every later update()."""
+        if guess is None:
+            self._guess = None
+            return
+        if self.storage != "f32":
+            raise ValueError("the learned step needs float32 storage "
+                             f"(this simulator stores {self.storage})")
+        if tol is None or isinstance(tol, bool) or not (float(tol) > 0 and
+                                                        math.isfinite(float(tol))):
+            raise ValueError("tol must be a positive finite number")
         self._guess = guess
         self._guess_tol = float(tol)
 

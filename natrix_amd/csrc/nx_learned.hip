@@ -428,16 +428,24 @@ int nx_learned_correct(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
 int nx_learned_cycles(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
                       float lambda_min, int smooth_iters, float fine_lambda_min,
                       const int* tile, double target, double stall, int max_cycles,
-                      double* records, int* result) {
-  if (!sim || !u || !result || factor < 1 || smooth_iters < 1 || max_cycles < 0 ||
-      !(stall > 0.0))
+                      double* records, int records_capacity, int* result) {
+  // every argument is checked before anything is enqueued: a caller's
+  // record buffer is written only within the capacity it states, and the
+  // smoothing tile and interval are validated here rather than by the
+  // first smoothing call (which runs after a correction was enqueued)
+  if (!sim || !u || !result || factor < 1 || coarse_iters < 0 || smooth_iters < 1 ||
+      max_cycles < 0 || !(stall > 0.0) || !(fine_lambda_min < 8.0f))
     return fail(NX_ERR_ARG, "bad learned cycle arguments");
+  if (records && (records_capacity < 0 || (long long)records_capacity < 3LL * max_cycles))
+    return fail(NX_ERR_ARG, "records_capacity must be at least 3 * max_cycles");
+  // with Chebyshev smoothing the correction's step is taken by the first
+  // smoothing launch as it loads the pressure (one pass less)
+  const bool cheb = fine_lambda_min > 0.0f;
+  if (cheb && tile && !nx::cheb_cfg_supported(nx::JacobiCfg{tile[0], tile[1], tile[2], tile[3]}))
+    return fail(NX_ERR_ARG, "unsupported Chebyshev tile");
   int n = 0, kept = 0, converged = 0, abandoned = 0;
   double before = -1.0;
   for (int c = 0; c < max_cycles; ++c) {
-    // with Chebyshev smoothing the correction's step is taken by the first
-    // smoothing launch as it loads the pressure (one pass less)
-    const bool cheb = fine_lambda_min > 0.0f;
     int rc = learned_correct_impl(sim, u, factor, coarse_iters, lambda_min, nullptr, !cheb);
     if (rc != NX_OK) return rc;
     double out[2] = {0.0, 0.0};

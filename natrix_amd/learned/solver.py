@@ -20,12 +20,18 @@ One learned cycle:
      search (the step along the correction that minimises the solve's
      energy functional: it never increases the error in the energy norm,
      and a useless, NaN or infinite correction gets step 0);
-  5. `smooth` fine Jacobi sweeps, the last one damped (omega = 1/2).
+  5. `smooth` fine smoothing iterations: by default Chebyshev on
+     [1.536 / f^2, 8] (nx_smooth_chebyshev, temporal-blocked k_cheb_tb,
+     whose first launch also takes step 4's line-search step as it loads
+     the pressure); with fine_lmin = 0, Jacobi sweeps ending in one damped
+     sweep (omega = 1/2, nx_smooth).
 Cycles repeat while they reduce the residual by a useful factor; then plain
 Jacobi chunks run until the relative residual is <= the target, so the
 step's result satisfies the same criterion as the plain path.  With the
-native runner, steps 1-4 are one C-ABI call (nx_learned_correct) whose
-coarse-grid part replays as a HIP graph, and step 5 one more (nx_smooth).
+native runner the cycles of a correction solve, decisions included, are one
+C-ABI call (nx_learned_cycles), whose coarse-grid part (steps 2-3) replays
+as a HIP graph; native_loop=False runs the same cycles from Python, one
+nx_learned_correct and one smoothing call per cycle.
 """
 from __future__ import annotations
 
@@ -221,12 +227,13 @@ def _native_cycles(sim, guess, nat, target, smooth, max_cycles, stall, trace, lm
     and decisions as the loop in _inner).  Returns (kept, converged,
     abandoned, residual after the last cycle or None)."""
     f, runner = nat
-    rec = (C.c_double * (3 * max(1, max_cycles)))()
+    cap = 3 * max(1, max_cycles)
+    rec = (C.c_double * cap)()
     res = (C.c_int * 4)()
     t = None if tile is None else (C.c_int * 4)(*(int(v) for v in tile))
     _lib.call("nx_learned_cycles", sim._alive(), runner.handle, f, int(guess.coarse_iters),
               float(guess.lmin), int(smooth), float(lmin), t, float(target), float(stall),
-              int(max_cycles), rec, res)
+              int(max_cycles), rec, cap, res)
     n, kept, conv, abandoned = res[0], res[1], bool(res[2]), bool(res[3])
     if trace is not None:
         for c in range(n):

@@ -142,3 +142,41 @@ def test_python_property_validation_matches_reference():
     s.viscosity = 0.0
     s.speed = 250
     assert (s.vorticity, s.viscosity, s.speed) == (0, 0.0, 250)
+
+
+def _header_arity():
+    """name -> number of parameters of every function include/natrix.h
+    declares (declarations may span lines; `(void)` has none)."""
+    txt = open(HEADER).read()
+    out = {}
+    for m in re.finditer(r"^(?:int|const char\*)\s+(nx_\w+)\(([^;]*?)\);", txt, re.M | re.S):
+        params = " ".join(m.group(2).split())
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_binding_arity_matches_header():
+    """Every ctypes signature has as many arguments as the C declaration:
+    a stale table would shift arguments into the wrong slots."""
+    ar = _header_arity()
+    assert sorted(ar) == sorted(_lib.SIGNATURES)
+    for name, args in _lib.SIGNATURES.items():
+        assert len(args) == ar[name], name
+
+
+def test_learned_cycles_binding_refuses_integers_as_pointers():
+    """The tile, record and result slots of nx_learned_cycles are typed
+    pointers: ctypes raises before the call when a Python integer lands
+    there (a c_void_p slot would pass it on as an address).  Also the
+    null-simulator check returns NX_ERR_ARG without touching the GPU."""
+    lib = _lib.lib()
+    res = (C.c_int * 4)()
+    rec = (C.c_double * 3)()
+    good = [None, None, 16, 100, 0.01, 72, 0.006, None, 0.1, 0.5, 1, rec, 3, res]
+    for slot in (7, 11, 13):
+        a = list(good)
+        a[slot] = 4096
+        with pytest.raises(C.ArgumentError):
+            lib.nx_learned_cycles(*a)
+    assert lib.nx_learned_cycles(*good) == _lib.NX_ERR_ARG
+    assert "bad learned cycle arguments" in _lib.last_error()

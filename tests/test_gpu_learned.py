@@ -480,10 +480,19 @@ def test_set_guess_routes_update_through_the_learned_step(gpu):
     b.update(1 / 60)
     a.update(1 / 60)
     assert np.array_equal(a.download("velocity"), b.download("velocity"))
-    with pytest.raises(ValueError):
-        a.set_guess(g, tol=0.0)
+    for bad in (0.0, None, float("nan"), float("inf"), -1e-4):
+        with pytest.raises(ValueError):
+            a.set_guess(g, tol=bad)
+    a.set_guess(None, tol=None)  # no guess: tol is ignored
     a.destroy()
     b.destroy()
+    from natrix_amd import FluidSimulator
+    for st in ("bf16", "f16"):
+        m = FluidSimulator(64, 64, storage=st)
+        with pytest.raises(ValueError, match="float32 storage"):
+            m.set_guess(g)
+        m.update(1 / 60)  # the simulator still steps
+        m.destroy()
 
 
 @pytest.mark.parametrize("W,H,grid", [(640, 360, (5, 128, 72)), (1000, 700, (8, 128, 88))])
@@ -644,7 +653,12 @@ def test_native_cycle_loop_equals_the_python_loop(gpu, fine_lmin):
 
 def test_native_cycle_loop_arguments(gpu):
     """nx_learned_cycles rejects bad arguments before enqueueing anything
-    and returns at once for zero cycles."""
+    and returns at once for zero cycles.  Regression cases: a record
+    buffer smaller than 3 * max_cycles, an unsupported Chebyshev tile and a
+    fine interval bound >= 8 are rejected up front (the records were once
+    written with no length, the tile read and the interval checked only by
+    the first smoothing call), and the binding refuses a Python integer in
+    a pointer slot instead of handing it to the library as an address."""
     from natrix_amd.learned.solver import LearnedGuess
     from natrix_amd.learned.train import load
     net, meta = load()
@@ -653,18 +667,32 @@ def test_native_cycle_loop_arguments(gpu):
     f, runner = g.native(sim)
     sim._push_params()
     _lib.call("nx_step_begin", sim._alive(), 1 / 60)
+    before = sim.download("pressure")
     res = (C.c_int * 4)()
     rec = (C.c_double * 3)()
-    args = [f, 100, float(g.lmin), 80, 0.3, None, 0.1, 0.5, 1, rec, res]
-    # factor, smoothing iterations, stall factor, cycle count
-    for i, bad in ((0, 0), (3, 0), (7, 0.0), (8, -1)):
+    bad_tile = (C.c_int * 4)(8, 3, 8, 2)
+    args = [f, 100, float(g.lmin), 80, 0.3, None, 0.1, 0.5, 1, rec, 3, res]
+    # factor, coarse iterations, smoothing iterations, fine interval, tile,
+    # stall factor, cycle count, record capacity (too small for 2 cycles)
+    for i, bad in ((0, 0), (1, -1), (3, 0), (4, 8.0), (4, float("nan")), (5, bad_tile),
+                   (7, 0.0), (8, -1), (10, 2), (10, -1)):
         a = list(args)
         a[i] = bad
         with pytest.raises(ValueError):
             _lib.call("nx_learned_cycles", sim._alive(), runner.handle, *a)
+    a = list(args)
+    a[8] = 2  # two cycles need six record slots
+    with pytest.raises(ValueError):
+        _lib.call("nx_learned_cycles", sim._alive(), runner.handle, *a)
     with pytest.raises(ValueError):
         _lib.call("nx_learned_cycles", sim._alive(), runner.handle, *args[:-1], None)
-    before = sim.download("pressure")
+    # an integer where a pointer belongs never reaches the library
+    for i in (5, 9):
+        a = list(args)
+        a[i] = 4096
+        with pytest.raises(C.ArgumentError):
+            _lib.call("nx_learned_cycles", sim._alive(), runner.handle, *a)
+    assert np.array_equal(sim.download("pressure"), before)
     a = list(args)
     a[8] = 0
     _lib.call("nx_learned_cycles", sim._alive(), runner.handle, *a)
