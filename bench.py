@@ -68,6 +68,68 @@ def committed_traffic(N, iters, tile, kernel="k_jacobi_tb", suffix="fixed"):
         return None, None
 
 
+def roofline_entry(kernel, algo_bytes, launch_us, traffic, traffic_src, **extra):
+    """The roofline block of one kernel.  `algo_bytes`: its compulsory bytes
+    per launch (every array it reads or writes moved once per launch);
+    `traffic`: HBM bytes per launch from the committed counter summary
+    (FETCH_SIZE x 2 + WRITE_SIZE x 1 per dispatch, the copy calibration) or
+    None.  `achieved` and `frac` are the measured traffic's rate when the
+    summary matches this launch shape, else the compulsory bytes' rate;
+    either way a fraction of the HBM peak, not an on-chip iteration rate."""
+    t = launch_us * 1e-6
+    algo_gbs = algo_bytes / t / 1e9
+    out = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": kernel,
+           "avg_launch_us": round(launch_us, 3)}
+    if traffic:
+        ach = traffic / t / 1e9
+        out.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4),
+                   basis="hbm counter bytes per launch / launch time", traffic=traffic,
+                   traffic_source=traffic_src,
+                   traffic_over_algorithmic=round(traffic / algo_bytes, 3))
+    else:
+        out.update(achieved=round(algo_gbs, 1), frac=round(algo_gbs / HBM_PEAK_GBS, 4),
+                   basis="compulsory bytes per launch / launch time (no matching counter "
+                         "summary)", traffic=None, traffic_source=None)
+    out.update(algorithmic_bytes_per_launch=int(algo_bytes),
+               algorithmic_GBps=round(algo_gbs, 1),
+               algorithmic_frac=round(algo_gbs / HBM_PEAK_GBS, 4))
+    out.update(extra)
+    return out
+
+
+def cheb_launch_plan(iterations, k):
+    """(passes, compulsory bytes per cell) of each launch of one
+    nx_smooth_chebyshev call with k passes per launch: the first launch
+    spends a pass on the setup, the last one a pass on the FP64 residual
+    (the library's launch loop)."""
+    plan, done = [], 0
+    while True:
+        setup = 1 if done == 0 else 0
+        last = setup + (iterations - done) + 1 <= k
+        kk = setup + (iterations - done) + 1 if last else k
+        its = kk - setup - (1 if last else 0)
+        # reads: x, mask, and (r, d) or b; the last also b; writes x and,
+        # unless last, r and d
+        rd = 4 + 1 + (4 if setup else 8) + (4 if last and not setup else 0)
+        wr = 4 if last else 12
+        plan.append((kk, rd + wr))
+        done += its
+        if last:
+            return plan
+
+
+def baseline_formula(cells, K, ms_per_step):
+    """The step-level figure of the survey's bandwidth model:
+    N^2 (104 + 13 K) bytes per step (each array once per stage, K Jacobi /
+    smoother passes each 13 B) over the step time.  Temporal blocking keeps
+    iterates on chip between passes, so this ratio to the HBM peak is not
+    bounded by 1 and is not a roofline fraction."""
+    by = cells * (STEP_BYTES_PER_CELL + JACOBI_BYTES_PER_CELL_SWEEP * K)
+    tbs = by / (ms_per_step * 1e-3) / 1e12
+    return {"bytes_per_step": int(by), "passes_per_step": round(float(K), 2),
+            "TBps": round(tbs, 3), "ratio_to_peak": round(tbs / (HBM_PEAK_GBS / 1e3), 4)}
+
+
 def make_scene(N, seed=2):
     rng = np.random.default_rng(seed)
     c = rng.uniform(0.3, 0.7, 2)
@@ -233,11 +295,12 @@ def bench_batch(args, copy_bandwidth):
     tb_sweeps = (iters - 1) * args.steps
     roof = None
     if jac_n and jac_ms > 0:
-        ach = JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps / (jac_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "kb_jacobi", "avg_launch_us": round(1e3 * jac_ms / jac_n, 3)}
-    step_bytes = cells * (STEP_BYTES_PER_CELL + JACOBI_BYTES_PER_CELL_SWEEP * iters)
+        # compulsory bytes per launch: every scene's p, b, mask read, p written
+        roof = roofline_entry(
+            "kb_jacobi", JACOBI_BYTES_PER_CELL_SWEEP * cells, 1e3 * jac_ms / jac_n, None, None,
+            sweeps_per_launch=round(tb_sweeps / jac_n, 3),
+            per_sweep_GBps=round(JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps
+                                 / (jac_ms * 1e-3) / 1e9, 1))
     ms = 1e3 * el / args.steps
     out = {"metric": "batched sim steps/sec (64 uneven scenes, one launch per stage)",
            "value": round(args.steps / el, 3), "unit": "steps/s", "n_gpus": 1,
@@ -248,8 +311,7 @@ def bench_batch(args, copy_bandwidth):
                                   f"free-slip walls, splats every step",
                       "jacobi_tile": list(b.jacobi_tile), "graph": bool(args.graph)},
            "cells_per_s": round(cells * args.steps / el, 1),
-           "step_algorithmic_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
-           "step_roofline_frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "baseline_formula": baseline_formula(cells, iters, ms),
            "stage_ms_per_step": {k: round(v[0] / args.steps, 5)
                                  for k, v in timing.items() if v[1]},
            "roofline": roof, "cpu_baseline": None}
@@ -275,18 +337,32 @@ def mixed_scene(N, storage, iters, dye=True):
     return sim, splats
 
 
-def mixed_error_vs_f32(N, storage, iters, steps=3):
-    """Relative L2 velocity error of the reduced-storage rollout against the
-    float32 GPU rollout of the same full-size scene, 3 steps from rest."""
-    out = {}
+def mixed_error_vs_f32(N, storage, iters, steps=500):
+    """Error of each field of the reduced-storage rollout against the
+    float32 GPU rollout of the same full-size scene (8 splats every step,
+    `steps` steps from rest, the mixed configuration's length): relative
+    max-norm and L2 error of velocity, pressure (the corrected float32
+    pressure, mean-free over the grid) and dye."""
+    fields = {}
     for st in (storage, "f32"):
-        sim, splats = mixed_scene(N, st, iters, dye=False)
+        sim, splats = mixed_scene(N, st, iters, dye=True)
         for _ in range(steps):
             sim.add_splats(splats)
             sim.update(1 / 60)
-        out[st] = sim.download("velocity").astype(np.float64)
+        fields[st] = {"velocity": sim.download("velocity").astype(np.float64),
+                      "pressure": sim.download("pressure32").astype(np.float64),
+                      "dye": sim.download("dye").astype(np.float64)}
         sim.destroy()
-    return float(np.linalg.norm(out[storage] - out["f32"]) / np.linalg.norm(out["f32"]))
+    out = {"steps": steps, "reference": "float32 storage, same scene and steps"}
+    for f in ("velocity", "pressure", "dye"):
+        a, b = fields[storage][f], fields["f32"][f]
+        if f == "pressure":
+            a = a - a.mean()
+            b = b - b.mean()
+        d = a - b
+        out[f] = {"max_rel": float(np.abs(d).max() / np.abs(b).max()),
+                  "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(b))}
+    return out
 
 
 def bench_mixed(args):
@@ -294,6 +370,7 @@ def bench_mixed(args):
     N = 4096 if args.size == 2048 else args.size
     iters = 120 if args.iters == 80 else args.iters
     sim, splats = mixed_scene(N, args.storage, iters)
+    mixed_k = sim.jacobi_tile[0]
     log(f"bench mixed: {N}^2 {args.storage} I={iters} tile {sim.jacobi_tile}")
     if args.graph:
         sim.use_graph = True
@@ -313,25 +390,28 @@ def bench_mixed(args):
     el = time.perf_counter() - t0
     timing = sim.collect_timing()
     sim.destroy()
-    err32 = None if args.no_errors else mixed_error_vs_f32(N, args.storage, iters)
+    err32 = None if args.no_errors else mixed_error_vs_f32(N, args.storage, iters,
+                                                           args.error_steps)
     ms = 1e3 * el / args.steps
     cells = N * N
     # bf16/f16 bytes per cell: 54 for the other stages + 7 per Jacobi sweep
     # (p 2 + b 2 + mask 1 in, p 2 out) + splats 12 + the correction launch
     # (p 2 + b 2 + mask 1 in, p32 4 out) + projection's float32 p (2 extra)
     step_bytes = cells * (54 + 7 * (iters - 1) + 12 + 9 + 2)
+    step_tbs = step_bytes / (ms * 1e-3) / 1e12
     jac_ms, jac_n = timing["jacobi"]
     roof = None
     if jac_n and jac_ms > 0:
-        # km_jacobi: 7 B per cell-sweep (p 2 + b 2 + mask 1 in, p 2 out) for
-        # the I - 1 sweeps after the fused first one; the correction launch
-        # reads p 2 + b 2 + mask 1 and writes the float32 pressure 4
-        jb = cells * (7 * (iters - 1) + 9) * args.steps
-        ach = jb / (jac_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "km_jacobi (+ the correction launch)",
-                "avg_launch_us": round(1e3 * jac_ms / jac_n, 3)}
+        # km_jacobi: per launch p 2 + b 2 + mask 1 in, p 2 out (7 B/cell);
+        # the correction launch reads p 2 + b 2 + mask 1 and writes the
+        # float32 pressure 4 (9 B/cell); launches per step from the tile
+        k = mixed_k
+        per_step_launches = -(-(iters - 1) // k) + 1
+        algo = cells * (7 * (per_step_launches - 1) + 9) / per_step_launches
+        roof = roofline_entry(
+            "km_jacobi (+ the correction launch)", algo, 1e3 * jac_ms / jac_n, None, None,
+            per_sweep_GBps=round(cells * (7 * (iters - 1) + 9) * args.steps
+                                 / (jac_ms * 1e-3) / 1e9, 1))
     out = {"metric": f"sim steps/sec at {N}^2, {args.storage} storage",
            "value": round(args.steps / el, 3), "unit": "steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
@@ -340,11 +420,11 @@ def bench_mixed(args):
            "config": {"workload": f"{N}x{N}, 8 Gaussian splats/step, I={iters} + "
                                   f"8 float32 correction sweeps, free-slip walls, dye",
                       "jacobi_tile": None, "graph": bool(args.graph)},
-           "step_algorithmic_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
-           "step_roofline_frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "baseline_formula": {"bytes_per_step": int(step_bytes), "TBps": round(step_tbs, 3),
+                                "ratio_to_peak": round(step_tbs / (HBM_PEAK_GBS / 1e3), 4)},
            "stage_ms_per_step": {k: round(v[0] / args.steps, 5)
                                  for k, v in timing.items() if v[1]},
-           "velocity_rel_l2_err_vs_f32_gpu_3_steps": err32,
+           "field_errors_vs_f32_gpu": err32,
            "roofline": roof, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
@@ -370,7 +450,7 @@ def bench_learned(args, copy_bandwidth):
     dt = 1.0 / 60.0
     cells = N * N
 
-    def fresh():
+    def fresh(develop=None):
         sim = FluidSimulator(N, N)
         sim.vorticity = 0.3
         sim.viscosity = 0.0
@@ -378,34 +458,52 @@ def bench_learned(args, copy_bandwidth):
         sim.bc_mode = "inflow"
         sim.inflow = 1.0
         sim.set_obstacle_map(solid)
-        for _ in range(args.warmup):
+        for _ in range(args.warmup if develop is None else develop):
             sim.update(dt)
         sim.synchronize()
         return sim
 
+    def timed_learned(sim, steps):
+        for _ in range(2):  # first-call costs (allocator, code objects) off the clock
+            solve_step(sim, guess, dt, tol=args.tol, smooth=CHEB_SMOOTH)
+        sim.synchronize()
+        torch.cuda.synchronize()
+        # no per-launch events in this timed region (they cost ~10 % of a
+        # learned step); the kernels' rooflines are taken in phases of their own
+        sweeps, outer, conv = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = solve_step(sim, guess, dt, tol=args.tol, smooth=CHEB_SMOOTH)
+            sweeps.append(st["sweeps"])
+            outer.append(st["outer"])
+            conv.append(st["converged"] and st["residual"] <= args.tol)
+        sim.synchronize()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms = 1e3 * el / steps
+        K = float(np.mean(sweeps))
+        return {"steps_per_s": round(steps / el, 3), "ms_per_step": round(ms, 3),
+                "sweeps_per_step": K, "outer_per_step": float(np.mean(outer)),
+                "all_converged": bool(all(conv)), "timed_steps": steps,
+                "baseline_formula": baseline_formula(cells, K, ms)}
+
     log(f"bench learned: {N}x{N}, warmup {args.warmup} fixed steps (I={args.iters})")
     sim = fresh()
-    for _ in range(2):  # first-call costs (allocator, code objects) off the clock
-        solve_step(sim, guess, dt, tol=args.tol, smooth=CHEB_SMOOTH)
-    sim.synchronize()
-    torch.cuda.synchronize()
-    # no per-launch events in this timed region (they cost ~10 % of a
-    # learned step); the kernels' rooflines are taken in phases of their own
-    sweeps, outer, conv = [], [], []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st = solve_step(sim, guess, dt, tol=args.tol, smooth=CHEB_SMOOTH)
-        sweeps.append(st["sweeps"])
-        outer.append(st["outer"])
-        conv.append(st["converged"] and st["residual"] <= args.tol)
-    sim.synchronize()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    learned = timed_learned(sim, args.steps)
     sim.destroy()
-    learned = {"steps_per_s": round(args.steps / el, 3), "ms_per_step": round(1e3 * el / args.steps, 3),
-               "sweeps_per_step": float(np.mean(sweeps)), "outer_per_step": float(np.mean(outer)),
-               "all_converged": bool(all(conv))}
+    learned["flow"] = f"steps {args.warmup + 3}..{args.warmup + 2 + args.steps} from rest"
     log(f"bench learned: {learned}")
+
+    developed = None
+    if args.develop > 0:
+        # the same scene after `develop` fixed-iteration steps (the wake
+        # behind the cylinder has formed), then learned steps
+        sim = fresh(args.develop)
+        developed = timed_learned(sim, args.developed_steps)
+        sim.destroy()
+        developed["flow"] = (f"steps {args.develop + 3}..{args.develop + 2 + args.developed_steps}"
+                             f" ({args.develop} fixed-iteration steps of development)")
+        log(f"bench learned, developed flow: {developed}")
 
     plain = None
     if not args.no_plain:
@@ -420,12 +518,13 @@ def bench_learned(args, copy_bandwidth):
         plain = {"steps_per_s": round(1.0 / t_p, 6), "ms_per_step": round(1e3 * t_p, 1),
                  "sweeps_per_step": st["sweeps"], "converged": bool(done),
                  "residual_reached": st["residual"], "sweep_cap_per_correction_solve": args.plain_cap,
-                 "rate_is_upper_bound": not done}
+                 "rate_is_upper_bound": not done,
+                 "baseline_formula": baseline_formula(cells, st["sweeps"], 1e3 * t_p)}
         log(f"bench plain: {plain}")
 
     # the learned step's dominant kernel: the fine Chebyshev smoother, timed
     # per launch with HIP events on the simulator's stream over calls of the
-    # step's own length (iterations per call, default tile) on this scene
+    # step's own length (iterations per call, the simulator's smoother tile)
     import ctypes as C
     from natrix_amd import _lib
     sim = fresh()
@@ -433,6 +532,9 @@ def bench_learned(args, copy_bandwidth):
     _lib.call("nx_step_begin", sim._alive(), dt)
     lmin = guess.fine_lmin(N, N)
     n_iter, calls = CHEB_SMOOTH, 20
+    tile = (C.c_int * 5)()
+    _lib.call("nx_get_smoother_tile", sim._alive(), tile)
+    tile = tuple(tile) if tile[4] > 0 else tuple(tile)[:4]
     _lib.call("nx_smooth_chebyshev", sim._alive(), n_iter, lmin, None, None)
     sim.set_timing(True)
     sim.collect_timing()
@@ -443,16 +545,23 @@ def bench_learned(args, copy_bandwidth):
     _lib.call("nx_step_finish", sim._alive())
     sim.destroy()
     ch_ms, ch_n = tc["chebyshev"]
-    ch_iters = n_iter * calls
-    ach_c = CHEB_BYTES_PER_CELL_ITER * cells * ch_iters / (ch_ms * 1e-3) / 1e9
-    traffic_c, tsrc_c = committed_traffic(N, n_iter, CHEB_TILE, "k_cheb_tb", "iters")
-    roof_cheb = {"bound": "hbm", "achieved": round(ach_c, 1), "peak": HBM_PEAK_GBS,
-                 "unit": "GB/s", "frac": round(ach_c / HBM_PEAK_GBS, 4), "traffic": traffic_c,
-                 "traffic_source": tsrc_c, "kernel": "k_cheb_tb",
-                 "avg_launch_us": round(1e3 * ch_ms / ch_n, 3),
-                 "iterations_per_launch": round(ch_iters / ch_n, 3),
-                 "algorithmic_bytes_per_launch": int(CHEB_BYTES_PER_CELL_ITER * cells * ch_iters / ch_n),
-                 "lambda_min": lmin, "tile": list(CHEB_TILE)}
+    plan = cheb_launch_plan(n_iter, tile[0])
+    assert ch_n == calls * len(plan), (ch_n, len(plan))
+    algo_call = cells * sum(b for _, b in plan)
+    kname = "k_cheb_stream" if len(tile) > 4 else "k_cheb_tb"
+    traffic_c, tsrc_c = committed_traffic(N, n_iter, tile, kname, "iters")
+    roof_cheb = roofline_entry(
+        kname, algo_call / len(plan), 1e3 * ch_ms / ch_n, traffic_c, tsrc_c,
+        iterations_per_launch=round(n_iter / len(plan), 3),
+        algorithmic_bytes_note="per call of 72 iterations: the first launch reads x, b, mask "
+                               "and writes x, r, d (21 B/cell), the middle ones read x, r, d, "
+                               "mask and write x, r, d (25 B/cell), the last reads x, r, d, b, "
+                               "mask and writes x (21 B/cell); averaged per launch",
+        per_iteration_GBps=round(CHEB_BYTES_PER_CELL_ITER * cells * n_iter * calls
+                                 / (ch_ms * 1e-3) / 1e9, 1),
+        per_iteration_note="25 B per cell-iteration as if every iteration went to HBM "
+                           "(temporal blocking keeps x, r, d on chip between passes)",
+        lambda_min=lmin, tile=list(tile))
     log(f"bench chebyshev: {roof_cheb}")
 
     # fixed-sweep plain fused step: exact sweeps per Jacobi launch
@@ -466,27 +575,27 @@ def bench_learned(args, copy_bandwidth):
     el_f = time.perf_counter() - t0
     tf = sim.collect_timing()
     sim.set_timing(False)
-    tile = list(sim.jacobi_tile)
+    jtile = tuple(sim.jacobi_tile)
     sim.destroy()
     jac_ms, jac_n = tf["jacobi"]
     tb_sweeps = (args.iters - 1) * args.fixed_steps  # sweep 1 runs in the fused divergence kernel
-    ach = JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps / (jac_ms * 1e-3) / 1e9
-    traffic, tsrc = committed_traffic(N, args.iters, tile)
-    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
-            "kernel": "k_jacobi_tb",
-            "avg_launch_us": round(1e3 * jac_ms / jac_n, 3),
-            "sweeps_per_launch": round(tb_sweeps / jac_n, 3),
-            "algorithmic_bytes_per_launch": int(JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps / jac_n)}
+    jname = "k_jacobi_stream" if len(jtile) > 4 and jtile[4] > 0 else "k_jacobi_tb"
+    traffic, tsrc = committed_traffic(N, args.iters, jtile, jname)
+    roof = roofline_entry(
+        jname, JACOBI_BYTES_PER_CELL_SWEEP * cells, 1e3 * jac_ms / jac_n, traffic, tsrc,
+        sweeps_per_launch=round(tb_sweeps / jac_n, 3),
+        algorithmic_bytes_note="13 B per cell per launch: read p, b, mask, write p",
+        per_sweep_GBps=round(JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps
+                             / (jac_ms * 1e-3) / 1e9, 1),
+        tile=list(jtile))
     ms_f = 1e3 * el_f / args.fixed_steps
-    step_bytes = cells * (STEP_BYTES_PER_CELL + JACOBI_BYTES_PER_CELL_SWEEP * args.iters)
     fixed = {"steps_per_s": round(args.fixed_steps / el_f, 2), "ms_per_step": round(ms_f, 4),
-             "iterations": args.iters, "jacobi_tile": tile,
-             "step_algorithmic_GBps": round(step_bytes / (ms_f * 1e-3) / 1e9, 1),
-             "step_roofline_frac": round(step_bytes / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "iterations": args.iters, "jacobi_tile": list(jtile),
+             "baseline_formula": baseline_formula(cells, args.iters, ms_f),
              "stage_ms_per_step": {k: round(v[0] / args.fixed_steps, 5) for k, v in tf.items() if v[1]}}
     log(f"bench fixed: {fixed} roofline {roof}")
     copy_gbs = copy_bandwidth(1 << 30, 20)
+    attainable = copy_bandwidth(1 << 30, 20, -1)
 
     cpu = None
     if not args.no_cpu:
@@ -495,14 +604,15 @@ def bench_learned(args, copy_bandwidth):
         cpu_sweeps_per_s = sps * args.iters
         plain_sweeps = plain["sweeps_per_step"] if plain else None
         cpu = {"value": (float(f"{cpu_sweeps_per_s / plain_sweeps:.4g}") if plain_sweeps else None),
-               "unit": "steps/s", "cores": 1, "kind": "port",
+               "unit": "steps/s", "cores": 1, "kind": "port", "derived": True,
                "sample": f"oracle/fluid.py step() on the same {N}x{N} scene with {args.iters} "
                          f"Jacobi sweeps: {nst} steps in {el_c:.1f} s = {cpu_sweeps_per_s:.1f} "
                          f"sweeps/s (NumPy float32, one thread; host reports {os.cpu_count()} "
-                         f"cpus); value = that rate / the plain path's sweeps per step to the "
-                         f"tolerance ({plain_sweeps}"
+                         f"cpus); value is derived, not timed to the tolerance: that rate / "
+                         f"the plain path's sweeps per step to the tolerance ({plain_sweeps}"
                          + (", a lower bound: capped" if plain and plain["rate_is_upper_bound"] else "")
                          + ") = the CPU's plain-Jacobi steps/s at the tolerance",
+               "measured_sweeps_per_s": round(cpu_sweeps_per_s, 2),
                "fixed_iteration_steps_per_s": round(sps, 4),
                "threadpools": threadpools(),
                "config_256_200_steps": config_256()}
@@ -518,10 +628,11 @@ def bench_learned(args, copy_bandwidth):
                       "grid": N, "solver": "learned two-grid cycle (U-Net bf16 MFMA guess, "
                                            "Chebyshev coarse and fine smoothing), FP64 refinement",
                       "weights": os.path.basename(args.weights or DEFAULT_WEIGHTS)},
-           "learned": learned, "plain": plain,
+           "learned": learned, "learned_developed_flow": developed, "plain": plain,
            "speedup_learned_vs_plain": round(speedup, 1) if speedup else None,
            "speedup_is_lower_bound": bool(plain and plain["rate_is_upper_bound"]),
            "plain_fixed_iterations": fixed, "copy_probe_GBps": round(copy_gbs, 1),
+           "attainable_copy_GBps": round(attainable, 1),
            "roofline": roof_cheb, "roofline_plain_jacobi": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
@@ -553,12 +664,19 @@ def main():
     ap.add_argument("--storage", choices=["bf16", "f16"], default="bf16")
     ap.add_argument("--no-errors", action="store_true",
                     help="mixed workload: skip the error rollouts (for profiling)")
+    ap.add_argument("--error-steps", type=int, default=500,
+                    help="mixed workload: steps of the rollouts compared with float32")
     ap.add_argument("--weights", default=None, help="U-Net weights (learned workload)")
     ap.add_argument("--plain-cap", type=int, default=1_000_000,
                     help="sweep cap per correction solve of the plain path (learned workload)")
     ap.add_argument("--plain-outer", type=int, default=2,
                     help="refinement iterations of the capped plain step (learned workload)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--develop", type=int, default=500,
+                    help="learned workload: also time learned steps after this many "
+                         "fixed-iteration steps of development (0: skip)")
+    ap.add_argument("--developed-steps", type=int, default=10,
+                    help="learned workload: timed steps of the developed-flow leg")
     ap.add_argument("--fixed-steps", type=int, default=100,
                     help="fixed-iteration steps timed for the Jacobi roofline (learned workload)")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
@@ -578,9 +696,11 @@ def main():
     import torch
     from natrix_amd import FluidSimulator, _lib
 
-    def copy_bandwidth(nbytes, iters):
+    def copy_bandwidth(nbytes, iters, variant=0):
+        """GB/s of the counter-calibration copy (variant 0) or, variant -1,
+        the best of the probe's copy variants (the attainable rate)."""
         g = ctypes.c_double()
-        _lib.call("nx_copy_bandwidth", nbytes, iters, ctypes.byref(g))
+        _lib.call("nx_copy_bandwidth_variant", nbytes, iters, variant, ctypes.byref(g))
         return g.value
 
     if args.probe_copy:
@@ -638,23 +758,19 @@ def main():
     tb_sweeps = sweeps_total - args.steps
     roof = None
     if jac_launches > 0 and jac_ms > 0:
-        jac_bytes = JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps
-        achieved = jac_bytes / (jac_ms * 1e-3) / 1e9
-        traffic, tsrc = (committed_traffic(N, args.iters, sim.jacobi_tile)
+        jtile = tuple(sim.jacobi_tile)
+        jname = "k_jacobi_stream" if len(jtile) > 4 and jtile[4] > 0 else "k_jacobi_tb"
+        traffic, tsrc = (committed_traffic(N, args.iters, jtile, jname)
                          if args.mode == "fixed" else (None, None))
-        roof = {"bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": tsrc,
-                "kernel": "k_jacobi_tb",
-                "avg_launch_us": round(1e3 * jac_ms / jac_launches, 3),
-                "sweeps_per_launch": round(tb_sweeps / jac_launches, 3)}
-    step_bytes = cells * (STEP_BYTES_PER_CELL +
-                          JACOBI_BYTES_PER_CELL_SWEEP * sweeps_total / args.steps)
-    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
+        roof = roofline_entry(
+            jname, JACOBI_BYTES_PER_CELL_SWEEP * cells, 1e3 * jac_ms / jac_launches, traffic,
+            tsrc, sweeps_per_launch=round(tb_sweeps / jac_launches, 3),
+            per_sweep_GBps=round(JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps
+                                 / (jac_ms * 1e-3) / 1e9, 1))
 
     copy_gbs = copy_bandwidth(1 << 30, 20)
-    log(f"bench: copy probe {copy_gbs:.0f} GB/s")
+    attainable = copy_bandwidth(1 << 30, 20, -1)
+    log(f"bench: copy probe {copy_gbs:.0f} GB/s, best copy variant {attainable:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -691,11 +807,11 @@ def main():
         },
         "sweeps_per_step": round(sweeps_total / args.steps, 2),
         "final_rel_residual": rel,
-        "step_algorithmic_GBps": round(step_gbs, 1),
-        "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "baseline_formula": baseline_formula(cells, sweeps_total / args.steps, ms_per_step),
         "stage_ms_per_step": {k: round(v[0] / args.steps, 5)
                               for k, v in timing.items() if v[1]},
         "copy_probe_GBps": round(copy_gbs, 1),
+        "attainable_copy_GBps": round(attainable, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -77,6 +77,11 @@ typedef struct nx_params {
   int check_every;       /* sweeps between residual checks */
   int correction_sweeps; /* reduced storage: float32 residual-correction
                             sweeps after the solve (0..16, default 8) */
+  int jacobi_seg;        /* 0 (default): the Jacobi tile kernel; > 0 (float32
+                            single grids): the streaming kernel, each
+                            workgroup walking jacobi_seg output rows of a
+                            strip (shapes as nx_set_smoother_tile); the
+                            same sweeps bitwise */
 } nx_params;
 
 /* Statistics of the last pressure solve. */
@@ -192,15 +197,27 @@ int nx_smooth(nx_sim* sim, int sweeps, float omega, double* out);
  * 12.1, from the current pressure): every mode of -L above lambda_min is
  * damped by 1 / T_n((8 + lambda_min) / (8 - lambda_min)) and none grows;
  * then the FP64 relative residual, as nx_smooth (same out).  tile (may be
- * NULL: {8, 4, 8, 2}) = {stencil passes per launch, rows per wave, waves
- * per block, columns per lane}; shapes (2,16,2), (4,8,2), (4,16,2),
- * (8,8,1), (8,16,1), at most 32 passes, 64 cols - 2 passes >= 8 and
- * rows waves - 2 passes >= 8.  Replaces the Jacobi smoothing of the
- * learned cycle; no reference counterpart (the reference smooths only by
- * Jacobi, shader.Poisson.comp:24-37).  Counted in the step's sweeps (one
- * per iteration). */
+ * NULL: the simulator's smoother tile, see below) = {stencil passes per
+ * launch, rows per wave, waves per block, columns per lane} of the tile
+ * kernel; shapes (2,16,2), (4,8,2), (4,16,2), (8,8,1), (8,16,1), at most
+ * 32 passes, 64 cols - 2 passes >= 8 and rows waves - 2 passes >= 8.
+ * Replaces the Jacobi smoothing of the learned cycle; no reference
+ * counterpart (the reference smooths only by Jacobi,
+ * shader.Poisson.comp:24-37).  Counted in the step's sweeps (one per
+ * iteration). */
 int nx_smooth_chebyshev(nx_sim* sim, int iterations, float lambda_min, const int* tile,
                         double* out);
+/* the simulator's smoother tile, used when nx_smooth_chebyshev or
+ * nx_learned_cycles get tile == NULL: n = 4 values as nx_smooth_chebyshev's
+ * tile (tile kernel), or n = 5 with a fifth value seg > 0: the streaming
+ * kernel, whose workgroups each walk `seg` output rows of a (64 cols)-column
+ * strip in chunks of rows x waves rows (shapes (4,8,2), (2,16,2), (4,16,2),
+ * (8,8,2); at most 16 passes per launch, 64 cols - 2 passes >= 8); the
+ * same results bitwise.  Default {8, 4, 8, 2}.  nx_get_smoother_tile
+ * writes the 5 values (seg 0 for the tile kernel).  No reference
+ * counterpart (launch shape of the smoother). */
+int nx_set_smoother_tile(nx_sim* sim, const int* tile, int n);
+int nx_get_smoother_tile(nx_sim* sim, int* tile);
 /* `count` damped Jacobi sweeps p <- p + omega (J(p) - p) of the current
  * pressure system (0 < omega <= 1; omega = 1/2 removes the checkerboard
  * modes the plain sweep cannot damp); counted in the step's sweeps */
@@ -242,9 +259,10 @@ int nx_stage_clear_obstacles(nx_sim* sim);          /* ClearBuffer on obstacles 
 int nx_stage_advect_dye(nx_sim* sim, float dt);     /* AdvectParticle + flip */
 /* fused middle stage: curl, confinement, viscosity, divergence, p1 */
 int nx_stage_fused_vort_div(nx_sim* sim, float dt);
-/* `sweeps` Jacobi sweeps with the temporal-blocked kernel, k per launch */
+/* `sweeps` Jacobi sweeps with the temporal-blocked kernel, k per launch;
+ * seg = 0: the tile kernel, seg > 0: the streaming kernel (nx_params) */
 int nx_jacobi_blocked(nx_sim* sim, int sweeps, int k, int rows, int waves,
-                      int cols);
+                      int cols, int seg);
 
 /* ---- state access ---- */
 int nx_upload(nx_sim* sim, int field, const void* src, size_t bytes);
@@ -327,13 +345,17 @@ int nx_batch_collect_timing(nx_batch* b, double* ms, int* launches);
 int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                    int h, int wd, int cin, int cout, int relu, void* stream);
 /* the same convolution with the 9 cin / 16 K steps split over `splits`
- * groups of launches that write float32 partial sums to
- * work[splits * n*h*w * cout] (device), then summed in split order with
- * bias and ReLU: more workgroups for small images with many channels
- * (cin % 16 == 0) */
+ * groups of workgroups that write float32 partial sums to
+ * work[splits * n*h*w * cout] (device); the last group of each output tile
+ * to finish sums the tile's partials in split order with bias and ReLU
+ * (one launch).  counters (device, nx_conv3x3_split_counters(n, h, w, cout)
+ * unsigned ints) must be zero on entry and are zero again on return.  More
+ * workgroups for small images with many channels (cin % 16 == 0). */
 int nx_conv3x3_fwd_split(const void* x, const void* w, const float* bias, void* y,
-                         float* work, int splits, int n, int h, int wd, int cin, int cout,
-                         int relu, void* stream);
+                         float* work, unsigned* counters, int splits, int n, int h, int wd,
+                         int cin, int cout, int relu, void* stream);
+/* number of arrival counters nx_conv3x3_fwd_split needs (0 for bad shapes) */
+long long nx_conv3x3_split_counters(int n, int h, int wd, int cout);
 /* dx = conv3x3 input gradient of dy; wt: scratch of 9*cin*cout bfloat16 */
 int nx_conv3x3_dgrad(const void* dy, const void* w, void* dx, void* wt, int n, int h,
                      int wd, int cin, int cout, void* stream);
@@ -410,6 +432,11 @@ int nx_learned_cycles(nx_sim* sim, nx_unet* net, int factor, int coarse_iters,
                       double* records, int records_capacity, int* result);
 /* the runner's coarse buffers (rhs, mask, correction), for inspection */
 int nx_unet_coarse(nx_unet* net, float** rc, uint8_t** mc, float** e);
+/* bypass != 0: the runner's corrections start the coarse smoothing from
+ * zero instead of the network's output (the same cycle without the
+ * network, to measure what the network contributes); 0 restores it.  No
+ * reference counterpart. */
+int nx_unet_set_bypass(nx_unet* net, int bypass);
 
 /* ---- FP64 oracle solve on the GPU (conjugate gradients on the same
  *      operator, fluid cells, right-hand side centred on its fluid mean) ---- */
@@ -417,6 +444,20 @@ int nx_unet_coarse(nx_unet* net, float** rc, uint8_t** mc, float** e);
  * out[0] = true relative residual of x, out[1] = iterations */
 int nx_fp64_solve(const float* b, const uint8_t* mask, int W, int H, double* x,
                   double tol, int max_iter, double* out, void* stream);
+/* FP64 conjugate gradients on `count` independent n x n problems
+ * (n * n <= 16384) stored back to back: b (float32) and mask (uint8, != 0
+ * solid) of count * n * n cells each, device.  One workgroup per problem
+ * runs the whole solve (same operator, centring and mean projection as
+ * nx_fp64_solve) until the relative residual is <= tol or max_iter
+ * iterations.  xf (float32, device) gets each problem's solution with its
+ * fluid mean removed (0 on solid cells), residuals[count] and
+ * iterations[count] (device) the true relative residual of the FP64
+ * solution and the iterations run; work: count * n * n doubles (device).
+ * Used to make the learned guess's training targets; no reference
+ * counterpart. */
+int nx_fp64_solve_batch(const float* b, const uint8_t* mask, int n, int count, double* work,
+                        float* xf, double tol, int max_iter, double* residuals,
+                        int* iterations, void* stream);
 /* relative L2 error of the mean-free float32 p against the mean-free x */
 int nx_fp64_compare(const float* p, const double* x, const uint8_t* mask, int W, int H,
                     double* rel, void* stream);

@@ -58,6 +58,7 @@ class Params(C.Structure):
         ("max_iters", C.c_int),
         ("check_every", C.c_int),
         ("correction_sweeps", C.c_int),
+        ("jacobi_seg", C.c_int),
     ]
 
 
@@ -108,7 +109,7 @@ SIGNATURES = {
     "nx_stage_clear_obstacles": [_P],
     "nx_stage_advect_dye": [_P, _F],
     "nx_stage_fused_vort_div": [_P, _F],
-    "nx_jacobi_blocked": [_P, _I, _I, _I, _I, _I],
+    "nx_jacobi_blocked": [_P, _I, _I, _I, _I, _I, _I],
     "nx_upload": [_P, _I, _P, _SZ],
     "nx_download": [_P, _I, _P, _SZ],
     "nx_field_ptr": [_P, _I, C.POINTER(_P), C.POINTER(_SZ)],
@@ -145,6 +146,8 @@ SIGNATURES = {
     "nx_damped_sweeps": [_P, _I, _F],
     "nx_smooth": [_P, _I, _F, _P],
     "nx_smooth_chebyshev": [_P, _I, _F, _P, _P],
+    "nx_set_smoother_tile": [_P, C.POINTER(_I), _I],
+    "nx_get_smoother_tile": [_P, C.POINTER(_I)],
     "nx_coarse_smooth": [_P, _I, _I, _P, _P, _P, _I, _F],
     "nx_refine_begin": [_P],
     "nx_refine_residual": [_P, C.POINTER(_D)],
@@ -152,7 +155,8 @@ SIGNATURES = {
     "nx_refine_end": [_P],
     "nx_step_finish": [_P],
     "nx_conv3x3_fwd": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
-    "nx_conv3x3_fwd_split": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
+    "nx_conv3x3_fwd_split": [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
+    "nx_conv3x3_split_counters": [_I, _I, _I, _I],
     "nx_unet_create": [_I, _I, _I, _P, _P, _P, _P, _P],
     "nx_unet_destroy": [_P],
     "nx_unet_forward": [_P, _P, _P, _P],
@@ -162,6 +166,7 @@ SIGNATURES = {
     "nx_learned_cycles": [_P, _P, _I, _I, _F, _I, _F, C.POINTER(_I), _D, _D, _I,
                           C.POINTER(_D), _I, C.POINTER(_I)],
     "nx_unet_coarse": [_P, _P, _P, _P],
+    "nx_unet_set_bypass": [_P, _I],
     "nx_get_size": [_P, _P, _P],
     "nx_conv3x3_dgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
     "nx_conv3x3_wgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
@@ -174,11 +179,12 @@ SIGNATURES = {
     "nx_head_bwd": [_P, _P, _P, _P, _P, _P, C.c_longlong, _I, _P],
     "nx_fp64_solve": [_P, _P, _I, _I, _P, _D, _I, C.POINTER(_D), _P],
     "nx_fp64_compare": [_P, _P, _P, _I, _I, C.POINTER(_D), _P],
+    "nx_fp64_solve_batch": [_P, _P, _I, _I, _P, _P, _D, _I, _P, _P, _P],
     "nx_copy_bandwidth": [_SZ, _I, C.POINTER(_D)],
     "nx_copy_bandwidth_variant": [_SZ, _I, _I, C.POINTER(_D)],
     "nx_last_error": [],
 }
-_RESTYPE = {"nx_last_error": C.c_char_p}
+_RESTYPE = {"nx_last_error": C.c_char_p, "nx_conv3x3_split_counters": C.c_longlong}
 
 _lock = threading.Lock()
 _lib = None

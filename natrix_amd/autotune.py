@@ -68,6 +68,26 @@ def lookup(W, H, dtype="f32"):
     return best if best is not None else DEFAULT
 
 
+SMOOTHER_DEFAULT = (8, 4, 8, 2)
+
+
+def lookup_smoother(W, H):
+    """Tuned launch shape of the fine Chebyshev smoother for a W x H grid
+    (the entry of that grid, else of the nearest cell count, else the
+    library's default tile)."""
+    table = _load().get(ARCH, {}).get("cheb", {})
+    key = f"{W}x{H}"
+    if key in table:
+        return tuple(table[key]["tile"])
+    best, dist = None, None
+    for k, v in table.items():
+        w, h = (int(t) for t in k.split("x"))
+        d = abs(w * h - W * H)
+        if dist is None or d < dist:
+            best, dist = tuple(v["tile"]), d
+    return best if best is not None else SMOOTHER_DEFAULT
+
+
 def tune(W, H, dtype="f32", sweeps_per_candidate=None, log=print):
     """Time every candidate on a W x H scene; returns (best tile, results)."""
     import numpy as np

@@ -124,8 +124,10 @@ class FluidSimulator:
         self.max_iters = 1_000_000
         self.check_every = 256
         self.correction_sweeps = 8
-        from ..autotune import lookup
+        from ..autotune import lookup, lookup_smoother
         self.jacobi_tile = lookup(self._width, self._height, storage)
+        if storage == "f32":
+            self.smoother_tile = lookup_smoother(self._width, self._height)
         self._dye_speed = 500.0
         self._dye_dissipation = 1.0
         self._dye_in_step = True
@@ -240,8 +242,9 @@ class FluidSimulator:
         p.dye_dissipation = self._dye_dissipation
         p.dye_in_step = 1 if self._dye_in_step else 0
         p.fused = 1 if self.fused else 0
-        p.jacobi_k, p.jacobi_rows, p.jacobi_waves, p.jacobi_cols = (
-            int(v) for v in self.jacobi_tile)
+        tile = tuple(int(v) for v in self.jacobi_tile)
+        p.jacobi_k, p.jacobi_rows, p.jacobi_waves, p.jacobi_cols = tile[:4]
+        p.jacobi_seg = tile[4] if len(tile) > 4 else 0
         p.tol = float(self.tol)
         p.max_iters = int(self.max_iters)
         p.check_every = int(self.check_every)
@@ -493,11 +496,28 @@ class FluidSimulator:
         else:
             raise ValueError(f"unknown stage {name!r}")
 
-    def jacobi_blocked(self, sweeps, k, rows, waves, cols=1):
+    @property
+    def smoother_tile(self):
+        """Launch shape of the fine Chebyshev smoother of the learned step:
+        (passes per launch, rows, waves, cols) of the tile kernel, or with a
+        fifth value seg > 0 the streaming kernel (nx_set_smoother_tile)."""
+        t = (C.c_int * 5)()
+        _lib.call("nx_get_smoother_tile", self._alive(), t)
+        return tuple(t) if t[4] > 0 else tuple(t)[:4]
+
+    @smoother_tile.setter
+    def smoother_tile(self, tile):
+        tile = tuple(int(v) for v in tile)
+        _lib.call("nx_set_smoother_tile", self._alive(), (C.c_int * 5)(*tile, *([0] * (5 - len(tile)))),
+                  len(tile))
+
+    def jacobi_blocked(self, sweeps, k, rows, waves, cols=1, seg=0):
         """``sweeps`` Jacobi sweeps with the temporal-blocked kernel, k per
-        launch, on a (64*cols) x (rows*waves) register tile."""
+        launch, on a (64*cols) x (rows*waves) register tile (seg = 0), or
+        with the streaming kernel whose workgroups each walk `seg` output
+        rows of a (64*cols)-column strip in chunks of rows*waves rows."""
         _lib.call("nx_jacobi_blocked", self._alive(), int(sweeps), int(k),
-                  int(rows), int(waves), int(cols))
+                  int(rows), int(waves), int(cols), int(seg))
 
     # ------------------------------------------------------------ timing
     def set_timing(self, on=True):

@@ -94,6 +94,9 @@ struct nx_sim {
   size_t cws_cap = 0;
   size_t rrow_cap = 0;
   double* line = nullptr;   // {alpha, d.r, d.Ld, 0}
+  // fine Chebyshev smoother's launch shape when the caller passes no tile
+  // (nx_set_smoother_tile): tile kernel when seg == 0, streaming otherwise
+  JacobiCfg cheb_cfg{8, 4, 8, 2, 0};
   bool refining = false;
   // step phases (nx_step_begin .. nx_step_finish)
   bool in_step = false;
@@ -173,9 +176,9 @@ static int validate(const nx_sim* s, const nx_params& p) {
   if (p.bc_mode < 0 || p.bc_mode > 2) return fail(NX_ERR_ARG, "unknown bc_mode");
   if (p.bc_mode != BC_REFERENCE && s && (s->W < 3 || s->H < 3))
     return fail(NX_ERR_ARG, "free-slip and inflow walls need a grid of at least 3x3");
-  JacobiCfg c{p.jacobi_k, p.jacobi_rows, p.jacobi_waves, p.jacobi_cols};
-  if (!jacobi_cfg_supported(c))
-    return fail(NX_ERR_ARG, "unsupported Jacobi tile (k, rows, waves, cols)");
+  JacobiCfg c{p.jacobi_k, p.jacobi_rows, p.jacobi_waves, p.jacobi_cols, p.jacobi_seg};
+  if (p.jacobi_seg < 0 || !jacobi_cfg_supported(c))
+    return fail(NX_ERR_ARG, "unsupported Jacobi tile (k, rows, waves, cols, seg)");
   if (!(p.tol >= 0)) return fail(NX_ERR_ARG, "tol must be >= 0");
   if (p.tol > 0 && !p.fused)
     return fail(NX_ERR_ARG, "the tolerance-driven solve needs the fused path");
@@ -184,6 +187,8 @@ static int validate(const nx_sim* s, const nx_params& p) {
     return fail(NX_ERR_ARG, "correction_sweeps must be in [0, 16]");
   if (s && s->storage != ST_F32) {
     if (!p.fused) return fail(NX_ERR_ARG, "reduced-precision storage runs the fused path only");
+    if (p.jacobi_seg != 0)
+      return fail(NX_ERR_ARG, "the streaming Jacobi kernel needs float32 storage");
     if (p.tol > 0)
       return fail(NX_ERR_ARG, "the tolerance-driven solve needs float32 storage");
     JacobiCfg cr{1 + p.correction_sweeps, p.jacobi_rows, p.jacobi_waves, p.jacobi_cols};
@@ -318,7 +323,7 @@ static int exact_residual(nx_sim* s, double* rel) {
 // can move the last digits); iteration resumes if it is above tol.
 static int solve_tolerance(nx_sim* s, int sweeps_done, bool from_zero) {
   const nx_params& P = s->prm;
-  JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols};
+  JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols, P.jacobi_seg};
   NX_TRY(ensure_red2(s, c));
   const int n = s->W * s->H;
   NX_HIP(launch_sums(nullptr, s->div, s->mask[s->mcur], s->W, s->H, s->red5,
@@ -423,7 +428,7 @@ static int step_mixed_enqueue(nx_sim* s, float dt) {
                                  s->div, s->p[0], W, H, dt, P.vorticity, viscous,
                                  alpha, rbeta, s->stream));
   s->pcur = 0;
-  JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols};
+  JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols, P.jacobi_seg};
   int sweeps = P.iterations - 1;
   while (sweeps > 0) {
     JacobiCfg cc = c;
@@ -486,7 +491,7 @@ static int step_fused_enqueue(nx_sim* s, float dt, bool tolerance) {
   if (tolerance) {
     NX_TRY(solve_tolerance(s, 1, true));
   } else {
-    JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols};
+    JacobiCfg c{P.jacobi_k, P.jacobi_rows, P.jacobi_waves, P.jacobi_cols, P.jacobi_seg};
     NX_TRY(run_jacobi_tb(s, P.iterations - 1, c));
     s->stats.sweeps = P.iterations;
     s->stats.converged = 0;
@@ -1277,6 +1282,7 @@ int nx_default_params(nx_params* out) {
   p.max_iters = 1000000;
   p.check_every = 256;
   p.correction_sweeps = 8;
+  p.jacobi_seg = 0;
   *out = p;
   return NX_OK;
 }
@@ -1650,7 +1656,8 @@ int nx_solve(nx_sim* s, double tol, int max_sweeps, nx_stats* out) {
   nx_params saved = s->prm;
   s->prm.tol = tol;
   s->prm.max_iters = max_sweeps;
-  JacobiCfg c{s->prm.jacobi_k, s->prm.jacobi_rows, s->prm.jacobi_waves, s->prm.jacobi_cols};
+  JacobiCfg c{s->prm.jacobi_k, s->prm.jacobi_rows, s->prm.jacobi_waves, s->prm.jacobi_cols,
+              s->prm.jacobi_seg};
   int rc = ensure_red2(s, c);
   int sweeps = 0;
   bool converged = false;
@@ -1683,7 +1690,8 @@ int nx_smooth(nx_sim* s, int sweeps, float omega, double* out) {
   if (!s->in_step) return fail(NX_ERR_STATE, "nx_smooth outside nx_step_begin/finish");
   if (sweeps < 0 || !(omega >= 0.0f && omega <= 1.0f))
     return fail(NX_ERR_ARG, "sweeps >= 0 and 0 <= omega <= 1 required");
-  JacobiCfg c{s->prm.jacobi_k, s->prm.jacobi_rows, s->prm.jacobi_waves, s->prm.jacobi_cols};
+  JacobiCfg c{s->prm.jacobi_k, s->prm.jacobi_rows, s->prm.jacobi_waves, s->prm.jacobi_cols,
+              s->prm.jacobi_seg};
   NX_TRY(run_jacobi_tb(s, sweeps, c));
   s->step_sweeps += sweeps;
   if (omega > 0.0f) NX_TRY(nx_damped_sweeps(s, 1, omega));
@@ -1717,16 +1725,20 @@ static int smooth_cheb_impl(nx_sim* s, int iterations, float lambda_min, const i
   if (!s->in_step) return fail(NX_ERR_STATE, "nx_smooth_chebyshev outside nx_step_begin/finish");
   if (iterations < 1 || !(lambda_min > 0.0f && lambda_min < 8.0f))
     return fail(NX_ERR_ARG, "iterations >= 1 and 0 < lambda_min < 8 required");
-  const JacobiCfg c = tile ? JacobiCfg{tile[0], tile[1], tile[2], tile[3]}
-                          : JacobiCfg{8, 4, 8, 2};
-  if (!cheb_cfg_supported(c)) return fail(NX_ERR_ARG, "unsupported Chebyshev tile");
+  const JacobiCfg c = tile ? JacobiCfg{tile[0], tile[1], tile[2], tile[3]} : s->cheb_cfg;
+  const bool stream = c.seg > 0;
+  if (stream ? !stream_cfg_supported(c) : !cheb_cfg_supported(c))
+    return fail(NX_ERR_ARG, "unsupported Chebyshev tile");
   if (!s->fcheb) NX_HIP(hipMalloc(&s->fcheb, (size_t)4 * s->W * s->H * sizeof(float)));
   // x ping-pongs between the pressure buffers, (r, d) between the halves of
   // fcheb; the first launch spends one pass on the setup, the last one pass
   // on the FP64 residual of the result (records in cheb_red, one per block)
   const size_t n = (size_t)s->W * s->H;
   float* rd[2][2] = {{s->fcheb, s->fcheb + n}, {s->fcheb + 2 * n, s->fcheb + 3 * n}};
-  const size_t red_need = (size_t)jacobi_launch_blocks(s->W, s->H, c) * 5;
+  // a launch with fewer passes has a wider interior, so no launch of the
+  // run has more workgroups than one with c.k passes
+  const size_t red_need =
+      (size_t)(stream ? stream_launch_blocks(s->W, s->H, c) : jacobi_launch_blocks(s->W, s->H, c)) * 5;
   if (red_need > s->cheb_red_cap) {
     if (s->cheb_red) NX_HIP(hipFree(s->cheb_red));
     s->cheb_red = nullptr;
@@ -1743,15 +1755,27 @@ static int smooth_cheb_impl(nx_sim* s, int iterations, float lambda_min, const i
     const int its = cc.k - setup - (last ? 1 : 0);
     ChebCoef cf{};
     rec.next(setup, setup + its, cf);
-    NX_LAUNCH_TIMED(s, NX_TIME_CHEB,
-                    launch_cheb_tb(s->stream, cc, s->p[s->pcur], rd[cur][0], rd[cur][1],
-                                   s->div, s->mask[s->mcur], s->p[1 - s->pcur],
-                                   last ? nullptr : rd[1 - cur][0], rd[1 - cur][1], s->W,
-                                   s->H, setup, rec.theta, cf,
-                                   last ? s->cheb_red : nullptr,
-                                   pending && setup ? s->dcorr : nullptr,
-                                   pending && setup ? s->line : nullptr));
-    if (last) last_blocks = jacobi_launch_blocks(s->W, s->H, cc);
+    if (stream)
+      NX_LAUNCH_TIMED(s, NX_TIME_CHEB,
+                      launch_cheb_stream(s->stream, cc, s->p[s->pcur], rd[cur][0], rd[cur][1],
+                                         s->div, s->mask[s->mcur], s->p[1 - s->pcur],
+                                         last ? nullptr : rd[1 - cur][0], rd[1 - cur][1], s->W,
+                                         s->H, setup, rec.theta, cf,
+                                         last ? s->cheb_red : nullptr,
+                                         pending && setup ? s->dcorr : nullptr,
+                                         pending && setup ? s->line : nullptr));
+    else
+      NX_LAUNCH_TIMED(s, NX_TIME_CHEB,
+                      launch_cheb_tb(s->stream, cc, s->p[s->pcur], rd[cur][0], rd[cur][1],
+                                     s->div, s->mask[s->mcur], s->p[1 - s->pcur],
+                                     last ? nullptr : rd[1 - cur][0], rd[1 - cur][1], s->W,
+                                     s->H, setup, rec.theta, cf,
+                                     last ? s->cheb_red : nullptr,
+                                     pending && setup ? s->dcorr : nullptr,
+                                     pending && setup ? s->line : nullptr));
+    if (last)
+      last_blocks = stream ? stream_launch_blocks(s->W, s->H, cc)
+                           : jacobi_launch_blocks(s->W, s->H, cc);
     s->pcur ^= 1;
     cur ^= 1;
     done += its;
@@ -1780,6 +1804,25 @@ static int smooth_cheb_impl(nx_sim* s, int iterations, float lambda_min, const i
 int nx_smooth_chebyshev(nx_sim* s, int iterations, float lambda_min, const int* tile,
                         double* out) {
   return smooth_cheb_impl(s, iterations, lambda_min, tile, out, false);
+}
+
+int nx_set_smoother_tile(nx_sim* s, const int* cfg, int n) {
+  if (!s || !cfg || (n != 4 && n != 5)) return fail(NX_ERR_ARG, "4 or 5 tile values required");
+  JacobiCfg c{cfg[0], cfg[1], cfg[2], cfg[3], n == 5 ? cfg[4] : 0};
+  if (c.seg < 0 || (c.seg > 0 ? !stream_cfg_supported(c) : !cheb_cfg_supported(c)))
+    return fail(NX_ERR_ARG, "unsupported Chebyshev tile");
+  s->cheb_cfg = c;
+  return NX_OK;
+}
+
+int nx_get_smoother_tile(nx_sim* s, int* cfg) {
+  if (!s || !cfg) return fail(NX_ERR_ARG, "null argument");
+  cfg[0] = s->cheb_cfg.k;
+  cfg[1] = s->cheb_cfg.rows;
+  cfg[2] = s->cheb_cfg.waves;
+  cfg[3] = s->cheb_cfg.cols;
+  cfg[4] = s->cheb_cfg.seg;
+  return NX_OK;
 }
 
 // restriction onto a Wp x Hp coarse grid, Wp >= ceil(W / f), Hp >= ceil(H / f)
@@ -2156,12 +2199,12 @@ int nx_stage_fused_vort_div(nx_sim* s, float dt) {
 }
 
 int nx_jacobi_blocked(nx_sim* s, int sweeps, int k, int rows, int waves,
-                      int cols) {
+                      int cols, int seg) {
   if (!s) return fail(NX_ERR_ARG, "null simulator");
   if (s->storage != ST_F32)
     return fail(NX_ERR_STATE, "this entry point needs float32 storage");
   if (sweeps < 0) return fail(NX_ERR_ARG, "sweeps must be >= 0");
-  JacobiCfg c{k, rows, waves, cols};
+  JacobiCfg c{k, rows, waves, cols, seg};
   if (!jacobi_cfg_supported(c))
     return fail(NX_ERR_ARG, "unsupported Jacobi tile (k, rows, waves, cols)");
   return run_jacobi_tb(s, sweeps, c);

@@ -478,6 +478,8 @@ int nx_batch_set_params(nx_batch* b, const nx_params* P) {
   if (!b || !P) return bfail(NX_ERR_ARG, "null argument");
   if (nx_check_params(P) != NX_OK) return NX_ERR_ARG;
   if (P->tol > 0) return bfail(NX_ERR_ARG, "the batched step runs a fixed sweep count");
+  if (P->jacobi_seg != 0)
+    return bfail(NX_ERR_ARG, "the batched step runs the Jacobi tile kernel (jacobi_seg 0)");
   b->prm = *P;
   for (auto& d : b->scenes) fill_scene(d, *P);
   b->scenes_dirty = true;
@@ -488,6 +490,8 @@ int nx_batch_set_scene_params(nx_batch* b, int scene, const nx_params* P) {
   NXB_TRY(check_scene(b, scene));
   if (!P) return bfail(NX_ERR_ARG, "null params");
   if (nx_check_params(P) != NX_OK) return NX_ERR_ARG;
+  if (P->jacobi_seg != 0)
+    return bfail(NX_ERR_ARG, "the batched step runs the Jacobi tile kernel (jacobi_seg 0)");
   fill_scene(b->scenes[scene], *P);
   b->scenes_dirty = true;
   return NX_OK;

@@ -191,6 +191,7 @@ hipError_t launch_tb(const float* pin, const float* b, const uint8_t* mask,
   X(8, 8, 2)
 
 bool jacobi_cfg_supported(JacobiCfg c) {
+  if (c.seg != 0) return c.seg > 0 && stream_cfg_supported(c);
   bool shape = false;
 #define NX_TB_CHECK(R, WV, CL) \
   shape |= (c.rows == R && c.waves == WV && c.cols == CL);
@@ -202,6 +203,7 @@ bool jacobi_cfg_supported(JacobiCfg c) {
 }
 
 int jacobi_launch_blocks(int W, int H, JacobiCfg c) {
+  if (c.seg > 0) return stream_launch_blocks(W, H, c);
   const int VX = 64 * c.cols - 2 * c.k, VY = c.rows * c.waves - 2 * c.k;
   return ((W + VX - 1) / VX) * ((H + VY - 1) / VY);
 }
@@ -211,6 +213,7 @@ hipError_t launch_jacobi_tb(const float* pin, const float* b,
                             JacobiCfg c, double* red, const int* done,
                             hipStream_t s) {
   if (!jacobi_cfg_supported(c)) return hipErrorInvalidValue;
+  if (c.seg > 0) return launch_jacobi_stream(pin, b, mask, pout, W, H, c, red, done, s);
 #define NX_TB_LAUNCH(R, WV, CL)                                              \
   if (c.rows == R && c.waves == WV && c.cols == CL)                          \
     return launch_tb<R, WV, CL>(pin, b, mask, pout, W, H, c.k, red, done, s);

@@ -69,10 +69,11 @@ hipError_t launch_fused_vort_div(const float2* uin, const uint8_t* mask,
 // red[2*block + {0,1}] (sum r, sum r^2 over fluid cells).  `done`
 // (nullable) is a device flag: when non-zero the launch returns at once.
 struct JacobiCfg {
-  int k;      // sweeps per launch
-  int rows;   // rows per wave
-  int waves;  // waves per block
-  int cols;   // columns per lane
+  int k;        // sweeps per launch
+  int rows;     // rows per wave
+  int waves;    // waves per block
+  int cols;     // columns per lane
+  int seg = 0;  // > 0: streaming kernel, output rows per workgroup segment
 };
 int jacobi_launch_blocks(int W, int H, JacobiCfg cfg);
 hipError_t launch_jacobi_tb(const float* pin, const float* b,
@@ -117,6 +118,22 @@ struct ChebRecurrence {
   void next(int first, int passes, ChebCoef& cf);
 };
 bool cheb_cfg_supported(JacobiCfg c);
+// Streaming form of the same passes (nx_stream.hip): a workgroup walks a
+// (64 cols)-column strip over `seg` output rows in chunks of rows x waves
+// rows, carrying the two rows above each chunk per pass in LDS (no vertical
+// halo re-reads).  Bitwise equal to launch_cheb_tb.  Records (red) are per
+// workgroup: stream_launch_blocks of the launch's shape.
+constexpr int STREAM_KMAX = 16;
+bool stream_cfg_supported(JacobiCfg c);
+int stream_launch_blocks(int W, int H, JacobiCfg c);
+hipError_t launch_jacobi_stream(const float* pin, const float* b, const uint8_t* mask,
+                                float* pout, int W, int H, JacobiCfg c, double* red,
+                                const int* done, hipStream_t st);
+hipError_t launch_cheb_stream(hipStream_t st, JacobiCfg c, const float* xin, const float* rin,
+                              const float* din, const float* b, const uint8_t* mask,
+                              float* xout, float* rout, float* dout, int W, int H, int setup,
+                              float theta, const ChebCoef& cf, double* red,
+                              const float* dapp = nullptr, const double* line = nullptr);
 hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const float* rin,
                           const float* din, const float* b, const uint8_t* mask, float* xout,
                           float* rout, float* dout, int W, int H, int setup, float theta,
@@ -128,7 +145,8 @@ hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const f
 // as nx_conv3x3_fwd / nx_conv3x3_fwd_split with n = 1.
 int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t s);
 int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
-                      int splits, int h, int wd, int cin, int cout, int relu, hipStream_t s);
+                      unsigned* counters, int splits, int h, int wd, int cin, int cout,
+                      int relu, hipStream_t s);
 
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
 // sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 8

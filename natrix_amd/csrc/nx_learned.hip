@@ -55,6 +55,8 @@ struct nx_unet {
   uint8_t* mc = nullptr;   // coarse mask
   float* work = nullptr;   // split-K partial sums
   size_t work_elems = 0;
+  unsigned* cnt = nullptr;  // split-K arrival counters (zero between launches)
+  long long cnt_elems = 0;
   float* scal = nullptr;   // [0] = s (RMS of rc over fluid cells)
   float* cws = nullptr;    // Chebyshev workspace, 6 H W floats
   // steps 2-6 captured as one HIP graph (every pointer in them is the
@@ -65,6 +67,10 @@ struct nx_unet {
   hipStream_t g_stream = nullptr, w_stream = nullptr;
   int g_iters = -1, w_iters = -1;
   float g_lmin = 0.0f, w_lmin = 0.0f;
+  // bypass != 0: the coarse correction starts from zero instead of the
+  // network's output (the same cycle without the network, for comparison)
+  int bypass = 0;
+  int g_bypass = -1, w_bypass = -1;
 };
 
 namespace {
@@ -136,6 +142,11 @@ __global__ void k_scale_output(float* __restrict__ e, const float* __restrict__ 
   if (i < n) e[i] = e[i] * scal[0];
 }
 
+__global__ void k_zero_output(float* __restrict__ e, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) e[i] = 0.0f;
+}
+
 // e -= mean of e over fluid cells; e = 0 on solid cells
 __global__ void __launch_bounds__(1024)
     k_center_output(float* __restrict__ e, const uint8_t* __restrict__ mc, int n) {
@@ -169,11 +180,11 @@ int conv(nx_unet* u, int i, const void* x, void* y, int h, int w, hipStream_t s)
   const long long m = (long long)h * w;
   const int sp = conv_splits(m, u->cin[i], u->cout[i]);
   if (u->wt[i])
-    return nx::conv3x3_fwd_tiled(x, u->wt[i], u->b[i], y, u->work, sp, h, w, u->cin[i],
-                                 u->cout[i], 1, s);
+    return nx::conv3x3_fwd_tiled(x, u->wt[i], u->b[i], y, u->work, u->cnt, sp, h, w,
+                                 u->cin[i], u->cout[i], 1, s);
   if (sp > 1)
-    return nx_conv3x3_fwd_split(x, u->w[i], u->b[i], y, u->work, sp, 1, h, w, u->cin[i],
-                                u->cout[i], 1, s);
+    return nx_conv3x3_fwd_split(x, u->w[i], u->b[i], y, u->work, u->cnt, sp, 1, h, w,
+                                u->cin[i], u->cout[i], 1, s);
   return nx_conv3x3_fwd(x, u->w[i], u->b[i], y, 1, h, w, u->cin[i], u->cout[i], 1, s);
 }
 
@@ -216,12 +227,18 @@ int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s) {
 // centring (only the runner's buffers)
 int coarse_pipeline(nx_unet* u, int iters, float lmin, hipStream_t s) {
   const int n = u->H * u->W;
-  k_pack_input<<<1, 1024, 0, s>>>(u->rc, u->mc, u->x0, u->scal, n);
-  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "pack launch failed");
-  int rc = forward(u, u->x0, u->out, s);
-  if (rc != NX_OK) return rc;
-  k_scale_output<<<(n + 255) / 256, 256, 0, s>>>(u->out, u->scal, n);
-  if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "scale launch failed");
+  int rc = NX_OK;
+  if (u->bypass) {
+    k_zero_output<<<(n + 255) / 256, 256, 0, s>>>(u->out, n);
+    if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "zero launch failed");
+  } else {
+    k_pack_input<<<1, 1024, 0, s>>>(u->rc, u->mc, u->x0, u->scal, n);
+    if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "pack launch failed");
+    rc = forward(u, u->x0, u->out, s);
+    if (rc != NX_OK) return rc;
+    k_scale_output<<<(n + 255) / 256, 256, 0, s>>>(u->out, u->scal, n);
+    if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "scale launch failed");
+  }
   rc = nx::launch_coarse_smooth(s, u->W, u->H, u->out, u->rc, u->mc, iters, lmin, u->cws);
   if (rc != NX_OK) return rc;
   k_center_output<<<1, 1024, 0, s>>>(u->out, u->mc, n);
@@ -230,13 +247,15 @@ int coarse_pipeline(nx_unet* u, int iters, float lmin, hipStream_t s) {
 }
 
 int run_pipeline(nx_unet* u, int iters, float lmin, hipStream_t s) {
-  if (u->graph && u->g_stream == s && u->g_iters == iters && u->g_lmin == lmin)
+  if (u->graph && u->g_stream == s && u->g_iters == iters && u->g_lmin == lmin &&
+      u->g_bypass == u->bypass)
     return hipGraphLaunch(u->graph, s) == hipSuccess ? NX_OK
                                                      : fail(NX_ERR_HIP, "hipGraphLaunch failed");
-  if (u->w_stream != s || u->w_iters != iters || u->w_lmin != lmin) {
+  if (u->w_stream != s || u->w_iters != iters || u->w_lmin != lmin || u->w_bypass != u->bypass) {
     u->w_stream = s;
     u->w_iters = iters;
     u->w_lmin = lmin;
+    u->w_bypass = u->bypass;
     return coarse_pipeline(u, iters, lmin, s);
   }
   if (u->graph) (void)hipGraphExecDestroy(u->graph);
@@ -260,6 +279,7 @@ int run_pipeline(nx_unet* u, int iters, float lmin, hipStream_t s) {
   u->g_stream = s;
   u->g_iters = iters;
   u->g_lmin = lmin;
+  u->g_bypass = u->bypass;
   return hipGraphLaunch(u->graph, s) == hipSuccess ? NX_OK
                                                    : fail(NX_ERR_HIP, "hipGraphLaunch failed");
 }
@@ -300,6 +320,8 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
     const int sp = conv_splits(m, ci[i], co[i]);
     if (sp > 1 && (size_t)(sp * m * co[i]) > u->work_elems)
       u->work_elems = (size_t)sp * m * co[i];
+    const long long tiles = nx_conv3x3_split_counters(1, H >> lvl[i], W >> lvl[i], co[i]);
+    if (sp > 1 && tiles > u->cnt_elems) u->cnt_elems = tiles;
   }
   u->head_w = head_w;
   u->head_b = head_b;
@@ -342,6 +364,11 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
   u->rc = (float*)alloc(P0 * sizeof(float));
   u->mc = (uint8_t*)alloc(P0);
   u->work = (float*)alloc((u->work_elems ? u->work_elems : 1) * sizeof(float));
+  u->cnt = (unsigned*)alloc((u->cnt_elems ? u->cnt_elems : 1) * sizeof(unsigned));
+  // the counters start at zero; every split convolution leaves them there
+  if (ok && hipMemset(u->cnt, 0, (u->cnt_elems ? u->cnt_elems : 1) * sizeof(unsigned)) !=
+                hipSuccess)
+    ok = false;
   u->scal = (float*)alloc(4 * sizeof(float));
   u->cws = (float*)alloc(6 * P0 * sizeof(float));
   void* wcopy[14] = {};
@@ -478,6 +505,12 @@ int nx_learned_cycles(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
   result[1] = kept;
   result[2] = converged;
   result[3] = abandoned;
+  return NX_OK;
+}
+
+int nx_unet_set_bypass(nx_unet* u, int bypass) {
+  if (!u) return fail(NX_ERR_ARG, "null argument");
+  u->bypass = bypass != 0;
   return NX_OK;
 }
 

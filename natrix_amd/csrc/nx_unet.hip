@@ -52,13 +52,17 @@ __device__ __forceinline__ uint16_t tobf(float f) { return (uint16_t)nx::f32_to_
 // along M.  With part == nullptr (one split) the block runs the whole K loop
 // and writes bias + ReLU in bf16; otherwise (Cin % 16 == 0) split z runs
 // K steps [z * steps_per_split, ...) of 16 channels of one tap each and
-// writes its float32 partial sums to part[z][m][co] for k_conv_finish.
+// writes its float32 partial sums to part[z][m][co].  The last of a tile's
+// splits to finish (a per-tile arrival counter in cnt, zero on entry and
+// reset to zero by that block) then sums the tile's partials in split
+// order, adds the bias, applies ReLU and writes bf16: the same values, in
+// the same order, as a separate reduction pass would, with no extra launch.
 template <int MT, int NT, bool WT>
 __global__ void __launch_bounds__(256)
     k_conv3x3(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
               const float* __restrict__ bias, uint16_t* __restrict__ y, int N,
               int H, int W, int Cin, int Cout, int relu, float* __restrict__ part,
-              int steps_per_split) {
+              int steps_per_split, unsigned* __restrict__ cnt) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 31, kb = lane >> 5;
   const long long M = (long long)N * H * W;
@@ -168,7 +172,8 @@ __global__ void __launch_bounds__(256)
   }
 
   if (part) {
-    float* pz = part + (long long)blockIdx.z * M * Cout;
+    const long long MC = M * Cout;
+    float* pz = part + (long long)blockIdx.z * MC;
 #pragma unroll
     for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -178,6 +183,43 @@ __global__ void __launch_bounds__(256)
         for (int e = 0; e < 16; ++e) {
           const long long m = m_base + 32 * i + 8 * (e >> 2) + 4 * kb + (e & 3);
           if (m < M) pz[m * Cout + co] = acc[i][j][e];
+        }
+      }
+    // arrival: this block's partials are visible device-wide before its
+    // count; the block that brings the count to gridDim.z reduces the tile
+    __shared__ unsigned last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned* c = cnt + blockIdx.y * gridDim.x + blockIdx.x;
+      const unsigned before = __hip_atomic_fetch_add(c, 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+      last = before + 1 == gridDim.z;
+      if (last) __hip_atomic_store(c, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const float* pr = part;
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int co = n_base + 32 * j + r;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const long long m = m_base + 32 * i + 8 * (e >> 2) + 4 * kb + (e & 3);
+          if (m < M) {
+            const long long idx = m * Cout + co;
+            float v = 0.0f;
+            for (int z = 0; z < (int)gridDim.z; ++z)
+              v += __hip_atomic_load(pr + z * MC + idx, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+            v += bv;
+            if (relu) v = fmaxf(v, 0.0f);
+            y[idx] = tobf(v);
+          }
         }
       }
     return;
@@ -198,19 +240,6 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
-}
-
-// y[m][co] = bf16(relu(sum over splits z in order of part[z][m][co] + bias))
-__global__ void __launch_bounds__(256)
-    k_conv_finish(const float* __restrict__ part, const float* __restrict__ bias,
-                  uint16_t* __restrict__ y, long long MC, int Cout, int splits, int relu) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= MC) return;
-  float v = 0.0f;
-  for (int z = 0; z < splits; ++z) v += part[z * MC + i];
-  v += bias ? bias[i % Cout] : 0.0f;
-  if (relu) v = fmaxf(v, 0.0f);
-  y[i] = tobf(v);
 }
 
 // w[co][tap][ci] -> w'[ci][8 - tap][co]
@@ -442,7 +471,7 @@ extern "C" {
 
 static int conv_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                     int h, int wd, int cin, int cout, int relu, float* part, int splits,
-                    hipStream_t s, bool tiled = false) {
+                    hipStream_t s, bool tiled = false, unsigned* cnt = nullptr) {
   const long long M = (long long)n * h * wd;
   const int steps = 9 * cin / 16;
   const int per = part ? (steps + splits - 1) / splits : 0;
@@ -454,25 +483,25 @@ static int conv_fwd(const void* x, const void* w, const float* bias, void* y, in
     dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64, used);
     if (tiled)
       k_conv3x3<2, 2, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
-                                                 part, per);
+                                                 part, per, cnt);
     else
       k_conv3x3<2, 2, false><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
-                                                  part, per);
+                                                  part, per, cnt);
   } else {
     dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 32, used);
     if (tiled)
       k_conv3x3<2, 1, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
-                                                 part, per);
+                                                 part, per, cnt);
     else
       k_conv3x3<2, 1, false><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
-                                                  part, per);
-  }
-  if (part) {
-    const long long mc = M * cout;
-    k_conv_finish<<<(unsigned)((mc + 255) / 256), 256, 0, s>>>(part, bias, (uint16_t*)y, mc,
-                                                               cout, used, relu);
+                                                  part, per, cnt);
   }
   return 0;
+}
+
+// arrival counters a split forward convolution needs (one per output tile)
+static long long conv_tiles(long long M, int cout) {
+  return ((M + 4 * 64 - 1) / (4 * 64)) * (cout % 64 == 0 ? cout / 64 : cout / 32);
 }
 
 // w[co][tap][ci] (KRSC) -> wt[tap][ci / 16][co][16] for the WT convolution
@@ -497,14 +526,20 @@ int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int
 }
 
 int nx_conv3x3_fwd_split(const void* x, const void* w, const float* bias, void* y,
-                         float* work, int splits, int n, int h, int wd, int cin, int cout,
-                         int relu, void* stream) {
-  if (!x || !w || !y || !work || n < 1 || h < 1 || wd < 1 || cin % 16 || cout % 32 ||
-      splits < 1 || splits > 9 * cin / 16)
+                         float* work, unsigned* counters, int splits, int n, int h, int wd,
+                         int cin, int cout, int relu, void* stream) {
+  if (!x || !w || !y || !work || !counters || n < 1 || h < 1 || wd < 1 || cin % 16 ||
+      cout % 32 || splits < 1 || splits > 9 * cin / 16)
     return nx::set_error(NX_ERR_ARG,
                          "conv3x3 split: cin % 16 == 0, cout % 32 == 0, 1 <= splits <= 9 cin / 16");
-  conv_fwd(x, w, bias, y, n, h, wd, cin, cout, relu, work, splits, NX_U(stream));
+  conv_fwd(x, w, bias, y, n, h, wd, cin, cout, relu, work, splits, NX_U(stream), false,
+           counters);
   NX_UNET_RET();
+}
+
+long long nx_conv3x3_split_counters(int n, int h, int wd, int cout) {
+  if (n < 1 || h < 1 || wd < 1 || cout < 32 || cout % 32) return 0;
+  return conv_tiles((long long)n * h * wd, cout);
 }
 
 int nx_conv3x3_dgrad(const void* dy, const void* w, void* dx, void* wt, int n, int h,
@@ -622,12 +657,13 @@ int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t
 }
 
 int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
-                      int splits, int h, int wd, int cin, int cout, int relu, hipStream_t s) {
+                      unsigned* counters, int splits, int h, int wd, int cin, int cout,
+                      int relu, hipStream_t s) {
   if (!x || !wt || !y || h < 1 || wd < 1 || cin % 16 || cout % 32 || splits < 1 ||
-      splits > 9 * cin / 16 || (splits > 1 && !work))
+      splits > 9 * cin / 16 || (splits > 1 && (!work || !counters)))
     return set_error(NX_ERR_ARG, "tiled conv3x3: cin % 16 == 0, cout % 32 == 0, valid splits");
   conv_fwd(x, wt, bias, y, 1, h, wd, cin, cout, relu, splits > 1 ? work : nullptr, splits, s,
-           true);
+           true, counters);
   NX_UNET_RET();
 }
 }  // namespace nx

@@ -1,8 +1,12 @@
 """Training pairs for the learned pressure guess, produced by the project's
-own batched float32 Jacobi solver.
+own solvers: FP64 conjugate gradients on the GPU, one workgroup per problem
+(nx_fp64_solve_batch, make_dataset), converged to a relative residual of
+1e-6 and stored as the float32 rounding of the FP64 solution; or, as the
+first weights were made, the batched float32 Jacobi solver run to its
+round-off floor (make_dataset_jacobi).
 
 Each sample is a coarse Poisson problem (the grid the U-Net runs on): a
-right-hand side b, a solid mask and the pressure p that Jacobi converged to.
+right-hand side b, a solid mask and the pressure p the solver converged to.
 The right-hand sides are smooth random fields (sums of Gaussian blobs of
 random width and sign plus low Fourier modes), like the restricted
 divergence and residual fields the solver hands the network; masks are
@@ -45,8 +49,46 @@ def random_problem(rng, n):
     return b.astype(np.float32), solid
 
 
-def make_dataset(count, n=128, seed=500000, tol=1e-6, max_sweeps=400000,
-                 chunk_sweeps=20000, per_batch=1024, tile=None, log=print):
+def make_dataset(count, n=128, seed=500000, tol=1e-6, max_iter=20000, per_batch=2048,
+                 log=print):
+    """Returns (b [count,n,n] f32, solid [count,n,n] bool, p [count,n,n] f32,
+    residual [count]): problem i from seed + i, p the float32 rounding of
+    its FP64 conjugate-gradient solution (fluid mean removed, 0 on solid
+    cells), residual the FP64 solution's true relative residual."""
+    import ctypes as C
+
+    import torch
+
+    from .. import _lib
+    bs, ss, ps, rs = [], [], [], []
+    done = 0
+    while done < count:
+        m = min(per_batch, count - done)
+        probs = [random_problem(np.random.default_rng(seed + done + i), n) for i in range(m)]
+        b = torch.from_numpy(np.stack([q[0] for q in probs])).cuda()
+        mask = torch.from_numpy(np.stack([q[1] for q in probs]).astype(np.uint8)).cuda()
+        work = torch.empty(m * n * n, dtype=torch.float64, device="cuda")
+        xf = torch.empty(m, n, n, dtype=torch.float32, device="cuda")
+        res = torch.empty(m, dtype=torch.float64, device="cuda")
+        it = torch.empty(m, dtype=torch.int32, device="cuda")
+        p = lambda t: C.c_void_p(t.data_ptr())
+        _lib.call("nx_fp64_solve_batch", p(b), p(mask), n, m, p(work), p(xf), float(tol),
+                  int(max_iter), p(res), p(it),
+                  C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        r = res.cpu().numpy()
+        log(f"  data batch {done // per_batch}: FP64 CG residual median {np.median(r):.2e} "
+            f"max {r.max():.2e}, iterations median {int(it.median())} max {int(it.max())}")
+        bs.append(b.cpu().numpy())
+        ss.append(np.stack([q[1] for q in probs]))
+        ps.append(xf.cpu().numpy())
+        rs.append(r)
+        done += m
+    return np.concatenate(bs), np.concatenate(ss), np.concatenate(ps), np.concatenate(rs)
+
+
+def make_dataset_jacobi(count, n=128, seed=500000, tol=1e-6, max_sweeps=400000,
+                        chunk_sweeps=20000, per_batch=1024, tile=None, log=print):
     """Returns (b [count,n,n] f32, solid [count,n,n] bool, p [count,n,n] f32,
     residual [count]) with p from Jacobi from zero on the GPU.
 

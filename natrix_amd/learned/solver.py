@@ -101,8 +101,13 @@ class LearnedGuess:
     correction, nx_learned_correct) unless `native` is False; other modules
     run through PyTorch."""
 
-    def __init__(self, net, coarse=128, coarse_iters=100, damping=1e-3, native=True):
+    def __init__(self, net, coarse=128, coarse_iters=100, damping=1e-3, native=True,
+                 use_network=True):
+        """use_network=False keeps the native runner but starts every coarse
+        smoothing from zero (nx_unet_set_bypass): the same cycle, launches
+        and timing path without the network's guess."""
         self.net = net.eval() if net is not None else None
+        self.use_network = bool(use_network)
         if coarse < 8 or coarse % 8:
             raise ValueError(f"coarse grid side {coarse} is not a positive multiple of 8")
         self.coarse = coarse
@@ -135,6 +140,8 @@ class LearnedGuess:
         r = self._runners.get((wc, hc))
         if r is None:
             r = self._runners[(wc, hc)] = NativeUNet(self.net, hc, wc)
+            if not self.use_network:
+                _lib.call("nx_unet_set_bypass", r.handle, 1)
         return r
 
     @torch.no_grad()
@@ -171,7 +178,7 @@ class LearnedGuess:
         fluid = mc == 0
         n = fluid.sum().clamp_min(1)
         s = torch.sqrt((rc * rc * fluid).sum() / n).clamp_min(1e-30)
-        if self.net is not None:
+        if self.net is not None and self.use_network:
             x = pack_input((rc / s).view(1, hc, wc), (~fluid).view(1, hc, wc))
             e = (self.net(x).view(-1) * s).contiguous()
         else:

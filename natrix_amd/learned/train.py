@@ -1,10 +1,10 @@
-"""Train the U-Net pressure guess on Jacobi-generated pairs.
+"""Train the U-Net pressure guess on pairs solved by FP64 conjugate gradients.
 
-    python -m natrix_amd.learned.train --samples 20000 --size 128 --epochs 6 \\
+    python -m natrix_amd.learned.train --samples 20480 --size 128 --epochs 10 \\
         --out natrix_amd/learned/unet_weights.pt
 
 Loss: mean squared error of the mean-free prediction against the mean-free
-Jacobi pressure over fluid cells (the operator fixes p only up to a
+target pressure over fluid cells (the operator fixes p only up to a
 constant), relative to the target's mean square.  The problem is invariant
 under the 8 symmetries of the square, so every batch is randomly flipped and
 transposed.
@@ -122,7 +122,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--max-sweeps", type=int, default=400000)
+    ap.add_argument("--max-iter", type=int, default=20000,
+                    help="FP64 conjugate-gradient iterations per training problem")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--time-budget", type=float, default=None,
                     help="stop training after this many seconds (weights are saved)")
@@ -130,17 +131,23 @@ def main(argv=None):
     from .data import make_dataset
     log = lambda m: print(m, flush=True)
     t0 = time.time()
-    b, solid, p, res = make_dataset(a.samples, a.size, tol=a.tol,
-                                    max_sweeps=a.max_sweeps, log=log)
+    b, solid, p, res = make_dataset(a.samples, a.size, tol=a.tol, max_iter=a.max_iter,
+                                    log=log)
+    edges = [0.0] + [10.0 ** e for e in range(-12, 0)] + [float("inf")]
+    hist, _ = np.histogram(res, bins=edges)
+    residual_hist = {f"<={edges[i + 1]:g}": int(c) for i, c in enumerate(hist) if c}
     log(json.dumps({"dataset": a.samples, "size": a.size, "seconds": round(time.time() - t0, 1),
                     "residual_median": float(np.median(res)), "residual_max": float(res.max()),
-                    "frac_at_tol": float((res <= a.tol).mean())}))
+                    "frac_at_tol": float((res <= a.tol).mean()),
+                    "residual_histogram": residual_hist}))
     net, hist = train(b, solid, p, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log,
                       time_budget=a.time_budget)
     meta = {"samples": a.samples, "size": a.size, "epochs": a.epochs,
-            "data_residual_median": float(np.median(res)),
-            "data_residual_max": float(res.max()), "history": hist,
-            "params": net.n_params()}
+            "data": "FP64 conjugate gradients (nx_fp64_solve_batch), float32 rounding of "
+                    "the FP64 solution",
+            "data_tol": a.tol, "data_residual_median": float(np.median(res)),
+            "data_residual_max": float(res.max()), "data_residual_histogram": residual_hist,
+            "history": hist, "params": net.n_params()}
     save(net, a.out, meta)
     log(json.dumps({"saved": a.out, "params": net.n_params()}))
 

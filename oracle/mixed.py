@@ -38,7 +38,12 @@ def round_f16(x):
 
 
 def rounder(storage):
+    """Rounding at a store: "f32" is the float32 step, "bf16" / "f16" the
+    reduced storage, and "f64" the identity in float64 (with float64 inputs
+    and dt, step_mixed is then the FP64 twin of the same dataflow, the
+    reference the reduced-storage error is measured against)."""
     return {"f32": lambda a: np.asarray(a, np.float32),
+            "f64": lambda a: np.asarray(a, np.float64),
             "bf16": round_bf16, "f16": round_f16}[storage]
 
 

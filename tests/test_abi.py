@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "natrix.h")
 
 def _declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(nx_\w+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|long long|const char\*)\s+(nx_\w+)\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -149,7 +149,7 @@ def _header_arity():
     declares (declarations may span lines; `(void)` has none)."""
     txt = open(HEADER).read()
     out = {}
-    for m in re.finditer(r"^(?:int|const char\*)\s+(nx_\w+)\(([^;]*?)\);", txt, re.M | re.S):
+    for m in re.finditer(r"^(?:int|long long|const char\*)\s+(nx_\w+)\(([^;]*?)\);", txt, re.M | re.S):
         params = " ".join(m.group(2).split())
         out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
     return out

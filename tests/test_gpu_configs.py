@@ -148,13 +148,15 @@ def test_batched_configuration_scenes_equal_single_grids(gpu):
     b.destroy()
 
 
-def test_mixed_configuration_4096_step_equals_replay(gpu):
-    """4096², bf16 storage, 8 seeded splats, free-slip, vorticity 0.3, dye:
-    one step (16 sweeps, so the oracle stays within seconds) bitwise equal to
-    the oracle's replay of the reduced-storage dataflow."""
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_mixed_configuration_4096_step_equals_replay(gpu, storage):
+    """4096², bf16 or f16 storage, 8 seeded splats, free-slip, vorticity 0.3,
+    dye, 120 Jacobi sweeps and the float32 correction launch (the mixed
+    configuration's workload): one step bitwise equal to the oracle's replay
+    of the reduced-storage dataflow (about 45 s of NumPy per storage)."""
     import bench
-    N, iters = 4096, 16
-    sim, splats = bench.mixed_scene(N, "bf16", iters)
+    N, iters = 4096, 120
+    sim, splats = bench.mixed_scene(N, storage, iters)
     k = sim.jacobi_tile[0]
     corr = sim.correction_sweeps
     sim.add_splats(splats)
@@ -163,9 +165,127 @@ def test_mixed_configuration_4096_step_equals_replay(gpu):
     u = np.zeros((N, N, 2), np.float32)
     q = np.zeros((N, N), np.float32)
     obs = np.zeros((N, N, 2), np.float32)
-    u, p, p32, b, q = M.step_mixed(u, obs, np.float32(1 / 60), prm, "bf16", k, corr, dye=q,
+    u, p, p32, b, q = M.step_mixed(u, obs, np.float32(1 / 60), prm, storage, k, corr, dye=q,
                                    splats=[tuple(np.float32(v) for v in s) for s in splats])
-    assert_bitwise(sim.download("velocity"), u, "4096² bf16 velocity")
+    assert_bitwise(sim.download("velocity"), u, f"4096² {storage} velocity")
     assert_bitwise(sim.download("pressure32"), p32, "4096² corrected pressure")
-    assert_bitwise(sim.download("dye"), q, "4096² bf16 dye")
+    assert_bitwise(sim.download("dye"), q, f"4096² {storage} dye")
     sim.destroy()
+
+
+def unseen_scene_1024(kind, seed):
+    """Evaluation scenes of the learned step at 1024² with seeds the network
+    never saw (training used synthetic coarse problems from seeds < 900000):
+    a cylinder (centre U[0.3, 0.7]², r = 0.1 N, inflow 1.0) or random
+    rectangles covering U(0, 0.15) of the cells (free-slip walls, 1-5
+    Gaussian splats every step)."""
+    from natrix_amd.batch import random_rect_obstacles
+    N = 1024
+    rng = np.random.default_rng(seed)
+    splats = []
+    if kind == "cylinder":
+        c = rng.uniform(0.3, 0.7, 2)
+        X = np.arange(N)[None, :]
+        Y = np.arange(N)[:, None]
+        solid = (X - c[0] * N) ** 2 + (Y - c[1] * N) ** 2 <= (0.1 * N) ** 2
+        sim = make_sim(N, N, bc_mode="inflow")
+    else:
+        solid = random_rect_obstacles(rng, N, N, rng.uniform(0, 0.15))
+        sim = make_sim(N, N, bc_mode="free_slip")
+        for _ in range(int(rng.integers(1, 6))):
+            f = rng.uniform(50, 200) / 60.0
+            a = rng.uniform(0, 2 * np.pi)
+            splats.append((rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), f * np.cos(a),
+                           f * np.sin(a), 0.02 * N, 1.0))
+    sim.vorticity = 0.3
+    sim.viscosity = 0.0
+    sim.iterations = 80
+    sim.set_obstacle_map(solid)
+    return sim, splats
+
+
+@pytest.mark.parametrize("kind,seed", [("cylinder", 900001), ("rectangles", 900002),
+                                       ("rectangles", 900003)])
+def test_learned_solve_1024_unseen_seeds(gpu, kind, seed):
+    """The learned step at 1024² (the evaluation size next to 2048²) on
+    scenes with unseen seeds, after 10 developed steps: three learned steps
+    each meet FP64 relative residual <= 1e-4, and the pressure of the last
+    is within 1e-3 (relative L2, mean free) of the FP64 conjugate-gradient
+    solution of the same system."""
+    from natrix_amd import fp64
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    g = LearnedGuess(net, meta.get("size", 128))
+    sim, splats = unseen_scene_1024(kind, seed)
+    for _ in range(10):
+        if splats:
+            sim.add_splats(splats)
+        sim.update(1 / 60)
+    for _ in range(3):
+        if splats:
+            sim.add_splats(splats)
+        st = solve_step(sim, g, 1 / 60, tol=1e-4)
+        print(kind, seed, st)
+        assert st["converged"] and st["residual"] <= 1e-4
+        assert not st["fallback"]
+    x, info = fp64.solve(sim, tol=1e-11, max_iter=100000)
+    assert info["residual"] <= 1e-8
+    err = fp64.pressure_error(sim, x)
+    print(f"{kind} {seed}: FP64 relative error of the learned pressure {err:.3e}")
+    assert err <= 1e-3
+    sim.destroy()
+
+
+def test_network_share_of_the_learned_step_2048(gpu):
+    """What the network contributes at the headline size: the same 2048²
+    cylinder scene (10 developed steps) and the same two-grid cycle with
+    three coarse solvers: the trained U-Net's guess followed by 100 coarse
+    Chebyshev iterations; no network (the coarse smoothing starts from
+    zero) with the same 100 iterations; no network with 400 iterations,
+    more coarse work than the network's forward pass costs.  Each runs 2
+    untimed and 3 timed steps to FP64 relative residual <= 1e-4, the plain
+    Jacobi fallback of a correction solve capped at 10^5 sweeps and the
+    refinement at 4 iterations; the table of fine iterations, residual and
+    ms per step is printed."""
+    import time
+
+    import torch
+
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    N = 2048
+    variants = [("network + 100", dict(coarse_iters=100)),
+                ("zero + 100", dict(coarse_iters=100, use_network=False)),
+                ("zero + 400", dict(coarse_iters=400, use_network=False))]
+    rows = {}
+    for name, kw in variants:
+        g = LearnedGuess(net, meta.get("size", 128), **kw)
+        sim = make_sim(N, N, bc_mode="inflow")
+        sim.vorticity = 0.3
+        sim.viscosity = 0.0
+        sim.iterations = 80
+        sim.set_obstacle_map(cylinder_2048())
+        for _ in range(10):
+            sim.update(1 / 60)
+        cap = dict(tol=1e-4, max_inner_sweeps=100_000, max_outer=4)
+        for _ in range(2):
+            solve_step(sim, g, 1 / 60, **cap)
+        sim.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sts = [solve_step(sim, g, 1 / 60, **cap) for _ in range(3)]
+        sim.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / 3
+        rows[name] = (float(np.mean([st["sweeps"] for st in sts])), ms,
+                      float(np.mean([st["learned_corrections"] for st in sts])),
+                      max(st["residual"] for st in sts),
+                      all(st["converged"] and st["residual"] <= 1e-4 for st in sts))
+        sim.destroy()
+    for name, (it, ms, cyc, res, ok) in rows.items():
+        print(f"{name:14s}: {it:10.1f} fine iterations/step, {cyc:4.1f} cycles/step, "
+              f"max residual {res:.2e} ({'converged' if ok else 'capped'}), {ms:9.3f} ms/step")
+    assert rows["network + 100"][4]
+    # the network's guess saves fine iterations against the same cycle from zero
+    assert rows["network + 100"][0] < rows["zero + 100"][0]

@@ -284,6 +284,53 @@ def test_fine_chebyshev_matches_float32_statement(gpu, rng, H, W, tile, iters):
     sim.destroy()
 
 
+@pytest.mark.parametrize("H,W", [(72, 100), (131, 257), (5, 300), (300, 9)])
+@pytest.mark.parametrize("cfg,iters", [((8, 4, 8, 2, 112), 40), ((8, 4, 8, 2, 1), 9),
+                                       ((8, 4, 8, 2, 7), 17), ((5, 4, 8, 2, 40), 23),
+                                       ((16, 2, 16, 2, 48), 40), ((8, 4, 16, 2, 80), 8),
+                                       ((7, 8, 8, 2, 33), 1), ((16, 8, 8, 2, 1000), 50),
+                                       ((2, 4, 8, 2, 16), 3)])
+def test_fine_chebyshev_streaming_matches_float32_statement(gpu, rng, H, W, cfg, iters):
+    """The streaming smoother (a workgroup walks a column strip over `seg`
+    output rows, carrying the rows above each chunk per pass) equals the
+    float32 statement bitwise: every shape, odd and even passes per launch,
+    segments of one row to longer than the grid, grids thinner than a
+    chunk or a strip; the reported residual is the FP64 relative residual
+    of the result.  Unsupported streaming tiles are refused."""
+    lmin = 0.01
+    sim = in_step_sim(rng, H, W)
+    solid = sim.download("mask") != 0
+    b = rng.normal(size=(H, W)).astype(np.float32)
+    p = rng.normal(size=(H, W)).astype(np.float32)
+    dev(sim, "divergence").copy_(torch.from_numpy(b))
+    dev(sim, "pressure").copy_(torch.from_numpy(p))
+    torch.cuda.synchronize()
+    _lib.call("nx_set_smoother_tile", sim._alive(), (C.c_int * 5)(*cfg), 5)
+    got_cfg = (C.c_int * 5)()
+    _lib.call("nx_get_smoother_tile", sim._alive(), got_cfg)
+    assert tuple(got_cfg) == cfg
+    out = (C.c_double * 2)()
+    _lib.call("nx_smooth_chebyshev", sim._alive(), iters, lmin, None, out)
+    got = sim.download("pressure")
+    want = cheb32(p, b, solid, iters, lmin)
+    assert np.array_equal(got, want)
+    r = (b.astype(np.float64) - lap(got.astype(np.float64), solid))[~solid]
+    bf = b.astype(np.float64)[~solid]
+    n = r.size
+    rel = np.sqrt((np.sum(r * r) - np.sum(r) ** 2 / n) / (np.sum(bf * bf) - np.sum(bf) ** 2 / n))
+    assert abs(out[0] - rel) <= 1e-9 * rel
+    for bad in ((8, 4, 8, 2, -1), (17, 4, 8, 2, 64), (8, 3, 8, 2, 64), (8, 4, 8, 1, 64),
+                (0, 4, 8, 2, 64), (30, 4, 8, 2, 64)):
+        with pytest.raises(ValueError):
+            _lib.call("nx_set_smoother_tile", sim._alive(), (C.c_int * 5)(*bad), 5)
+    with pytest.raises(ValueError):
+        _lib.call("nx_set_smoother_tile", sim._alive(), (C.c_int * 5)(*cfg), 3)
+    _lib.call("nx_get_smoother_tile", sim._alive(), got_cfg)
+    assert tuple(got_cfg) == cfg  # a refused tile leaves the setting alone
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
 def test_fine_chebyshev_damps_the_interval(gpu):
     """On a grid without obstacles the eigenvectors of -L are cosine modes
     with eigenvalue 4 sin^2(pi i / 2W) + 4 sin^2(pi j / 2H); n iterations
@@ -700,3 +747,46 @@ def test_native_cycle_loop_arguments(gpu):
     assert np.array_equal(sim.download("pressure"), before)
     _lib.call("nx_step_finish", sim._alive())
     sim.destroy()
+
+
+@pytest.mark.parametrize("n,count", [(128, 6), (37, 5)])
+def test_fp64_batch_solve_matches_single_grid_cg(gpu, n, count):
+    """nx_fp64_solve_batch (one workgroup per problem, the training
+    targets) reaches the tolerance on every problem and agrees with the
+    single-grid FP64 conjugate gradients of nx_fp64_solve on each (both
+    mean-free over fluid cells); the float32 output is
+    that solution rounded (within one float32 ulp of its magnitude, plus
+    the condition number times the residuals for the difference of two FP64
+    solves)."""
+    from natrix_amd.learned.data import random_problem
+    probs = [random_problem(np.random.default_rng(910000 + i), n) for i in range(count)]
+    b = torch.from_numpy(np.stack([q[0] for q in probs])).cuda()
+    m = torch.from_numpy(np.stack([q[1] for q in probs]).astype(np.uint8)).cuda()
+    work = torch.empty(count * n * n, dtype=torch.float64, device="cuda")
+    xf = torch.empty(count, n, n, dtype=torch.float32, device="cuda")
+    res = torch.empty(count, dtype=torch.float64, device="cuda")
+    its = torch.empty(count, dtype=torch.int32, device="cuda")
+    p = lambda t: C.c_void_p(t.data_ptr())
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.call("nx_fp64_solve_batch", p(b), p(m), n, count, p(work), p(xf), 1e-11, 20000,
+              p(res), p(its), s)
+    torch.cuda.synchronize()
+    assert float(res.max()) <= 1e-10
+    assert int(its.min()) > 0
+    for i in range(count):
+        x = torch.empty(n * n, dtype=torch.float64, device="cuda")
+        out = (C.c_double * 2)()
+        _lib.call("nx_fp64_solve", p(b[i]), p(m[i]), n, n, p(x), 1e-12, 100000, out, s)
+        torch.cuda.synchronize()
+        fluid = (m[i] == 0).view(-1)
+        ref = x - x[fluid].mean()
+        ref = torch.where(fluid, ref, torch.zeros_like(ref))
+        got = xf[i].view(-1).double()
+        assert torch.all(got[~fluid] == 0)
+        err = float((got - ref).abs().max())
+        # two FP64 solves to relative residuals 1e-11 and 1e-12 differ by at
+        # most cond(A) (about 1.3e4 at 128^2) times 1e-10 relative
+        assert err <= (2e-6 + 2.0 ** -23) * float(ref.abs().max()), (i, err)
+    with pytest.raises(ValueError):
+        _lib.call("nx_fp64_solve_batch", p(b), p(m), 129, count, p(work), p(xf), 1e-10, 100,
+                  p(res), p(its), s)

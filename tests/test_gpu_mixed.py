@@ -48,37 +48,94 @@ def test_mixed_step_matches_replay(gpu, storage, corr):
     sim.destroy()
 
 
+UNIT_ROUNDOFF = {"f32": 2.0 ** -24, "f16": 2.0 ** -11, "bf16": 2.0 ** -8}
+
+
+def _field_errors(got, ref, fluid=None):
+    """(max-norm, L2) error relative to the reference field's max and L2
+    norms; with `fluid`, both fields lose their fluid mean first (pressure
+    is defined up to a constant) and solid cells are left out."""
+    a = np.asarray(got, np.float64)
+    b = np.asarray(ref, np.float64)
+    if fluid is not None:
+        a = np.where(fluid, a - a[fluid].mean(), 0.0)
+        b = np.where(fluid, b - b[fluid].mean(), 0.0)
+    d = a - b
+    return float(np.abs(d).max() / np.abs(b).max()), float(np.linalg.norm(d) / np.linalg.norm(b))
+
+
 def test_mixed_error_against_fp64(gpu):
-    """Relative L2 velocity error of short rollouts against the float64
-    oracle: float32 storage < float16 < bfloat16, each within a bound set by
-    its unit round-off (bf16 2^-9, f16 2^-12) times a modest amplification."""
-    H, W, steps = 96, 96, 3
+    """Error of each field (velocity, corrected pressure, dye) of 5-step
+    rollouts at 96², speed 500 (8.3-cell back-traces), against the float64
+    twin of the same dataflow (oracle.mixed.step_mixed with identity
+    rounding in float64: 60 Jacobi sweeps and the 8-sweep correction launch
+    of A e = b - A p; float32 storage has no correction launch).  The
+    reference step without the correction launch is
+    a different solver: 8 more sweeps of an unconverged 60-sweep solve move
+    the velocity by about 1 % relative L2, which an earlier version of this
+    test attributed to rounding.
+
+    Bars from the rounding model: each step stores velocity three times,
+    dye once and pressure once per Jacobi launch, each store a relative
+    error of at most the format's unit round-off u, and the stencils and
+    the semi-Lagrangian advection do not amplify errors by more than a
+    small constant per step; so per field the L2 error is at most
+    4 steps u and the max-norm error 10 steps u.  float32 storage (no
+    rounding at stores, only float32 arithmetic) stays below 1e-4.  Where
+    the stores dominate, the error scales with u: bf16 / f16 lies within
+    a factor 2 of u_bf16 / u_f16 = 8 for every field."""
+    H, W, steps, iters, k = 96, 96, 5, 60, 8
     rng = np.random.default_rng(23)
     u0 = random_velocity(rng, H, W, 0.2)
+    q0 = np.zeros((H, W), np.float32)
+    q0[20:30, 20:70] = 1.0
     solid = np.zeros((H, W), bool)
     solid[40:56, 30:38] = True
-    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=60, speed=100.0,
+    fluid = ~solid
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=iters, speed=500.0,
                    bc_mode=F.BC_FREE_SLIP)
+    ref = {}
+    for storage in ("f32", "f16", "bf16"):
+        R = M.rounder(storage)
+        # the FP64 twin starts from the same stored fields
+        u = R(u0).astype(np.float64)
+        q = R(q0).astype(np.float64)
+        obs = np.zeros((H, W, 2))
+        # float32 storage runs no correction launch
+        corr = 0 if storage == "f32" else 8
+        for _ in range(steps):
+            u, _, p32, _, q = M.step_mixed(u, obs, 1 / 60, prm, "f64", k, corr, dye=q,
+                                           persistent=solid)
+        ref[storage] = (u, p32, q)
     errs = {}
     for storage in ("f32", "f16", "bf16"):
-        u0r = M.rounder(storage)(u0)
-        sim = make_sim(H, W, bc_mode=1, storage=storage)
+        R = M.rounder(storage)
+        sim = make_sim(H, W, dye=True, bc_mode=1, storage=storage)
         sim.vorticity = 0.3
         sim.viscosity = 0.0
-        sim.speed = 100.0
-        sim.iterations = 60
-        sim.upload("velocity", u0r)
+        sim.speed = 500.0
+        sim.iterations = iters
+        if storage != "f32":
+            sim.jacobi_tile = (k, 4, 8, 2)
+        sim.upload("velocity", R(u0))
+        sim.upload("dye", R(q0))
         sim.set_obstacle_map(solid)
-        u64 = u0r.astype(np.float64)
-        p64 = np.zeros((H, W))
-        o64 = np.zeros((H, W, 2))
         for _ in range(steps):
             sim.update(1 / 60)
-            u64, p64, o64, _ = F.step(u64, p64, o64, 1 / 60, prm, persistent=solid)
-        got = sim.download("velocity").astype(np.float64)
-        errs[storage] = float(np.linalg.norm(got - u64) / np.linalg.norm(u64))
+        u, p32, q = ref[storage]
+        errs[storage] = {
+            "velocity": _field_errors(sim.download("velocity"), u),
+            "pressure": _field_errors(sim.download("pressure32"), p32, fluid),
+            "dye": _field_errors(sim.download("dye"), q)}
         sim.destroy()
-    print("velocity L2 error vs fp64:", errs)
-    assert errs["f32"] < 1e-4
-    assert errs["f16"] < 1e-2 and errs["bf16"] < 5e-2
-    assert errs["f32"] < errs["f16"] < errs["bf16"]
+    for storage, e in errs.items():
+        print(storage, {f: f"max {m:.2e} L2 {l2:.2e}" for f, (m, l2) in e.items()})
+    for f in ("velocity", "pressure", "dye"):
+        assert max(errs["f32"][f]) < 1e-4, (f, errs["f32"][f])
+        for storage in ("f16", "bf16"):
+            u_s = UNIT_ROUNDOFF[storage]
+            mx, l2 = errs[storage][f]
+            assert l2 <= 4 * steps * u_s, (storage, f, l2)
+            assert mx <= 10 * steps * u_s, (storage, f, mx)
+        ratio = errs["bf16"][f][1] / errs["f16"][f][1]
+        assert 4.0 <= ratio <= 16.0, (f, ratio)

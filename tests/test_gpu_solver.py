@@ -45,6 +45,35 @@ def test_jacobi_blocked_bitwise(gpu, rows, waves, cols):
         sim.destroy()
 
 
+STREAM_TILES = [(4, 8, 2), (2, 16, 2), (4, 16, 2), (8, 8, 2)]
+
+
+@pytest.mark.parametrize("rows,waves,cols", STREAM_TILES)
+def test_jacobi_streaming_bitwise(gpu, rows, waves, cols):
+    """The streaming Jacobi kernel (a workgroup walks `seg` output rows of a
+    column strip, carrying the two rows above each chunk per sweep) is
+    bitwise equal to single sweeps of the oracle: odd and even widths and k,
+    segments of one row, shorter and longer than a chunk, and longer than
+    the grid, grids thinner than a chunk or a strip, k up to 16."""
+    for (H, W), k, seg in [((100, 77), 1, 7), ((100, 78), 4, 33), ((100, 77), 5, 1),
+                           ((257, 130), 8, 112), ((33, 200), 3, 1000), ((64, 256), 6, 48),
+                           ((5, 300), 8, 16), ((300, 9), 16, 80), ((131, 128), 2, 131)]:
+        rng = np.random.default_rng(H + W + k + seg)
+        m = blob_mask(rng, H, W)
+        b = rng.standard_normal((H, W)).astype(f32)
+        p = rng.standard_normal((H, W)).astype(f32)
+        sim = make_sim(H, W)
+        sim.upload("mask", m)
+        sim.upload("divergence", b)
+        sim.upload("pressure", p)
+        sweeps = 3 * k + 2  # includes a shortened final launch
+        sim.jacobi_blocked(sweeps, k, rows, waves, cols, seg)
+        ref = F.jacobi(p, b, obs_from_mask(m), sweeps)
+        assert_bitwise(sim.download("pressure"), ref,
+                       f"stream {rows}x{waves}x{cols} k={k} seg={seg} {H}x{W}")
+        sim.destroy()
+
+
 def test_residual_matches_oracle(gpu):
     H, W = 120, 90
     rng = np.random.default_rng(5)
@@ -93,12 +122,14 @@ def _scene(H, W, seed, bc_mode=2):
     return u, m
 
 
-@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("fused,tile", [(True, None), (False, None), (True, (8, 4, 8, 2, 40))])
 @pytest.mark.parametrize("bc_mode,nu", [(0, 0.1), (1, 0.0), (2, 0.0)])
-def test_rollout_10_steps_bitwise(gpu, fused, bc_mode, nu):
+def test_rollout_10_steps_bitwise(gpu, fused, tile, bc_mode, nu):
     H, W = 96, 128
     u, m = _scene(H, W, 21)
     sim = make_sim(H, W, dye=True, bc_mode=bc_mode, fused=fused)
+    if tile is not None:
+        sim.jacobi_tile = tile
     sim.vorticity = 0.3
     sim.viscosity = nu
     sim.iterations = 20
@@ -190,7 +221,10 @@ def test_graph_replay_matches_eager(gpu):
         s.destroy()
 
 
-def test_tolerance_solve_reaches_tol(gpu):
+@pytest.mark.parametrize("tile", [(8, 8, 8, 1), (8, 4, 8, 2, 24)])
+def test_tolerance_solve_reaches_tol(gpu, tile):
+    """Also with the streaming Jacobi kernel, whose last launch of a chunk
+    writes the residual records per workgroup."""
     H, W = 64, 64
     u, m = _scene(H, W, 44)
     sim = make_sim(H, W, bc_mode=2)
@@ -200,7 +234,7 @@ def test_tolerance_solve_reaches_tol(gpu):
     sim.set_obstacle_map(m)
     sim.tol = 1e-4
     sim.check_every = 64
-    sim.jacobi_tile = (8, 8, 8, 1)
+    sim.jacobi_tile = tile
     sim.update(1 / 60)
     st = sim.stats()
     assert st["converged"] and st["residual"] <= 1e-4

@@ -39,6 +39,30 @@ def test_identity_rounding_reproduces_reference_step(rng):
     assert np.array_equal(p32, pr) and np.array_equal(b, st["divergence"])
 
 
+def test_fp64_twin_is_the_float64_reference_step_plus_correction(rng):
+    """With float64 inputs and "f64" (identity) rounding, step_mixed is the
+    float64 step of oracle.fluid (no correction), and the correction
+    launch adds exactly kc Jacobi sweeps of A e = b - A p from 0 in
+    float64: the reference the reduced-storage errors are measured
+    against."""
+    H, W = 24, 32
+    u = rng.uniform(-0.5, 0.5, (H, W, 2))
+    obs = F.add_circle_obstacle(np.zeros((H, W, 2), np.float32), (0.5, 0.5), 3.0).astype(
+        np.float64)
+    prm = F.Params(vorticity=0.3, viscosity=0.0, iterations=11, bc_mode=F.BC_FREE_SLIP)
+    q = rng.uniform(0, 1, (H, W))
+    v, p, p32, b, dye = M.step_mixed(u, obs, 1 / 60, prm, "f64", k=4, corr_sweeps=0, dye=q)
+    assert v.dtype == p32.dtype == dye.dtype == np.float64
+    ur, pr, _, qr = F.step(u, np.zeros((H, W)), obs, 1 / 60, prm, dye=q)
+    assert np.array_equal(v, ur) and np.array_equal(p32, pr) and np.array_equal(dye, qr)
+    _, p8, p32c, b8, _ = M.step_mixed(u, obs, 1 / 60, prm, "f64", k=4, corr_sweeps=5, dye=q)
+    solid = F.solid_of(obs)
+    x1, x2, y1, y2 = F._replaced_neighbours(p8, solid)
+    r = b8 - ((((x1 + x2) + y1) + y2) - 4.0 * p8)
+    e = F.jacobi(np.zeros_like(p8), r, obs, 5)
+    assert np.array_equal(p32c, p8 + e)
+
+
 def test_correction_recovers_float32_residual(rng):
     # a bf16-stored Jacobi iterate carries ~2^-9 relative noise, which raises
     # its residual above the float32 iterate's; the float32 correction pass
