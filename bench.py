@@ -681,6 +681,9 @@ def main():
                     help="fixed-iteration steps timed for the Jacobi roofline (learned workload)")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the copy-bandwidth probe (counter calibration)")
+    ap.add_argument("--probe-variant", type=int, default=0,
+                    help="copy probe variant (0: float4 grid-stride, 5: float2, 6: float; "
+                         "-1: the best of all)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 20 if args.workload == "learned" else 1000
@@ -704,8 +707,9 @@ def main():
         return g.value
 
     if args.probe_copy:
-        gbs = copy_bandwidth(args.probe_copy, 10)
+        gbs = copy_bandwidth(args.probe_copy, 10, args.probe_variant)
         print(json.dumps({"probe": "copy", "bytes": args.probe_copy,
+                          "variant": args.probe_variant,
                           "launches": 11, "GBps": round(gbs, 1)}), flush=True)
         return
 

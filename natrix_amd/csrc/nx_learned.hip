@@ -10,7 +10,9 @@
 //   3. the U-Net forward (4 levels, 3x3 conv + ReLU pairs, 2x2 average pool,
 //      2x nearest upsample + channel concat, 1x1 head) with the forward
 //      convolution kernels, split over K exactly as the Python wrapper
-//      splits them, so nx_unet_forward equals the Python forward bitwise;
+//      splits them, so nx_unet_forward equals the Python forward bitwise
+//      (the decoder convolutions read the upsampled, concatenated input in
+//      their fragment loads: the same values as the Python path's copies);
 //   4. k_scale_output: e = s x head output;
 //   5. Chebyshev smoothing of the coarse problem from e (nx_coarse_smooth);
 //   6. k_center_output (one block): e -= fluid mean of e, 0 on solid cells;
@@ -47,9 +49,11 @@ struct nx_unet {
   uint16_t *t1 = nullptr, *e1 = nullptr, *p1 = nullptr;
   uint16_t *t2 = nullptr, *e2 = nullptr, *p2 = nullptr;
   uint16_t *t3 = nullptr, *e3 = nullptr;
-  uint16_t *u2 = nullptr, *c2 = nullptr, *s2 = nullptr, *d2 = nullptr;
-  uint16_t *u1 = nullptr, *c1 = nullptr, *s1 = nullptr, *d1 = nullptr;
-  uint16_t *u0 = nullptr, *c0 = nullptr, *s0 = nullptr, *d0 = nullptr;
+  // decoder: s = first conv (on the upsampled + concatenated input, read in
+  // place), d = second conv
+  uint16_t *s2 = nullptr, *d2 = nullptr;
+  uint16_t *s1 = nullptr, *d1 = nullptr;
+  uint16_t *s0 = nullptr, *d0 = nullptr;
   float* out = nullptr;    // head output / coarse correction e
   float* rc = nullptr;     // coarse right-hand side
   uint8_t* mc = nullptr;   // coarse mask
@@ -188,6 +192,16 @@ int conv(nx_unet* u, int i, const void* x, void* y, int h, int w, hipStream_t s)
   return nx_conv3x3_fwd(x, u->w[i], u->b[i], y, 1, h, w, u->cin[i], u->cout[i], 1, s);
 }
 
+// decoder convolution i on concat(upsample2(lo) (c1 channels), skip), read
+// in the convolution's fragment loads (no upsampled or concatenated copy)
+int conv_cat(nx_unet* u, int i, const void* lo, const void* skip, int c1, void* y, int h, int w,
+             hipStream_t s) {
+  const long long m = (long long)h * w;
+  const int sp = conv_splits(m, u->cin[i], u->cout[i]);
+  return nx::conv3x3_fwd_tiled(lo, u->wt[i], u->b[i], y, u->work, u->cnt, sp, h, w, u->cin[i],
+                               u->cout[i], 1, s, skip, c1);
+}
+
 int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s) {
   const int H = u->H, W = u->W, c = u->base;
 #define NX_TRY_U(expr)          \
@@ -206,17 +220,11 @@ int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s) {
   NX_TRY_U(nx_avgpool2(u->e2, u->p2, 1, H / 4, W / 4, 4 * c, 0, s));
   NX_TRY_U(conv(u, 6, u->p2, u->t3, H / 8, W / 8, s));
   NX_TRY_U(conv(u, 7, u->t3, u->e3, H / 8, W / 8, s));
-  NX_TRY_U(nx_upsample2(u->e3, u->u2, 1, H / 4, W / 4, 8 * c, 0, s));
-  NX_TRY_U(nx_concat(u->c2, u->u2, u->e2, (long long)(H / 4) * (W / 4), 8 * c, 4 * c, 0, s));
-  NX_TRY_U(conv(u, 8, u->c2, u->s2, H / 4, W / 4, s));
+  NX_TRY_U(conv_cat(u, 8, u->e3, u->e2, 8 * c, u->s2, H / 4, W / 4, s));
   NX_TRY_U(conv(u, 9, u->s2, u->d2, H / 4, W / 4, s));
-  NX_TRY_U(nx_upsample2(u->d2, u->u1, 1, H / 2, W / 2, 4 * c, 0, s));
-  NX_TRY_U(nx_concat(u->c1, u->u1, u->e1, (long long)(H / 2) * (W / 2), 4 * c, 2 * c, 0, s));
-  NX_TRY_U(conv(u, 10, u->c1, u->s1, H / 2, W / 2, s));
+  NX_TRY_U(conv_cat(u, 10, u->d2, u->e1, 4 * c, u->s1, H / 2, W / 2, s));
   NX_TRY_U(conv(u, 11, u->s1, u->d1, H / 2, W / 2, s));
-  NX_TRY_U(nx_upsample2(u->d1, u->u0, 1, H, W, 2 * c, 0, s));
-  NX_TRY_U(nx_concat(u->c0, u->u0, u->e0, (long long)H * W, 2 * c, c, 0, s));
-  NX_TRY_U(conv(u, 12, u->c0, u->s0, H, W, s));
+  NX_TRY_U(conv_cat(u, 12, u->d1, u->e0, 2 * c, u->s0, H, W, s));
   NX_TRY_U(conv(u, 13, u->s0, u->d0, H, W, s));
   NX_TRY_U(nx_head_fwd(u->d0, u->head_w, u->head_b, out, (long long)H * W, c, s));
 #undef NX_TRY_U
@@ -348,16 +356,10 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
   u->p2 = (uint16_t*)alloc(P3 * 4 * c * B);
   u->t3 = (uint16_t*)alloc(P3 * 8 * c * B);
   u->e3 = (uint16_t*)alloc(P3 * 8 * c * B);
-  u->u2 = (uint16_t*)alloc(P2 * 8 * c * B);
-  u->c2 = (uint16_t*)alloc(P2 * 12 * c * B);
   u->s2 = (uint16_t*)alloc(P2 * 4 * c * B);
   u->d2 = (uint16_t*)alloc(P2 * 4 * c * B);
-  u->u1 = (uint16_t*)alloc(P1 * 4 * c * B);
-  u->c1 = (uint16_t*)alloc(P1 * 6 * c * B);
   u->s1 = (uint16_t*)alloc(P1 * 2 * c * B);
   u->d1 = (uint16_t*)alloc(P1 * 2 * c * B);
-  u->u0 = (uint16_t*)alloc(P0 * 2 * c * B);
-  u->c0 = (uint16_t*)alloc(P0 * 3 * c * B);
   u->s0 = (uint16_t*)alloc(P0 * c * B);
   u->d0 = (uint16_t*)alloc(P0 * c * B);
   u->out = (float*)alloc(P0 * sizeof(float));

@@ -12,7 +12,12 @@
 //   3  U = 8 with non-temporal loads and stores (the copied data is not
 //      kept in the caches);
 //   4  one contiguous chunk per workgroup, U = 8 per thread per step
-//      (no grid stride: each workgroup streams its own region).
+//      (no grid stride: each workgroup streams its own region);
+//   5  float2 per lane (8-byte accesses, the stencil kernels' row loads),
+//      4 in flight per thread;
+//   6  float per lane (4-byte accesses), 4 in flight per thread.
+// The narrower variants calibrate the traffic counters for kernels whose
+// loads are narrower than 16 bytes per lane.
 // nx_copy_bandwidth runs variant 0; nx_copy_bandwidth_variant(-1) runs all
 // of them and reports the best: the attainable rate on this device, against
 // which the stencil kernels' measured traffic is read.
@@ -91,7 +96,27 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-constexpr int N_VARIANTS = 5;
+// U elements of type T per thread per iteration, grid-stride (n in T units)
+template <class T, int U>
+__global__ void __launch_bounds__(256)
+    k_copy_narrow(const T* __restrict__ src, T* __restrict__ dst, size_t n) {
+  const size_t step = (size_t)gridDim.x * 256 * U;
+  for (size_t base = blockIdx.x * (size_t)256 * U; base < n; base += step) {
+    T v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const size_t i = base + j * 256 + threadIdx.x;
+      if (i < n) v[j] = src[i];
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const size_t i = base + j * 256 + threadIdx.x;
+      if (i < n) dst[i] = v[j];
+    }
+  }
+}
+
+constexpr int N_VARIANTS = 7;
 
 void launch_variant(int v, const float4* a, float4* b, size_t n, hipStream_t s) {
   const unsigned blocks = 256 * 32;  // 32 blocks per CU for the grid-stride forms
@@ -107,6 +132,14 @@ void launch_variant(int v, const float4* a, float4* b, size_t n, hipStream_t s) 
       break;
     case 3:
       k_copy_unrolled<8, true><<<blocks, 256, 0, s>>>(a, b, n);
+      break;
+    case 5:
+      k_copy_narrow<float2, 4><<<blocks, 256, 0, s>>>(reinterpret_cast<const float2*>(a),
+                                                      reinterpret_cast<float2*>(b), 2 * n);
+      break;
+    case 6:
+      k_copy_narrow<float, 4><<<blocks, 256, 0, s>>>(reinterpret_cast<const float*>(a),
+                                                     reinterpret_cast<float*>(b), 4 * n);
       break;
     default: {
       // 8 workgroups per CU, each with one contiguous chunk

@@ -65,7 +65,8 @@ __device__ __forceinline__ void cheb_pass(float (&X)[ROWS][COLS], float (&R)[ROW
                                           const int (&ps)[COLS], float theta, float cdd,
                                           float crr, int row_new0, int gx0, int tcol0, int k,
                                           int Y0, int Y1, int W, int EX,
-                                          const float* __restrict__ b, double (&acc)[5]) {
+                                          const float (&BR)[ROWS][COLS], double (&acc)[5],
+                                          bool special) {
   float(&V)[ROWS][COLS] = KIND == P_ITER ? D : X;
   float(&Aa)[ROWS][COLS] = KIND == P_ITER ? X : R;
   float(&Bb)[ROWS][COLS] = KIND == P_ITER ? R : D;
@@ -86,8 +87,11 @@ __device__ __forceinline__ void cheb_pass(float (&X)[ROWS][COLS], float (&R)[ROW
     }
     const float lft = lane_from_left(vc[COLS - 1]);
     const float rgt = lane_from_right(vc[0]);
-    const int slf = ilane_from_left(sc[COLS - 1]);
-    const int srg = ilane_from_right(sc[0]);
+    int slf = 0, srg = 0;
+    if (special) {
+      slf = ilane_from_left(sc[COLS - 1]);
+      srg = ilane_from_right(sc[0]);
+    }
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       const float v = vc[c];
@@ -95,12 +99,14 @@ __device__ __forceinline__ void cheb_pass(float (&X)[ROWS][COLS], float (&R)[ROW
       float x2 = c < COLS - 1 ? vc[c + 1] : rgt;
       float y1 = va[c];
       float y2 = vb[c];
-      const int s1 = c > 0 ? sc[c - 1] : slf;
-      const int s2 = c < COLS - 1 ? sc[c + 1] : srg;
-      x1 = s1 ? v : x1;
-      x2 = s2 ? v : x2;
-      y1 = sa[c] ? v : y1;
-      y2 = sb[c] ? v : y2;
+      if (special) {
+        const int s1 = c > 0 ? sc[c - 1] : slf;
+        const int s2 = c < COLS - 1 ? sc[c + 1] : srg;
+        x1 = s1 ? v : x1;
+        x2 = s2 ? v : x2;
+        y1 = sa[c] ? v : y1;
+        y2 = sb[c] ? v : y2;
+      }
       if (KIND == P_SETUP) {
         const float lv = (((x1 + x2) + y1) + y2) - 4.0f * v;
         const float rn = lv - ac[c];  // r holds b before the setup
@@ -117,7 +123,7 @@ __device__ __forceinline__ void cheb_pass(float (&X)[ROWS][COLS], float (&R)[ROW
         const int gy = row_new0 + i, gx = gx0 + c, tcol = tcol0 + c;
         if (gy >= Y0 && gy < Y1 && tcol >= k && tcol < EX - k && gx >= 0 && gx < W &&
             !sc[c]) {
-          const double bd = (double)b[gy * W + gx];
+          const double bd = (double)BR[i][c];  // b of the output row, loaded with the chunk
           const double rr =
               bd - (((((double)x1 + (double)x2) + (double)y1) + (double)y2) - 4.0 * (double)v);
           acc[0] += rr;
@@ -161,41 +167,86 @@ __global__ void __launch_bounds__(WAVES * 64)
 
   float X[ROWS][COLS], R[ROWS][COLS], D[ROWS][COLS];
   int SL[ROWS][COLS];
+  // the next chunk's inputs, loaded while the current chunk's passes run
+  // (its loads stay in flight across the passes' LDS barriers); with RES
+  // also b of the rows the last pass writes (register rows k up)
+  float Xn[ROWS][COLS], Rn[ROWS][COLS], Dn[ROWS][COLS];
+  float BR[ROWS][COLS], BRn[ROWS][COLS];  // used by RES launches only
+  int SLn[ROWS][COLS];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
-  for (int ch = 0; ch < nch; ++ch) {
-    const int row0 = A + ch * S + wave * ROWS;  // level-0 row of register row 0
+  auto load_chunk = [&](int ch) {
+    const int row0 = A + ch * S + wave * ROWS;
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
       const int gy = row0 + i;
       const bool rin_grid = gy >= 0 && gy < H;
+      // b of the row the residual pass writes from register row i
+      const int gyr = gy - k;
+      const bool res_row = RES && gyr >= 0 && gyr < H;
+      // loads are unconditional, from a clamped in-grid address, and the
+      // values of cells outside the grid are replaced afterwards (a select
+      // on the loaded value keeps every load a plain global load)
+      const float* rsrc = first ? b : rin;
+      const float* dsrc = first ? xin : din;
       if (vec) {
         const bool in = rin_grid && gx0 >= 0 && gx0 < W;
         const int idx = in ? gy * W + gx0 : 0;
-        const float2 z = make_float2(0.0f, 0.0f);
-        const float2 xv = in ? *reinterpret_cast<const float2*>(xin + idx) : z;
-        const float2 rv = in ? *reinterpret_cast<const float2*>((first ? b : rin) + idx) : z;
-        const float2 dv = (in && !first) ? *reinterpret_cast<const float2*>(din + idx) : z;
-        const uint16_t mm = in ? *reinterpret_cast<const uint16_t*>(mask + idx) : 0x0101;
-        X[i][0] = xv.x;
-        X[i][COLS - 1] = xv.y;
-        R[i][0] = rv.x;
-        R[i][COLS - 1] = rv.y;
-        D[i][0] = dv.x;
-        D[i][COLS - 1] = dv.y;
-        SL[i][0] = (mm & 0xFF) != 0;
-        SL[i][COLS - 1] = (mm >> 8) != 0;
+        const float2 xv = *reinterpret_cast<const float2*>(xin + idx);
+        const float2 rv = *reinterpret_cast<const float2*>(rsrc + idx);
+        const float2 dv = *reinterpret_cast<const float2*>(dsrc + idx);
+        const uint16_t mm = *reinterpret_cast<const uint16_t*>(mask + idx);
+        const bool dz = !in || first;
+        Xn[i][0] = in ? xv.x : 0.0f;
+        Xn[i][COLS - 1] = in ? xv.y : 0.0f;
+        Rn[i][0] = in ? rv.x : 0.0f;
+        Rn[i][COLS - 1] = in ? rv.y : 0.0f;
+        Dn[i][0] = dz ? 0.0f : dv.x;
+        Dn[i][COLS - 1] = dz ? 0.0f : dv.y;
+        SLn[i][0] = in ? (mm & 0xFF) != 0 : 1;
+        SLn[i][COLS - 1] = in ? (mm >> 8) != 0 : 1;
+        if (RES) {
+          const bool inr = res_row && gx0 >= 0 && gx0 < W;
+          const float2 bv = *reinterpret_cast<const float2*>(b + (inr ? gyr * W + gx0 : 0));
+          BRn[i][0] = inr ? bv.x : 0.0f;
+          BRn[i][COLS - 1] = inr ? bv.y : 0.0f;
+        }
       } else {
 #pragma unroll
         for (int c = 0; c < COLS; ++c) {
           const int gx = gx0 + c;
           const bool in = rin_grid && gx >= 0 && gx < W;
           const int idx = in ? gy * W + gx : 0;
-          X[i][c] = in ? xin[idx] : 0.0f;
-          R[i][c] = in ? (first ? b[idx] : rin[idx]) : 0.0f;
-          D[i][c] = (in && !first) ? din[idx] : 0.0f;
-          SL[i][c] = in ? (mask[idx] != 0) : 1;
+          const float xv = xin[idx], rv = rsrc[idx], dv = dsrc[idx];
+          const uint8_t mv = mask[idx];
+          Xn[i][c] = in ? xv : 0.0f;
+          Rn[i][c] = in ? rv : 0.0f;
+          Dn[i][c] = (in && !first) ? dv : 0.0f;
+          SLn[i][c] = in ? (mv != 0) : 1;
+          if (RES) {
+            const bool inr = res_row && gx >= 0 && gx < W;
+            const float bv = b[inr ? gyr * W + gx : 0];
+            BRn[i][c] = inr ? bv : 0.0f;
+          }
         }
+      }
+    }
+  };
+
+  load_chunk(0);
+  for (int ch = 0; ch < nch; ++ch) {
+    const int row0 = A + ch * S + wave * ROWS;  // level-0 row of register row 0
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i) {
+      const int gy = row0 + i;
+      const bool rin_grid = gy >= 0 && gy < H;
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        X[i][c] = Xn[i][c];
+        R[i][c] = Rn[i][c];
+        D[i][c] = Dn[i][c];
+        SL[i][c] = SLn[i][c];
+        BR[i][c] = RES ? BRn[i][c] : 0.0f;
       }
       // a pending line-search step x += a dapp on the fluid cells
       if (a_line != 0.0f) {
@@ -207,6 +258,7 @@ __global__ void __launch_bounds__(WAVES * 64)
         }
       }
     }
+    if (ch + 1 < nch) load_chunk(ch + 1);
 
     for (int s = 0; s < k; ++s) {
       const int kind = (first && s == 0) ? P_SETUP : ((RES && s == k - 1) ? P_RES : P_ITER);
@@ -274,15 +326,25 @@ __global__ void __launch_bounds__(WAVES * 64)
         }
       }
       const int row_new0 = row0 - (s + 1);
+      // no solid or out-of-grid cell among the rows this pass reads in the
+      // wave (its own and the two above): no neighbour is replaced.  The
+      // strip's outermost lanes read garbage across the wave edge either
+      // way, which stays outside the columns the launch keeps.
+      int anys = ps[0] | (COLS > 1 ? ps[COLS - 1] : 0);
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) anys |= SL[i][c];
+      const bool special = __any(anys != 0);
       if (kind == P_ITER)
         cheb_pass<P_ITER>(X, R, D, SL, p0, p1, p2, p3, ps, theta, cf.cd[s], cf.cr[s], row_new0,
-                          gx0, tcol0, k, Y0, Y1, W, EX, b, acc);
+                          gx0, tcol0, k, Y0, Y1, W, EX, BR, acc, special);
       else if (kind == P_SETUP)
         cheb_pass<P_SETUP>(X, R, D, SL, p0, p1, p2, p3, ps, theta, 0.0f, 0.0f, row_new0, gx0,
-                           tcol0, k, Y0, Y1, W, EX, b, acc);
+                           tcol0, k, Y0, Y1, W, EX, BR, acc, special);
       else
         cheb_pass<P_RES>(X, R, D, SL, p0, p1, p2, p3, ps, theta, 0.0f, 0.0f, row_new0, gx0,
-                         tcol0, k, Y0, Y1, W, EX, b, acc);
+                         tcol0, k, Y0, Y1, W, EX, BR, acc, special);
     }
 
     // register row i now holds grid row row0 + i - k
@@ -365,7 +427,7 @@ __device__ __forceinline__ void jacobi_pass(float (&P)[ROWS][COLS], float (&B)[R
                                             const float (&p1)[COLS], const float (&p2)[COLS],
                                             const int (&ps)[COLS], int row_new0, int gx0,
                                             int tcol0, int k, int Y0, int Y1, int W, int EX,
-                                            double& sr, double& srr) {
+                                            double& sr, double& srr, bool special) {
 #pragma unroll
   for (int i = ROWS - 1; i >= 0; --i) {
     float vc[COLS], va[COLS], vb[COLS], bc[COLS];
@@ -382,8 +444,11 @@ __device__ __forceinline__ void jacobi_pass(float (&P)[ROWS][COLS], float (&B)[R
     }
     const float lft = lane_from_left(vc[COLS - 1]);
     const float rgt = lane_from_right(vc[0]);
-    const int slf = ilane_from_left(sc[COLS - 1]);
-    const int srg = ilane_from_right(sc[0]);
+    int slf = 0, srg = 0;
+    if (special) {
+      slf = ilane_from_left(sc[COLS - 1]);
+      srg = ilane_from_right(sc[0]);
+    }
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       const float v = vc[c];
@@ -391,12 +456,14 @@ __device__ __forceinline__ void jacobi_pass(float (&P)[ROWS][COLS], float (&B)[R
       float x2 = c < COLS - 1 ? vc[c + 1] : rgt;
       float y1 = va[c];
       float y2 = vb[c];
-      const int s1 = c > 0 ? sc[c - 1] : slf;
-      const int s2 = c < COLS - 1 ? sc[c + 1] : srg;
-      x1 = s1 ? v : x1;
-      x2 = s2 ? v : x2;
-      y1 = sa[c] ? v : y1;
-      y2 = sb[c] ? v : y2;
+      if (special) {
+        const int s1 = c > 0 ? sc[c - 1] : slf;
+        const int s2 = c < COLS - 1 ? sc[c + 1] : srg;
+        x1 = s1 ? v : x1;
+        x2 = s2 ? v : x2;
+        y1 = sa[c] ? v : y1;
+        y2 = sb[c] ? v : y2;
+      }
       P[i][c] = ((((x1 + x2) + y1) + y2) - bc[c]) * 0.25f;
       if (RESID) {
         const int gy = row_new0 + i, gx = gx0 + c, tcol = tcol0 + c;
@@ -442,39 +509,58 @@ __global__ void __launch_bounds__(WAVES * 64)
 
   float P[ROWS][COLS], B[ROWS][COLS];
   int SL[ROWS][COLS];
+  // the next chunk's inputs, loaded while the current chunk's sweeps run
+  float Pn[ROWS][COLS], Bn[ROWS][COLS];
+  int SLn[ROWS][COLS];
   double sr = 0.0, srr = 0.0;
 
-  for (int ch = 0; ch < nch; ++ch) {
+  auto load_chunk = [&](int ch) {
     const int row0 = A + ch * S + wave * ROWS;
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
       const int gy = row0 + i;
       const bool row_in = gy >= 0 && gy < H;
+      // unconditional loads from a clamped in-grid address, then selects
       if (vec) {
         const bool in = row_in && gx0 >= 0 && gx0 < W;
         const int idx = in ? gy * W + gx0 : 0;
-        const float2 z = make_float2(0.0f, 0.0f);
-        const float2 pv = in ? *reinterpret_cast<const float2*>(pin + idx) : z;
-        const float2 bv = in ? *reinterpret_cast<const float2*>(b + idx) : z;
-        const uint16_t mm = in ? *reinterpret_cast<const uint16_t*>(mask + idx) : 0x0101;
-        P[i][0] = pv.x;
-        P[i][COLS - 1] = pv.y;
-        B[i][0] = bv.x;
-        B[i][COLS - 1] = bv.y;
-        SL[i][0] = (mm & 0xFF) != 0;
-        SL[i][COLS - 1] = (mm >> 8) != 0;
+        const float2 pv = *reinterpret_cast<const float2*>(pin + idx);
+        const float2 bv = *reinterpret_cast<const float2*>(b + idx);
+        const uint16_t mm = *reinterpret_cast<const uint16_t*>(mask + idx);
+        Pn[i][0] = in ? pv.x : 0.0f;
+        Pn[i][COLS - 1] = in ? pv.y : 0.0f;
+        Bn[i][0] = in ? bv.x : 0.0f;
+        Bn[i][COLS - 1] = in ? bv.y : 0.0f;
+        SLn[i][0] = in ? (mm & 0xFF) != 0 : 1;
+        SLn[i][COLS - 1] = in ? (mm >> 8) != 0 : 1;
       } else {
 #pragma unroll
         for (int c = 0; c < COLS; ++c) {
           const int gx = gx0 + c;
           const bool in = row_in && gx >= 0 && gx < W;
           const int idx = in ? gy * W + gx : 0;
-          P[i][c] = in ? pin[idx] : 0.0f;
-          B[i][c] = in ? b[idx] : 0.0f;
-          SL[i][c] = in ? (mask[idx] != 0) : 1;
+          const float pv = pin[idx], bv = b[idx];
+          const uint8_t mv = mask[idx];
+          Pn[i][c] = in ? pv : 0.0f;
+          Bn[i][c] = in ? bv : 0.0f;
+          SLn[i][c] = in ? (mv != 0) : 1;
         }
       }
     }
+  };
+
+  load_chunk(0);
+  for (int ch = 0; ch < nch; ++ch) {
+    const int row0 = A + ch * S + wave * ROWS;
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        P[i][c] = Pn[i][c];
+        B[i][c] = Bn[i][c];
+        SL[i][c] = SLn[i][c];
+      }
+    if (ch + 1 < nch) load_chunk(ch + 1);
 
     for (int s = 0; s < k; ++s) {
       const int par = s & 1;
@@ -524,12 +610,18 @@ __global__ void __launch_bounds__(WAVES * 64)
         }
       }
       const int row_new0 = row0 - (s + 1);
+      int anys = ps[0] | (COLS > 1 ? ps[COLS - 1] : 0);
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) anys |= SL[i][c];
+      const bool special = __any(anys != 0);
       if (red != nullptr && s == k - 1)
         jacobi_pass<true>(P, B, SL, p0, p1, p2, ps, row_new0, gx0, tcol0, k, Y0, Y1, W, EX,
-                          sr, srr);
+                          sr, srr, special);
       else
         jacobi_pass<false>(P, B, SL, p0, p1, p2, ps, row_new0, gx0, tcol0, k, Y0, Y1, W, EX,
-                           sr, srr);
+                           sr, srr, special);
     }
 
 #pragma unroll

@@ -57,19 +57,27 @@ __device__ __forceinline__ uint16_t tobf(float f) { return (uint16_t)nx::f32_to_
 // reset to zero by that block) then sums the tile's partials in split
 // order, adds the bias, applies ReLU and writes bf16: the same values, in
 // the same order, as a separate reduction pass would, with no extra launch.
-template <int MT, int NT, bool WT>
+// CAT (decoder convolutions, Cin % 16 == 0): the input is the channel
+// concatenation of the 2x nearest upsampling of x (C1 channels, an
+// (H / 2) x (W / 2) image) and x2 (Cin - C1 channels, H x W), read where
+// the fragments are loaded instead of from a materialised copy; a K step's
+// 16 channels lie in one of the two (C1 % 16 == 0), so every value read is
+// the one the upsampled, concatenated tensor would hold.
+template <int MT, int NT, bool WT, bool CAT = false>
 __global__ void __launch_bounds__(256)
     k_conv3x3(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
               const float* __restrict__ bias, uint16_t* __restrict__ y, int N,
               int H, int W, int Cin, int Cout, int relu, float* __restrict__ part,
-              int steps_per_split, unsigned* __restrict__ cnt) {
+              int steps_per_split, unsigned* __restrict__ cnt,
+              const uint16_t* __restrict__ x2 = nullptr, int C1 = 0) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 31, kb = lane >> 5;
   const long long M = (long long)N * H * W;
   const long long m_base = ((long long)blockIdx.x * 4 + wave) * 32 * MT;
   const int n_base = blockIdx.y * 32 * NT;
   int py[MT], px[MT];
-  long long pimg[MT];  // first pixel of the pixel's image
+  long long pimg[MT];   // first pixel of the pixel's image
+  long long pimgh[MT];  // the same in the half-resolution image (CAT)
   bool valid[MT];
 #pragma unroll
   for (int i = 0; i < MT; ++i) {
@@ -82,6 +90,7 @@ __global__ void __launch_bounds__(256)
     py[i] = rem / W;
     px[i] = rem - py[i] * W;
     pimg[i] = img * hw;
+    pimgh[i] = img * (long long)(H >> 1) * (W >> 1);
   }
   f32x16 acc[MT][NT];
 #pragma unroll
@@ -108,11 +117,18 @@ __global__ void __launch_bounds__(256)
     const int tap_ = ks_ / cps;                                                        \
     const int ci_ = (ks_ - tap_ * cps) * 16 + kb * 8;                                  \
     const int dy_ = tap_ / 3 - 1, dx_ = tap_ % 3 - 1;                                  \
+    const bool up_ = CAT && ci_ < C1;                                                  \
     _Pragma("unroll") for (int i = 0; i < MT; ++i) {                                   \
       const int yy = py[i] + dy_, xx = px[i] + dx_;                                    \
       const bool ok = valid[i] && yy >= 0 && yy < H && xx >= 0 && xx < W;              \
-      const long long off = ok ? (pimg[i] + (long long)yy * W + xx) * Cin + ci_ : 0;   \
-      A_[i] = *reinterpret_cast<const uint4*>(x + off);                                \
+      long long off;                                                                   \
+      if (!CAT)                                                                        \
+        off = ok ? (pimg[i] + (long long)yy * W + xx) * Cin + ci_ : 0;                 \
+      else if (up_)                                                                    \
+        off = ok ? (pimgh[i] + (long long)(yy >> 1) * (W >> 1) + (xx >> 1)) * C1 + ci_ : 0; \
+      else                                                                             \
+        off = ok ? (pimg[i] + (long long)yy * W + xx) * (Cin - C1) + (ci_ - C1) : 0;   \
+      A_[i] = *reinterpret_cast<const uint4*>((CAT && !up_ ? x2 : x) + off);           \
       M_[i] = ok ? 0xFFFFFFFFu : 0u;                                                   \
     }                                                                                  \
     const int s_ = ks_ - tap_ * cps;                                                   \
@@ -471,7 +487,8 @@ extern "C" {
 
 static int conv_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                     int h, int wd, int cin, int cout, int relu, float* part, int splits,
-                    hipStream_t s, bool tiled = false, unsigned* cnt = nullptr) {
+                    hipStream_t s, bool tiled = false, unsigned* cnt = nullptr,
+                    const void* x2 = nullptr, int c1 = 0) {
   const long long M = (long long)n * h * wd;
   const int steps = 9 * cin / 16;
   const int per = part ? (steps + splits - 1) / splits : 0;
@@ -479,6 +496,20 @@ static int conv_fwd(const void* x, const void* w, const float* bias, void* y, in
   const uint16_t* xs = (const uint16_t*)x;
   const uint16_t* ws = (const uint16_t*)w;
   uint16_t* ys = (uint16_t*)y;
+  const uint16_t* x2s = (const uint16_t*)x2;
+  if (x2s) {
+    // decoder convolution on the upsampled + concatenated input (tiled weights)
+    if (cout % 64 == 0) {
+      dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64, used);
+      k_conv3x3<2, 2, true, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout,
+                                                       relu, part, per, cnt, x2s, c1);
+    } else {
+      dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 32, used);
+      k_conv3x3<2, 1, true, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout,
+                                                       relu, part, per, cnt, x2s, c1);
+    }
+    return 0;
+  }
   if (cout % 64 == 0) {
     dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64, used);
     if (tiled)
@@ -658,12 +689,14 @@ int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t
 
 int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
                       unsigned* counters, int splits, int h, int wd, int cin, int cout,
-                      int relu, hipStream_t s) {
+                      int relu, hipStream_t s, const void* x2, int c1) {
   if (!x || !wt || !y || h < 1 || wd < 1 || cin % 16 || cout % 32 || splits < 1 ||
       splits > 9 * cin / 16 || (splits > 1 && (!work || !counters)))
     return set_error(NX_ERR_ARG, "tiled conv3x3: cin % 16 == 0, cout % 32 == 0, valid splits");
+  if (x2 && (c1 < 16 || c1 % 16 || c1 >= cin || h % 2 || wd % 2))
+    return set_error(NX_ERR_ARG, "concatenating conv3x3: 16 <= c1 < cin, c1 % 16 == 0, even size");
   conv_fwd(x, wt, bias, y, 1, h, wd, cin, cout, relu, splits > 1 ? work : nullptr, splits, s,
-           true, counters);
+           true, counters, x2, c1);
   NX_UNET_RET();
 }
 }  // namespace nx
