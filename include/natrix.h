@@ -458,6 +458,17 @@ int nx_fp64_solve(const float* b, const uint8_t* mask, int W, int H, double* x,
 int nx_fp64_solve_batch(const float* b, const uint8_t* mask, int n, int count, double* work,
                         float* xf, double tol, int max_iter, double* residuals,
                         int* iterations, void* stream);
+/* nx_fp64_solve with no allocation: the same kernels, in the same order, on
+ * a static device array of the library (about 6 MiB, held on every GPU the
+ * library runs on until the process ends) instead of hipMalloc'd buffers,
+ * so the results are bitwise those of nx_fp64_solve.  Grids of at most
+ * 512 x 512 = 262144 cells (NX_ERR_ARG beyond).  Calls on one GPU share
+ * that array: calls that overlap on one GPU race; a call does not depend on
+ * what a previous call left there.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under stream capture,
+ * serving loops, harnesses that forbid it). */
+int nx_fp64_solve_noalloc(const float* b, const uint8_t* mask, int W, int H, double* x,
+                          double tol, int max_iter, double* out, void* stream);
 /* relative L2 error of the mean-free float32 p against the mean-free x */
 int nx_fp64_compare(const float* p, const double* x, const uint8_t* mask, int W, int H,
                     double* rel, void* stream);

@@ -178,6 +178,7 @@ SIGNATURES = {
     "nx_head_fwd": [_P, _P, _P, _P, C.c_longlong, _I, _P],
     "nx_head_bwd": [_P, _P, _P, _P, _P, _P, C.c_longlong, _I, _P],
     "nx_fp64_solve": [_P, _P, _I, _I, _P, _D, _I, C.POINTER(_D), _P],
+    "nx_fp64_solve_noalloc": [_P, _P, _I, _I, _P, _D, _I, C.POINTER(_D), _P],
     "nx_fp64_compare": [_P, _P, _P, _I, _I, C.POINTER(_D), _P],
     "nx_fp64_solve_batch": [_P, _P, _I, _I, _P, _P, _D, _I, _P, _P, _P],
     "nx_copy_bandwidth": [_SZ, _I, C.POINTER(_D)],

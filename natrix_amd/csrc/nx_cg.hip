@@ -11,6 +11,7 @@
 // reduced-storage and learned pressures at full size.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/natrix.h"
@@ -220,27 +221,16 @@ int set_error(int code, const std::string& msg);
     }                                                                      \
   } while (0)
 
-extern "C" {
-
-// FP64 CG solve of A x = b (b: float32 device field, mean removed) on the
-// fluid cells of `mask`; x (device, W*H doubles, 0 on solid) gets the
-// mean-free solution.  out[0] = relative residual |P(b - A x)| / |P b|,
-// out[1] = iterations.
-int nx_fp64_solve(const float* b, const uint8_t* mask, int W, int H, double* x,
-                  double tol, int max_iter, double* out, void* stream) {
-  if (!b || !mask || !x || !out || W < 1 || H < 1 || !(tol > 0) || max_iter < 1)
-    return nx::set_error(NX_ERR_ARG, "bad fp64 solve arguments");
+// The solve on caller-provided scratch: r, d, q (n doubles each), part
+// (6 CG_BLOCKS doubles) and s (16 doubles).  Every scratch element is
+// written by a kernel before it is read, so the scratch may hold anything.
+static int fp64_solve_impl(const float* b, const uint8_t* mask, int W, int H, double* x,
+                           double tol, int max_iter, double* out, hipStream_t st, double* r,
+                           double* d, double* q, double* part, double* s) {
   const long long n = (long long)W * H;
-  const hipStream_t st = (hipStream_t)stream;
-  double *r = nullptr, *d = nullptr, *q = nullptr, *part = nullptr, *s = nullptr;
   double hs[16];
   int rc = NX_OK, it = 0;
   const unsigned nb = CG_BLOCKS;
-  CG_HIP(hipMalloc(&r, n * sizeof(double)));
-  CG_HIP(hipMalloc(&d, n * sizeof(double)));
-  CG_HIP(hipMalloc(&q, n * sizeof(double)));
-  CG_HIP(hipMalloc(&part, 6 * CG_BLOCKS * sizeof(double)));
-  CG_HIP(hipMalloc(&s, 16 * sizeof(double)));
   k_cg_init_sums<<<nb, 256, 0, st>>>(b, mask, part, n);
   k_cg_sum<<<1, 256, 0, st>>>(part, nb, s + 2);
   k_cg_sum<<<1, 256, 0, st>>>(part + CG_BLOCKS, nb, s + 3);
@@ -289,12 +279,96 @@ int nx_fp64_solve(const float* b, const uint8_t* mask, int W, int H, double* x,
     out[1] = it;
   }
 done:
+  return rc;
+}
+
+static bool fp64_solve_args_ok(const float* b, const uint8_t* mask, int W, int H,
+                               const double* x, double tol, int max_iter, const double* out) {
+  return b && mask && x && out && W >= 1 && H >= 1 && tol > 0 && max_iter >= 1;
+}
+
+// Scratch of the allocation-free solve (nx_fp64_solve_noalloc): r, d, q for
+// up to NOALLOC_MAX_CELLS cells, then part and s; about 6 MiB, held by every
+// device the library runs on until the process ends.  Host code never uses
+// this name as an address: the device address of each GPU's copy comes from
+// hipGetSymbolAddress, taken once per device (noalloc_scratch).
+constexpr long long NOALLOC_MAX_CELLS = 1ll << 18;  // 512 x 512
+constexpr long long NOALLOC_DOUBLES = 3 * NOALLOC_MAX_CELLS + 6 * CG_BLOCKS + 16;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// the current device's address of `scratch`, looked up on the first call
+// on that device
+static int noalloc_scratch(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return nx::set_error(NX_ERR_HIP, "no usable current device for the allocation-free solve");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return nx::set_error(NX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                           hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return NX_OK;
+}
+
+extern "C" {
+
+// FP64 CG solve of A x = b (b: float32 device field, mean removed) on the
+// fluid cells of `mask`; x (device, W*H doubles, 0 on solid) gets the
+// mean-free solution.  out[0] = relative residual |P(b - A x)| / |P b|,
+// out[1] = iterations.
+int nx_fp64_solve(const float* b, const uint8_t* mask, int W, int H, double* x,
+                  double tol, int max_iter, double* out, void* stream) {
+  if (!fp64_solve_args_ok(b, mask, W, H, x, tol, max_iter, out))
+    return nx::set_error(NX_ERR_ARG, "bad fp64 solve arguments");
+  const long long n = (long long)W * H;
+  double *r = nullptr, *d = nullptr, *q = nullptr, *part = nullptr, *s = nullptr;
+  int rc = NX_OK;
+  CG_HIP(hipMalloc(&r, n * sizeof(double)));
+  CG_HIP(hipMalloc(&d, n * sizeof(double)));
+  CG_HIP(hipMalloc(&q, n * sizeof(double)));
+  CG_HIP(hipMalloc(&part, 6 * CG_BLOCKS * sizeof(double)));
+  CG_HIP(hipMalloc(&s, 16 * sizeof(double)));
+  rc = fp64_solve_impl(b, mask, W, H, x, tol, max_iter, out, (hipStream_t)stream, r, d, q,
+                       part, s);
+done:
   (void)hipFree(r);
   (void)hipFree(d);
   (void)hipFree(q);
   (void)hipFree(part);
   (void)hipFree(s);
   return rc;
+}
+
+// nx_fp64_solve on the library's static device scratch: no allocation.
+int nx_fp64_solve_noalloc(const float* b, const uint8_t* mask, int W, int H, double* x,
+                          double tol, int max_iter, double* out, void* stream) {
+  if (!fp64_solve_args_ok(b, mask, W, H, x, tol, max_iter, out))
+    return nx::set_error(NX_ERR_ARG, "bad fp64 solve arguments");
+  const long long n = (long long)W * H;
+  if (n > NOALLOC_MAX_CELLS)
+    return nx::set_error(NX_ERR_ARG,
+                         "nx_fp64_solve_noalloc: " + std::to_string(W) + " x " +
+                             std::to_string(H) + " exceeds its scratch of " +
+                             std::to_string(NOALLOC_MAX_CELLS) + " cells (512 x 512)");
+  double* base = nullptr;
+  const int rc = noalloc_scratch(&base);
+  if (rc != NX_OK) return rc;
+  double* r = base;
+  double* d = r + NOALLOC_MAX_CELLS;
+  double* q = d + NOALLOC_MAX_CELLS;
+  double* part = q + NOALLOC_MAX_CELLS;
+  double* s = part + 6 * CG_BLOCKS;
+  return fp64_solve_impl(b, mask, W, H, x, tol, max_iter, out, (hipStream_t)stream, r, d, q,
+                         part, s);
 }
 
 // relative L2 error of a float32 pressure p against the FP64 solution x,
