@@ -345,17 +345,13 @@ int nx_batch_collect_timing(nx_batch* b, double* ms, int* launches);
 int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                    int h, int wd, int cin, int cout, int relu, void* stream);
 /* the same convolution with the 9 cin / 16 K steps split over `splits`
- * groups of workgroups that write float32 partial sums to
- * work[splits * n*h*w * cout] (device); the last group of each output tile
- * to finish sums the tile's partials in split order with bias and ReLU
- * (one launch).  counters (device, nx_conv3x3_split_counters(n, h, w, cout)
- * unsigned ints) must be zero on entry and are zero again on return.  More
- * workgroups for small images with many channels (cin % 16 == 0). */
+ * groups of launches that write float32 partial sums to
+ * work[splits * n*h*w * cout] (device), then summed in split order with
+ * bias and ReLU: more workgroups for small images with many channels
+ * (cin % 16 == 0) */
 int nx_conv3x3_fwd_split(const void* x, const void* w, const float* bias, void* y,
-                         float* work, unsigned* counters, int splits, int n, int h, int wd,
-                         int cin, int cout, int relu, void* stream);
-/* number of arrival counters nx_conv3x3_fwd_split needs (0 for bad shapes) */
-long long nx_conv3x3_split_counters(int n, int h, int wd, int cout);
+                         float* work, int splits, int n, int h, int wd, int cin, int cout,
+                         int relu, void* stream);
 /* dx = conv3x3 input gradient of dy; wt: scratch of 9*cin*cout bfloat16 */
 int nx_conv3x3_dgrad(const void* dy, const void* w, void* dx, void* wt, int n, int h,
                      int wd, int cin, int cout, void* stream);

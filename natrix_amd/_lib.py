@@ -155,8 +155,7 @@ SIGNATURES = {
     "nx_refine_end": [_P],
     "nx_step_finish": [_P],
     "nx_conv3x3_fwd": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
-    "nx_conv3x3_fwd_split": [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
-    "nx_conv3x3_split_counters": [_I, _I, _I, _I],
+    "nx_conv3x3_fwd_split": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
     "nx_unet_create": [_I, _I, _I, _P, _P, _P, _P, _P],
     "nx_unet_destroy": [_P],
     "nx_unet_forward": [_P, _P, _P, _P],
@@ -185,7 +184,7 @@ SIGNATURES = {
     "nx_copy_bandwidth_variant": [_SZ, _I, _I, C.POINTER(_D)],
     "nx_last_error": [],
 }
-_RESTYPE = {"nx_last_error": C.c_char_p, "nx_conv3x3_split_counters": C.c_longlong}
+_RESTYPE = {"nx_last_error": C.c_char_p}
 
 _lock = threading.Lock()
 _lib = None

@@ -147,8 +147,8 @@ int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t
 // With x2: the input is concat(upsample2(x) (c1 channels, h/2 x wd/2), x2
 // (cin - c1 channels, h x wd)), read in the fragment loads (decoder layers).
 int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
-                      unsigned* counters, int splits, int h, int wd, int cin, int cout,
-                      int relu, hipStream_t s, const void* x2 = nullptr, int c1 = 0);
+                      int splits, int h, int wd, int cin, int cout, int relu, hipStream_t s,
+                      const void* x2 = nullptr, int c1 = 0);
 
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
 // sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 8

@@ -59,8 +59,6 @@ struct nx_unet {
   uint8_t* mc = nullptr;   // coarse mask
   float* work = nullptr;   // split-K partial sums
   size_t work_elems = 0;
-  unsigned* cnt = nullptr;  // split-K arrival counters (zero between launches)
-  long long cnt_elems = 0;
   float* scal = nullptr;   // [0] = s (RMS of rc over fluid cells)
   float* cws = nullptr;    // Chebyshev workspace, 6 H W floats
   // steps 2-6 captured as one HIP graph (every pointer in them is the
@@ -184,11 +182,11 @@ int conv(nx_unet* u, int i, const void* x, void* y, int h, int w, hipStream_t s)
   const long long m = (long long)h * w;
   const int sp = conv_splits(m, u->cin[i], u->cout[i]);
   if (u->wt[i])
-    return nx::conv3x3_fwd_tiled(x, u->wt[i], u->b[i], y, u->work, u->cnt, sp, h, w,
-                                 u->cin[i], u->cout[i], 1, s);
+    return nx::conv3x3_fwd_tiled(x, u->wt[i], u->b[i], y, u->work, sp, h, w, u->cin[i],
+                                 u->cout[i], 1, s);
   if (sp > 1)
-    return nx_conv3x3_fwd_split(x, u->w[i], u->b[i], y, u->work, u->cnt, sp, 1, h, w,
-                                u->cin[i], u->cout[i], 1, s);
+    return nx_conv3x3_fwd_split(x, u->w[i], u->b[i], y, u->work, sp, 1, h, w, u->cin[i],
+                                u->cout[i], 1, s);
   return nx_conv3x3_fwd(x, u->w[i], u->b[i], y, 1, h, w, u->cin[i], u->cout[i], 1, s);
 }
 
@@ -198,7 +196,7 @@ int conv_cat(nx_unet* u, int i, const void* lo, const void* skip, int c1, void* 
              hipStream_t s) {
   const long long m = (long long)h * w;
   const int sp = conv_splits(m, u->cin[i], u->cout[i]);
-  return nx::conv3x3_fwd_tiled(lo, u->wt[i], u->b[i], y, u->work, u->cnt, sp, h, w, u->cin[i],
+  return nx::conv3x3_fwd_tiled(lo, u->wt[i], u->b[i], y, u->work, sp, h, w, u->cin[i],
                                u->cout[i], 1, s, skip, c1);
 }
 
@@ -328,8 +326,6 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
     const int sp = conv_splits(m, ci[i], co[i]);
     if (sp > 1 && (size_t)(sp * m * co[i]) > u->work_elems)
       u->work_elems = (size_t)sp * m * co[i];
-    const long long tiles = nx_conv3x3_split_counters(1, H >> lvl[i], W >> lvl[i], co[i]);
-    if (sp > 1 && tiles > u->cnt_elems) u->cnt_elems = tiles;
   }
   u->head_w = head_w;
   u->head_b = head_b;
@@ -366,11 +362,6 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
   u->rc = (float*)alloc(P0 * sizeof(float));
   u->mc = (uint8_t*)alloc(P0);
   u->work = (float*)alloc((u->work_elems ? u->work_elems : 1) * sizeof(float));
-  u->cnt = (unsigned*)alloc((u->cnt_elems ? u->cnt_elems : 1) * sizeof(unsigned));
-  // the counters start at zero; every split convolution leaves them there
-  if (ok && hipMemset(u->cnt, 0, (u->cnt_elems ? u->cnt_elems : 1) * sizeof(unsigned)) !=
-                hipSuccess)
-    ok = false;
   u->scal = (float*)alloc(4 * sizeof(float));
   u->cws = (float*)alloc(6 * P0 * sizeof(float));
   void* wcopy[14] = {};

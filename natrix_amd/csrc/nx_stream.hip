@@ -29,6 +29,8 @@
 // first chunk's carry is zero with solid flags, which only reaches rows
 // above Y0 at level k (row Y0 - k + s at pass s is the first valid one).
 // Chunks per segment: ceil((Y1 - Y0 + 2k) / S).
+#include <type_traits>
+
 #include "../../include/natrix.h"
 #include "nx_jacobi_tile.h"
 #include "nx_kernels.h"
@@ -152,7 +154,10 @@ __global__ void __launch_bounds__(WAVES * 64)
   constexpr int EX = 64 * COLS, S = ROWS * WAVES;
   static_assert(ROWS >= 2, "the pass reads two rows above a wave's first row");
   static_assert(WAVES >= 2, "wave WAVES - 1 keeps the carry in the registers wave 0 reads it into");
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // the wave index is uniform across the wave: an SGPR, so branches on it are
+  // scalar branches with no exec-mask merging
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tcol0 = COLS * lane;
   const int gx0 = (int)blockIdx.x * (EX - 2 * k) - k + tcol0;
   const int Y0 = (int)blockIdx.y * seg;
@@ -260,8 +265,11 @@ __global__ void __launch_bounds__(WAVES * 64)
     }
     if (ch + 1 < nch) load_chunk(ch + 1);
 
-    for (int s = 0; s < k; ++s) {
-      const int kind = (first && s == 0) ? P_SETUP : ((RES && s == k - 1) ? P_RES : P_ITER);
+    // one pass s of kind KIND: exchange the rows above each wave's strip
+    // (wave 0: the previous chunk's, from the carry), then update in place
+    auto pass = [&](auto kindc, int s) {
+      constexpr int KIND = decltype(kindc)::value;
+      constexpr bool it = KIND == P_ITER;
       const int par = s & 1;
       float p0[COLS], p1[COLS], p2[COLS], p3[COLS];
       int ps[COLS];
@@ -279,29 +287,26 @@ __global__ void __launch_bounds__(WAVES * 64)
         }
       }
       // publish this wave's last rows of the pass's inputs
-      {
-        const bool it = kind == P_ITER;
 #pragma unroll
-        for (int c = 0; c < COLS; ++c) {
-          const float v2 = it ? D[ROWS - 2][c] : X[ROWS - 2][c];
-          const float v1 = it ? D[ROWS - 1][c] : X[ROWS - 1][c];
-          const float a1 = it ? X[ROWS - 1][c] : R[ROWS - 1][c];
-          const float b1 = it ? R[ROWS - 1][c] : D[ROWS - 1][c];
-          const uint8_t sb = (uint8_t)(SL[ROWS - 2][c] | (SL[ROWS - 1][c] << 1));
-          pub[par][wave].v[0][tcol0 + c] = v2;
-          pub[par][wave].v[1][tcol0 + c] = v1;
-          pub[par][wave].v[2][tcol0 + c] = a1;
-          pub[par][wave].v[3][tcol0 + c] = b1;
-          pub[par][wave].s[tcol0 + c] = sb;
-          if (wave == WAVES - 1) {
-            // kept for wave 0 of the next chunk; written after the barrier
-            // below so wave 0's read of the old carry above comes first
-            p0[c] = v2;
-            p1[c] = v1;
-            p2[c] = a1;
-            p3[c] = b1;
-            ps[c] = sb;
-          }
+      for (int c = 0; c < COLS; ++c) {
+        const float v2 = it ? D[ROWS - 2][c] : X[ROWS - 2][c];
+        const float v1 = it ? D[ROWS - 1][c] : X[ROWS - 1][c];
+        const float a1 = it ? X[ROWS - 1][c] : R[ROWS - 1][c];
+        const float b1 = it ? R[ROWS - 1][c] : D[ROWS - 1][c];
+        const uint8_t sb = (uint8_t)(SL[ROWS - 2][c] | (SL[ROWS - 1][c] << 1));
+        pub[par][wave].v[0][tcol0 + c] = v2;
+        pub[par][wave].v[1][tcol0 + c] = v1;
+        pub[par][wave].v[2][tcol0 + c] = a1;
+        pub[par][wave].v[3][tcol0 + c] = b1;
+        pub[par][wave].s[tcol0 + c] = sb;
+        if (wave == WAVES - 1) {
+          // kept for wave 0 of the next chunk; written after the barrier
+          // below so wave 0's read of the old carry above comes first
+          p0[c] = v2;
+          p1[c] = v1;
+          p2[c] = a1;
+          p3[c] = b1;
+          ps[c] = sb;
         }
       }
       __syncthreads();
@@ -325,7 +330,6 @@ __global__ void __launch_bounds__(WAVES * 64)
           ps[c] = pub[par][wave - 1].s[tcol0 + c];
         }
       }
-      const int row_new0 = row0 - (s + 1);
       // no solid or out-of-grid cell among the rows this pass reads in the
       // wave (its own and the two above): no neighbour is replaced.  The
       // strip's outermost lanes read garbage across the wave edge either
@@ -336,16 +340,15 @@ __global__ void __launch_bounds__(WAVES * 64)
 #pragma unroll
         for (int c = 0; c < COLS; ++c) anys |= SL[i][c];
       const bool special = __any(anys != 0);
-      if (kind == P_ITER)
-        cheb_pass<P_ITER>(X, R, D, SL, p0, p1, p2, p3, ps, theta, cf.cd[s], cf.cr[s], row_new0,
-                          gx0, tcol0, k, Y0, Y1, W, EX, BR, acc, special);
-      else if (kind == P_SETUP)
-        cheb_pass<P_SETUP>(X, R, D, SL, p0, p1, p2, p3, ps, theta, 0.0f, 0.0f, row_new0, gx0,
-                           tcol0, k, Y0, Y1, W, EX, BR, acc, special);
-      else
-        cheb_pass<P_RES>(X, R, D, SL, p0, p1, p2, p3, ps, theta, 0.0f, 0.0f, row_new0, gx0,
-                         tcol0, k, Y0, Y1, W, EX, BR, acc, special);
-    }
+      cheb_pass<KIND>(X, R, D, SL, p0, p1, p2, p3, ps, theta, it ? cf.cd[s] : 0.0f,
+                      it ? cf.cr[s] : 0.0f, row0 - (s + 1), gx0, tcol0, k, Y0, Y1, W, EX, BR,
+                      acc, special);
+    };
+    // the setup pass (first launch), the iterations, the residual pass (RES)
+    int s = 0;
+    if (first) pass(std::integral_constant<int, P_SETUP>(), s++);
+    for (; s < (RES ? k - 1 : k); ++s) pass(std::integral_constant<int, P_ITER>(), s);
+    if (RES) pass(std::integral_constant<int, P_RES>(), s);
 
     // register row i now holds grid row row0 + i - k
 #pragma unroll
@@ -495,7 +498,10 @@ __global__ void __launch_bounds__(WAVES * 64)
   static_assert(ROWS >= 2, "the pass reads two rows above a wave's first row");
   static_assert(WAVES >= 2, "wave WAVES - 1 keeps the carry in the registers wave 0 reads it into");
   if (done != nullptr && *done != 0) return;  // block-uniform early exit
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // the wave index is uniform across the wave: an SGPR, so branches on it are
+  // scalar branches with no exec-mask merging
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tcol0 = COLS * lane;
   const int gx0 = (int)blockIdx.x * (EX - 2 * k) - k + tcol0;
   const int Y0 = (int)blockIdx.y * seg;

@@ -52,11 +52,11 @@ __device__ __forceinline__ uint16_t tobf(float f) { return (uint16_t)nx::f32_to_
 // along M.  With part == nullptr (one split) the block runs the whole K loop
 // and writes bias + ReLU in bf16; otherwise (Cin % 16 == 0) split z runs
 // K steps [z * steps_per_split, ...) of 16 channels of one tap each and
-// writes its float32 partial sums to part[z][m][co].  The last of a tile's
-// splits to finish (a per-tile arrival counter in cnt, zero on entry and
-// reset to zero by that block) then sums the tile's partials in split
-// order, adds the bias, applies ReLU and writes bf16: the same values, in
-// the same order, as a separate reduction pass would, with no extra launch.
+// writes its float32 partial sums to part[z][m][co] for k_conv_finish.
+// (Letting the last split of each tile sum the tile's partials instead, with
+// an arrival counter, measured several times slower: one workgroup per tile
+// then reads every split's partial of the tile, where the finish pass
+// spreads those reads over the whole chip.)
 // CAT (decoder convolutions, Cin % 16 == 0): the input is the channel
 // concatenation of the 2x nearest upsampling of x (C1 channels, an
 // (H / 2) x (W / 2) image) and x2 (Cin - C1 channels, H x W), read where
@@ -68,8 +68,8 @@ __global__ void __launch_bounds__(256)
     k_conv3x3(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
               const float* __restrict__ bias, uint16_t* __restrict__ y, int N,
               int H, int W, int Cin, int Cout, int relu, float* __restrict__ part,
-              int steps_per_split, unsigned* __restrict__ cnt,
-              const uint16_t* __restrict__ x2 = nullptr, int C1 = 0) {
+              int steps_per_split, const uint16_t* __restrict__ x2 = nullptr,
+              int C1 = 0) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 31, kb = lane >> 5;
   const long long M = (long long)N * H * W;
@@ -188,8 +188,7 @@ __global__ void __launch_bounds__(256)
   }
 
   if (part) {
-    const long long MC = M * Cout;
-    float* pz = part + (long long)blockIdx.z * MC;
+    float* pz = part + (long long)blockIdx.z * M * Cout;
 #pragma unroll
     for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -199,43 +198,6 @@ __global__ void __launch_bounds__(256)
         for (int e = 0; e < 16; ++e) {
           const long long m = m_base + 32 * i + 8 * (e >> 2) + 4 * kb + (e & 3);
           if (m < M) pz[m * Cout + co] = acc[i][j][e];
-        }
-      }
-    // arrival: this block's partials are visible device-wide before its
-    // count; the block that brings the count to gridDim.z reduces the tile
-    __shared__ unsigned last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned* c = cnt + blockIdx.y * gridDim.x + blockIdx.x;
-      const unsigned before = __hip_atomic_fetch_add(c, 1u, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-      last = before + 1 == gridDim.z;
-      if (last) __hip_atomic_store(c, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const float* pr = part;
-#pragma unroll
-    for (int i = 0; i < MT; ++i)
-#pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        const int co = n_base + 32 * j + r;
-        const float bv = bias ? bias[co] : 0.0f;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-          const long long m = m_base + 32 * i + 8 * (e >> 2) + 4 * kb + (e & 3);
-          if (m < M) {
-            const long long idx = m * Cout + co;
-            float v = 0.0f;
-            for (int z = 0; z < (int)gridDim.z; ++z)
-              v += __hip_atomic_load(pr + z * MC + idx, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-            v += bv;
-            if (relu) v = fmaxf(v, 0.0f);
-            y[idx] = tobf(v);
-          }
         }
       }
     return;
@@ -256,6 +218,19 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
+}
+
+// y[m][co] = bf16(relu(sum over splits z in order of part[z][m][co] + bias))
+__global__ void __launch_bounds__(256)
+    k_conv_finish(const float* __restrict__ part, const float* __restrict__ bias,
+                  uint16_t* __restrict__ y, long long MC, int Cout, int splits, int relu) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= MC) return;
+  float v = 0.0f;
+  for (int z = 0; z < splits; ++z) v += part[z * MC + i];
+  v += bias ? bias[i % Cout] : 0.0f;
+  if (relu) v = fmaxf(v, 0.0f);
+  y[i] = tobf(v);
 }
 
 // w[co][tap][ci] -> w'[ci][8 - tap][co]
@@ -487,8 +462,8 @@ extern "C" {
 
 static int conv_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                     int h, int wd, int cin, int cout, int relu, float* part, int splits,
-                    hipStream_t s, bool tiled = false, unsigned* cnt = nullptr,
-                    const void* x2 = nullptr, int c1 = 0) {
+                    hipStream_t s, bool tiled = false, const void* x2 = nullptr,
+                    int c1 = 0) {
   const long long M = (long long)n * h * wd;
   const int steps = 9 * cin / 16;
   const int per = part ? (steps + splits - 1) / splits : 0;
@@ -502,37 +477,35 @@ static int conv_fwd(const void* x, const void* w, const float* bias, void* y, in
     if (cout % 64 == 0) {
       dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64, used);
       k_conv3x3<2, 2, true, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout,
-                                                       relu, part, per, cnt, x2s, c1);
+                                                       relu, part, per, x2s, c1);
     } else {
       dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 32, used);
       k_conv3x3<2, 1, true, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout,
-                                                       relu, part, per, cnt, x2s, c1);
+                                                       relu, part, per, x2s, c1);
     }
-    return 0;
-  }
-  if (cout % 64 == 0) {
+  } else if (cout % 64 == 0) {
     dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 64, used);
     if (tiled)
       k_conv3x3<2, 2, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
-                                                 part, per, cnt);
+                                                 part, per);
     else
       k_conv3x3<2, 2, false><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
-                                                  part, per, cnt);
+                                                  part, per);
   } else {
     dim3 grid((unsigned)((M + 4 * 64 - 1) / (4 * 64)), cout / 32, used);
     if (tiled)
       k_conv3x3<2, 1, true><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
-                                                 part, per, cnt);
+                                                 part, per);
     else
       k_conv3x3<2, 1, false><<<grid, 256, 0, s>>>(xs, ws, bias, ys, n, h, wd, cin, cout, relu,
-                                                  part, per, cnt);
+                                                  part, per);
+  }
+  if (part) {
+    const long long mc = M * cout;
+    k_conv_finish<<<(unsigned)((mc + 255) / 256), 256, 0, s>>>(part, bias, (uint16_t*)y, mc,
+                                                               cout, used, relu);
   }
   return 0;
-}
-
-// arrival counters a split forward convolution needs (one per output tile)
-static long long conv_tiles(long long M, int cout) {
-  return ((M + 4 * 64 - 1) / (4 * 64)) * (cout % 64 == 0 ? cout / 64 : cout / 32);
 }
 
 // w[co][tap][ci] (KRSC) -> wt[tap][ci / 16][co][16] for the WT convolution
@@ -557,20 +530,14 @@ int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int
 }
 
 int nx_conv3x3_fwd_split(const void* x, const void* w, const float* bias, void* y,
-                         float* work, unsigned* counters, int splits, int n, int h, int wd,
-                         int cin, int cout, int relu, void* stream) {
-  if (!x || !w || !y || !work || !counters || n < 1 || h < 1 || wd < 1 || cin % 16 ||
-      cout % 32 || splits < 1 || splits > 9 * cin / 16)
+                         float* work, int splits, int n, int h, int wd, int cin, int cout,
+                         int relu, void* stream) {
+  if (!x || !w || !y || !work || n < 1 || h < 1 || wd < 1 || cin % 16 || cout % 32 ||
+      splits < 1 || splits > 9 * cin / 16)
     return nx::set_error(NX_ERR_ARG,
                          "conv3x3 split: cin % 16 == 0, cout % 32 == 0, 1 <= splits <= 9 cin / 16");
-  conv_fwd(x, w, bias, y, n, h, wd, cin, cout, relu, work, splits, NX_U(stream), false,
-           counters);
+  conv_fwd(x, w, bias, y, n, h, wd, cin, cout, relu, work, splits, NX_U(stream));
   NX_UNET_RET();
-}
-
-long long nx_conv3x3_split_counters(int n, int h, int wd, int cout) {
-  if (n < 1 || h < 1 || wd < 1 || cout < 32 || cout % 32) return 0;
-  return conv_tiles((long long)n * h * wd, cout);
 }
 
 int nx_conv3x3_dgrad(const void* dy, const void* w, void* dx, void* wt, int n, int h,
@@ -688,15 +655,15 @@ int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t
 }
 
 int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
-                      unsigned* counters, int splits, int h, int wd, int cin, int cout,
-                      int relu, hipStream_t s, const void* x2, int c1) {
+                      int splits, int h, int wd, int cin, int cout, int relu, hipStream_t s,
+                      const void* x2, int c1) {
   if (!x || !wt || !y || h < 1 || wd < 1 || cin % 16 || cout % 32 || splits < 1 ||
-      splits > 9 * cin / 16 || (splits > 1 && (!work || !counters)))
+      splits > 9 * cin / 16 || (splits > 1 && !work))
     return set_error(NX_ERR_ARG, "tiled conv3x3: cin % 16 == 0, cout % 32 == 0, valid splits");
   if (x2 && (c1 < 16 || c1 % 16 || c1 >= cin || h % 2 || wd % 2))
     return set_error(NX_ERR_ARG, "concatenating conv3x3: 16 <= c1 < cin, c1 % 16 == 0, even size");
   conv_fwd(x, wt, bias, y, 1, h, wd, cin, cout, relu, splits > 1 ? work : nullptr, splits, s,
-           true, counters, x2, c1);
+           true, x2, c1);
   NX_UNET_RET();
 }
 }  // namespace nx

@@ -43,7 +43,7 @@ def conv_splits(m, cin, cout):
     """K splits of a forward convolution over m pixels: about one workgroup
     per compute unit (~256) for small images with many channels, at least 8
     K steps of 16 channels each (fewer, longer splits: every split writes
-    and the last one re-reads a float32 partial of the whole output); 1
+    and the finish pass re-reads a float32 partial of the whole output); 1
     (no split) when the image alone fills the chip."""
     if cin % 16:
         return 1
@@ -64,10 +64,8 @@ class Conv3x3Fn(torch.autograd.Function):
         if splits > 1:
             work = torch.empty(splits * n * h * wd * cout, dtype=torch.float32,
                                device=x.device)
-            tiles = _lib.lib().nx_conv3x3_split_counters(n, h, wd, cout)
-            cnt = torch.zeros(tiles, dtype=torch.int32, device=x.device)
-            _call("nx_conv3x3_fwd_split", _p(x), _p(wb), _p(b), _p(y), _p(work), _p(cnt),
-                  splits, n, h, wd, cin, cout, 1 if relu else 0, _s())
+            _call("nx_conv3x3_fwd_split", _p(x), _p(wb), _p(b), _p(y), _p(work), splits,
+                  n, h, wd, cin, cout, 1 if relu else 0, _s())
         else:
             _call("nx_conv3x3_fwd", _p(x), _p(wb), _p(b), _p(y), n, h, wd, cin, cout,
                   1 if relu else 0, _s())

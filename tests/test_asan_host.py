@@ -76,13 +76,10 @@ for H, W, base in ((7, 128, 32), (128, 136, 32), (64, 64, 24), (64, 64, 0)):
     assert not u.value
 assert lib.nx_unet_create(64, 64, 32, None, ptrs, None, None, C.byref(u)) == ARG
 assert lib.nx_unet_set_bypass(None, 1) == ARG
-assert lib.nx_conv3x3_split_counters(1, 32, 32, 64) == 4  # 256-pixel x 64-channel tiles
-assert lib.nx_conv3x3_split_counters(1, 16, 16, 96) == 3  # 32-channel tiles
-assert lib.nx_conv3x3_split_counters(1, 16, 16, 48) == 0
 one = C.c_void_p(1)
-assert lib.nx_conv3x3_fwd_split(one, one, one, one, one, None, 2, 1, 16, 16, 64, 64, 1,
+assert lib.nx_conv3x3_fwd_split(one, one, one, one, None, 2, 1, 16, 16, 64, 64, 1,
                                 None) == ARG
-assert lib.nx_conv3x3_fwd_split(one, one, one, one, one, one, 99, 1, 16, 16, 64, 64, 1,
+assert lib.nx_conv3x3_fwd_split(one, one, one, one, one, 99, 1, 16, 16, 64, 64, 1,
                                 None) == ARG
 assert lib.nx_fp64_solve_batch(one, one, 129, 4, one, one, 1e-6, 10, one, one, None) == ARG
 assert lib.nx_fp64_solve_batch(one, one, 64, 0, one, one, 1e-6, 10, one, one, None) == ARG
