@@ -26,6 +26,8 @@
 // its last pass on the FP64 residual of the result.  Every pass evaluates
 // each cell's expressions in a fixed order without contraction, so the
 // result is bitwise independent of the tile shape and of k.
+#include <type_traits>
+
 #include "../../include/natrix.h"
 #include "nx_jacobi_tile.h"
 #include "nx_kernels.h"
@@ -53,7 +55,8 @@ __global__ void __launch_bounds__(WAVES * 64)
               const float* __restrict__ dapp, const double* __restrict__ line) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform: a scalar register, so branches on it are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int VX = EX - 2 * k, VY = EY - 2 * k;
   const int gx0 = blockIdx.x * VX - k + COLS * lane;
   const int gy0 = blockIdx.y * VY - k + wave * ROWS;
@@ -65,62 +68,90 @@ __global__ void __launch_bounds__(WAVES * 64)
   float x[ROWS][COLS], r[ROWS][COLS], d[ROWS][COLS];
   float bb[RES ? ROWS : 1][RES ? COLS : 1];  // b for the residual pass
   int sol[ROWS][COLS];
+  // Every load is unconditional, from a clamped in-grid address, and all of
+  // a lane's loads are issued before the first is used; cells outside the
+  // grid are replaced afterwards.  The first launch holds b in r until the
+  // setup pass (its d input is never read: d is set by the setup).
+  const float* rsrc = first ? b : rin;
+  const bool apply = dapp != nullptr && (float)line[0] != 0.0f;
+  const float a_line = apply ? (float)line[0] : 0.0f;
+  if (vec) {
+    const int cx = min(max(gx0, 0), W - 2);  // even, like gx0
+    const bool col_in = gx0 >= 0 && gx0 < W;
+    float2 lx[ROWS], lr[ROWS], ld[ROWS], lb[RES ? ROWS : 1];
+    uint16_t lm[ROWS];
 #pragma unroll
-  for (int i = 0; i < ROWS; ++i) {
-    const int gy = gy0 + i;
-    if (vec) {
-      const bool in = gy >= 0 && gy < H && gx0 >= 0 && gx0 < W;
-      const int idx = in ? gy * W + gx0 : 0;
-      const float2 z = make_float2(0.0f, 0.0f);
-      const float2 xv = in ? *reinterpret_cast<const float2*>(xin + idx) : z;
-      // the first launch holds b in r until the setup pass
-      const float2 rv = in ? *reinterpret_cast<const float2*>((first ? b : rin) + idx) : z;
-      const float2 dv = (in && !first) ? *reinterpret_cast<const float2*>(din + idx) : z;
-      const uint16_t mm = in ? *reinterpret_cast<const uint16_t*>(mask + idx) : 0x0101;
-      x[i][0] = xv.x;
-      x[i][COLS - 1] = xv.y;
-      r[i][0] = rv.x;
-      r[i][COLS - 1] = rv.y;
-      d[i][0] = dv.x;
-      d[i][COLS - 1] = dv.y;
-      sol[i][0] = (mm & 0xFF) != 0;
-      sol[i][COLS - 1] = (mm >> 8) != 0;
+    for (int i = 0; i < ROWS; ++i) {
+      const int idx = min(max(gy0 + i, 0), H - 1) * W + cx;
+      lx[i] = *reinterpret_cast<const float2*>(xin + idx);
+      lr[i] = *reinterpret_cast<const float2*>(rsrc + idx);
+      ld[i] = *reinterpret_cast<const float2*>(din + idx);
+      lm[i] = *reinterpret_cast<const uint16_t*>(mask + idx);
+      if (RES) lb[RES ? i : 0] = *reinterpret_cast<const float2*>(b + idx);
+    }
+    float2 la[ROWS];
+    if (apply) {
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i)
+        la[i] = *reinterpret_cast<const float2*>(dapp + min(max(gy0 + i, 0), H - 1) * W + cx);
+    }
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i) {
+      const bool in = col_in && gy0 + i >= 0 && gy0 + i < H;
+      x[i][0] = in ? lx[i].x : 0.0f;
+      x[i][COLS - 1] = in ? lx[i].y : 0.0f;
+      r[i][0] = in ? lr[i].x : 0.0f;
+      r[i][COLS - 1] = in ? lr[i].y : 0.0f;
+      d[i][0] = (in && !first) ? ld[i].x : 0.0f;
+      d[i][COLS - 1] = (in && !first) ? ld[i].y : 0.0f;
+      sol[i][0] = in ? (lm[i] & 0xFF) != 0 : 1;
+      sol[i][COLS - 1] = in ? (lm[i] >> 8) != 0 : 1;
       if (RES) {
-        const float2 bv = in ? *reinterpret_cast<const float2*>(b + idx) : z;
-        bb[RES ? i : 0][0] = bv.x;
-        bb[RES ? i : 0][RES ? COLS - 1 : 0] = bv.y;
+        bb[RES ? i : 0][0] = in ? lb[RES ? i : 0].x : 0.0f;
+        bb[RES ? i : 0][RES ? COLS - 1 : 0] = in ? lb[RES ? i : 0].y : 0.0f;
       }
-      continue;
-    }
-#pragma unroll
-    for (int c = 0; c < COLS; ++c) {
-      const int gx = gx0 + c;
-      const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
-      const int idx = in ? gy * W + gx : 0;
-      x[i][c] = in ? xin[idx] : 0.0f;
-      r[i][c] = in ? (first ? b[idx] : rin[idx]) : 0.0f;
-      d[i][c] = (in && !first) ? din[idx] : 0.0f;
-      sol[i][c] = in ? (mask[idx] != 0) : 1;
-      if (RES) bb[RES ? i : 0][RES ? c : 0] = in ? b[idx] : 0.0f;
-    }
-  }
-
-  // a pending line-search step x += a d on the fluid cells, as the apply
-  // of the line search would have stored it (first launch only)
-  if (dapp != nullptr) {
-    const float a = (float)line[0];
-    if (a != 0.0f) {
-#pragma unroll
-      for (int i = 0; i < ROWS; ++i) {
-        const int gy = gy0 + i;
-#pragma unroll
-        for (int c = 0; c < COLS; ++c) {
-          const int gx = gx0 + c;
-          if (gy >= 0 && gy < H && gx >= 0 && gx < W && !sol[i][c])
-            x[i][c] = x[i][c] + a * dapp[gy * W + gx];
-        }
+      // a pending line-search step x += a d on the fluid cells, as the
+      // apply of the line search would have stored it (first launch only)
+      if (apply) {
+        if (in && !sol[i][0]) x[i][0] = x[i][0] + a_line * la[i].x;
+        if (in && !sol[i][COLS - 1]) x[i][COLS - 1] = x[i][COLS - 1] + a_line * la[i].y;
       }
     }
+  } else {
+    float lx[ROWS][COLS], lr[ROWS][COLS], ld[ROWS][COLS], lb[RES ? ROWS : 1][RES ? COLS : 1];
+    uint8_t lm[ROWS][COLS];
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const int idx = min(max(gy0 + i, 0), H - 1) * W + min(max(gx0 + c, 0), W - 1);
+        lx[i][c] = xin[idx];
+        lr[i][c] = rsrc[idx];
+        ld[i][c] = din[idx];
+        lm[i][c] = mask[idx];
+        if (RES) lb[RES ? i : 0][RES ? c : 0] = b[idx];
+      }
+    float la[ROWS][COLS];
+    if (apply) {
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+        for (int c = 0; c < COLS; ++c)
+          la[i][c] = dapp[min(max(gy0 + i, 0), H - 1) * W + min(max(gx0 + c, 0), W - 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const int gy = gy0 + i, gx = gx0 + c;
+        const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        x[i][c] = in ? lx[i][c] : 0.0f;
+        r[i][c] = in ? lr[i][c] : 0.0f;
+        d[i][c] = (in && !first) ? ld[i][c] : 0.0f;
+        sol[i][c] = in ? (lm[i][c] != 0) : 1;
+        if (RES) bb[RES ? i : 0][RES ? c : 0] = in ? lb[RES ? i : 0][RES ? c : 0] : 0.0f;
+        if (apply && in && !sol[i][c]) x[i][c] = x[i][c] + a_line * la[i][c];
+      }
   }
 
   __shared__ float xch[2][2][WAVES][EX];
@@ -143,87 +174,137 @@ __global__ void __launch_bounds__(WAVES * 64)
     for (int c = 0; c < COLS; ++c) {
       const int L = c > 0 ? sol[i][c - 1] : sl;
       const int R = c < COLS - 1 ? sol[i][c + 1] : sr;
-      const int B = i > 0 ? sol[i - 1][c] : (wave > 0 ? sxch[1][wave - 1][tcol0 + c] : 1);
+      // the rows beyond the tile's first and last count as fluid: their
+      // cells are outside the valid cone, and the waves holding them then
+      // take the fast path too
+      const int B = i > 0 ? sol[i - 1][c] : (wave > 0 ? sxch[1][wave - 1][tcol0 + c] : 0);
       const int T = i < ROWS - 1 ? sol[i + 1][c]
-                                 : (wave < WAVES - 1 ? sxch[0][wave + 1][tcol0 + c] : 1);
+                                 : (wave < WAVES - 1 ? sxch[0][wave + 1][tcol0 + c] : 0);
       cd[i][c] = L | (R << 1) | (B << 2) | (T << 3);
       any_nbr |= cd[i][c] != 0;
     }
   }
+  // No solid or out-of-grid neighbour anywhere in the wave's cells: the
+  // passes run a copy of the stencil with no neighbour replacement.  The
+  // flag is wave-uniform, so the choice is one scalar branch per launch.
   const bool special = __any(any_nbr);
-
-  // f(i, c, centre, x1, x2, y1, y2) for every tile cell with the neighbours
-  // of v (garbage on the tile's edge cells, which lie outside the valid cone)
-  auto stencil = [&](const float (&v)[ROWS][COLS], int buf, auto&& f) {
-#pragma unroll
-    for (int c = 0; c < COLS; ++c) {
-      xch[buf][0][wave][tcol0 + c] = v[0][c];
-      xch[buf][1][wave][tcol0 + c] = v[ROWS - 1][c];
-    }
-    __syncthreads();
-    float prev[COLS], above[COLS];
-#pragma unroll
-    for (int c = 0; c < COLS; ++c) {
-      prev[c] = wave > 0 ? xch[buf][1][wave - 1][tcol0 + c] : v[0][c];
-      above[c] = wave < WAVES - 1 ? xch[buf][0][wave + 1][tcol0 + c] : v[ROWS - 1][c];
-    }
-#pragma unroll
-    for (int i = 0; i < ROWS; ++i) {
-      const float lft = lane_from_left(v[i][COLS - 1]);
-      const float rgt = lane_from_right(v[i][0]);
-#pragma unroll
-      for (int c = 0; c < COLS; ++c) {
-        const float vc = v[i][c];
-        float x1 = c > 0 ? v[i][c - 1] : lft;
-        float x2 = c < COLS - 1 ? v[i][c + 1] : rgt;
-        float y1 = i > 0 ? v[i - 1][c] : prev[c];
-        float y2 = i < ROWS - 1 ? v[i + 1][c] : above[c];
-        if (special) {
-          const int code = cd[i][c];
-          x1 = (code & 1) ? vc : x1;
-          x2 = (code & 2) ? vc : x2;
-          y1 = (code & 4) ? vc : y1;
-          y2 = (code & 8) ? vc : y2;
-        }
-        f(i, c, vc, x1, x2, y1, y2);
-      }
-    }
-  };
-  // out = L v = ((((x1 + x2) + y1) + y2) - 4 v)
-  float lv[ROWS][COLS];
-  auto lap = [&](const float (&v)[ROWS][COLS], int buf) {
-    stencil(v, buf, [&](int i, int c, float vc, float x1, float x2, float y1, float y2) {
-      lv[i][c] = (((x1 + x2) + y1) + y2) - 4.0f * vc;
-    });
-  };
+  // the rows above and below a wave's strip come from its neighbours in the
+  // block; the first and last waves read their own published rows instead
+  // (tile rows outside every pass's valid cone, so any finite value serves),
+  // which keeps the LDS reads unconditional
+  const int wave_up = wave > 0 ? wave - 1 : 0;
+  const int wave_dn = wave < WAVES - 1 ? wave + 1 : WAVES - 1;
 
   int s = 0;
-  if (first) {
-    lap(x, 0);
-#pragma unroll
-    for (int i = 0; i < ROWS; ++i)
-#pragma unroll
-      for (int c = 0; c < COLS; ++c) {
-        r[i][c] = lv[i][c] - r[i][c];
-        d[i][c] = r[i][c] / theta;
-      }
-    s = 1;
-  }
-  const int iter_end = RES ? k - 1 : k;
-  for (; s < iter_end; ++s) {
-    lap(d, s & 1);
-    const float cdd = cf.cd[s], crr = cf.cr[s];
-#pragma unroll
-    for (int i = 0; i < ROWS; ++i)
+  // passes with (SP = true) or without neighbour replacement
+  auto run = [&](auto spc) {
+    constexpr bool SP = decltype(spc)::value;
+    // f(i, c, centre, x1, x2, y1, y2) for every tile cell with the
+    // neighbours of v (garbage on the tile's edge cells, which lie outside
+    // the valid cone: lane 0's left and lane 63's right neighbour read 0)
+    auto stencil = [&](const float (&v)[ROWS][COLS], int buf, auto&& f) {
 #pragma unroll
       for (int c = 0; c < COLS; ++c) {
-        x[i][c] = x[i][c] + d[i][c];
-        r[i][c] = r[i][c] + lv[i][c];
-        d[i][c] = cdd * d[i][c] + crr * r[i][c];
+        xch[buf][0][wave][tcol0 + c] = v[0][c];
+        xch[buf][1][wave][tcol0 + c] = v[ROWS - 1][c];
       }
-  }
+      __syncthreads();
+      float prev[COLS], above[COLS];
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        prev[c] = xch[buf][1][wave_up][tcol0 + c];
+        above[c] = xch[buf][0][wave_dn][tcol0 + c];
+      }
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i) {
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) {
+          const float vc = v[i][c];
+          float x1 = c > 0 ? v[i][c - 1] : jt::lane_shr0(v[i][COLS - 1]);
+          float x2 = c < COLS - 1 ? v[i][c + 1] : jt::lane_shl0(v[i][0]);
+          float y1 = i > 0 ? v[i - 1][c] : prev[c];
+          float y2 = i < ROWS - 1 ? v[i + 1][c] : above[c];
+          if (SP) {
+            const int code = cd[i][c];
+            x1 = (code & 1) ? vc : x1;
+            x2 = (code & 2) ? vc : x2;
+            y1 = (code & 4) ? vc : y1;
+            y2 = (code & 8) ? vc : y2;
+          }
+          f(i, c, vc, x1, x2, y1, y2);
+        }
+      }
+    };
+    // lv = L v = ((((x1 + x2) + y1) + y2) - 4 v): 4 v is exact, so the
+    // fused multiply-add rounds once, like the subtraction
+    float lv[ROWS][COLS];
+    auto lap = [&](const float (&v)[ROWS][COLS], int buf) {
+      stencil(v, buf, [&](int i, int c, float vc, float x1, float x2, float y1, float y2) {
+        lv[i][c] = __builtin_fmaf(-4.0f, vc, ((x1 + x2) + y1) + y2);
+      });
+    };
+
+    if (first) {
+      lap(x, 0);
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) {
+          r[i][c] = lv[i][c] - r[i][c];
+          d[i][c] = r[i][c] / theta;
+        }
+      s = 1;
+    }
+    const int iter_end = RES ? k - 1 : k;
+    for (; s < iter_end; ++s) {
+      lap(d, s & 1);
+      const float cdd = cf.cd[s], crr = cf.cr[s];
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) {
+          x[i][c] = x[i][c] + d[i][c];
+          r[i][c] = r[i][c] + lv[i][c];
+          d[i][c] = cdd * d[i][c] + crr * r[i][c];
+        }
+    }
+  };
+  if (special)
+    run(std::true_type());
+  else
+    run(std::false_type());
 
   if (RES) {
+    // the residual pass, with neighbour replacement wherever it applies
+    auto stencil = [&](const float (&v)[ROWS][COLS], int buf, auto&& f) {
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        xch[buf][0][wave][tcol0 + c] = v[0][c];
+        xch[buf][1][wave][tcol0 + c] = v[ROWS - 1][c];
+      }
+      __syncthreads();
+      float prev[COLS], above[COLS];
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        prev[c] = xch[buf][1][wave_up][tcol0 + c];
+        above[c] = xch[buf][0][wave_dn][tcol0 + c];
+      }
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i) {
+        const float lft = lane_from_left(v[i][COLS - 1]);
+        const float rgt = lane_from_right(v[i][0]);
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) {
+          const float vc = v[i][c];
+          const int code = cd[i][c];
+          const float x1 = (code & 1) ? vc : (c > 0 ? v[i][c - 1] : lft);
+          const float x2 = (code & 2) ? vc : (c < COLS - 1 ? v[i][c + 1] : rgt);
+          const float y1 = (code & 4) ? vc : (i > 0 ? v[i - 1][c] : prev[c]);
+          const float y2 = (code & 8) ? vc : (i < ROWS - 1 ? v[i + 1][c] : above[c]);
+          f(i, c, vc, x1, x2, y1, y2);
+        }
+      }
+    };
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     stencil(x, s & 1, [&](int i, int c, float vc, float x1, float x2, float y1, float y2) {
       const int trow = trow0 + i, tcol = tcol0 + c, gy = gy0 + i, gx = gx0 + c;

@@ -1,6 +1,8 @@
 // Temporal-blocked Jacobi tile (shared by the single-grid and batched
 // kernels); see nx_jacobi.hip for the scheme.
 #pragma once
+#include <type_traits>
+
 #include "nx_common.h"
 
 namespace nx {
@@ -33,6 +35,18 @@ __device__ __forceinline__ int ilane_from_right(int v) {
   return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);
 }
 
+// The same shifts with the edge lane reading 0 (bound_ctrl): no old value
+// to keep, so the compiler folds the shift into the instruction that
+// consumes it as a DPP source operand (v_add_f32_dpp), at no cost of its
+// own.  For callers whose edge lanes hold cells outside the valid region.
+__device__ __forceinline__ float lane_shr0(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ float lane_shl0(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
 // ROWS x COLS cells per lane: lane `l` of wave `w` owns tile columns
 // [COLS*l, COLS*l + COLS) and tile rows [ROWS*w, ROWS*w + ROWS).
 // k sweeps on tile (bx, by) of the (VX x VY)-interior tiling; `rid` is the
@@ -52,7 +66,8 @@ __device__ __forceinline__ void jacobi_tb_tile(
     double* __restrict__ red, int rid, float* __restrict__ pout32 = nullptr) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform: a scalar register, so branches on it are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int VX = EX - 2 * k, VY = EY - 2 * k;
   const int gx0 = bx * VX - k + COLS * lane;
   const int gy0 = by * VY - k + wave * ROWS;
@@ -62,43 +77,53 @@ __device__ __forceinline__ void jacobi_tb_tile(
 
   float p[ROWS][COLS], bb[ROWS][COLS];
   int sol[ROWS][COLS];  // cell is solid or outside the grid
+  // Every load is unconditional, from a clamped in-grid address, and all of
+  // a lane's loads are issued before the first is used; cells outside the
+  // grid are replaced afterwards.
+  if (vec) {
+    const int cx = min(max(gx0, 0), W - 2);  // even, like gx0
+    const bool col_in = gx0 >= 0 && gx0 < W;
+    float2 lp[ROWS], lb[ROWS];
+    uint16_t lm[ROWS];
 #pragma unroll
-  for (int r = 0; r < ROWS; ++r) {
-    const int gy = gy0 + r;
-    const bool row_in = gy >= 0 && gy < H;
-    if (vec) {
-      const bool in = row_in && gx0 >= 0 && gx0 < W;
-      float2 pv = make_float2(0.0f, 0.0f), bv = pv;
-      int m0 = 1, m1 = 1;
-      if (in) {
-        const int idx = gy * W + gx0;
-        pv = S::ldp(pin, idx);
-        bv = S::ldp(b, idx);
-        const uint16_t mm = *reinterpret_cast<const uint16_t*>(mask + idx);
-        m0 = (mm & 0xFF) != 0;
-        m1 = (mm >> 8) != 0;
-      }
-      p[r][0] = pv.x;
-      p[r][COLS - 1] = pv.y;
-      bb[r][0] = bv.x;
-      bb[r][COLS - 1] = bv.y;
-      sol[r][0] = m0;
-      sol[r][COLS - 1] = m1;
-    } else {
+    for (int r = 0; r < ROWS; ++r) {
+      const int idx = min(max(gy0 + r, 0), H - 1) * W + cx;
+      lp[r] = S::ldp(pin, idx);
+      lb[r] = S::ldp(b, idx);
+      lm[r] = *reinterpret_cast<const uint16_t*>(mask + idx);
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const bool in = col_in && gy0 + r >= 0 && gy0 + r < H;
+      p[r][0] = in ? lp[r].x : 0.0f;
+      p[r][COLS - 1] = in ? lp[r].y : 0.0f;
+      bb[r][0] = in ? lb[r].x : 0.0f;
+      bb[r][COLS - 1] = in ? lb[r].y : 0.0f;
+      sol[r][0] = in ? (lm[r] & 0xFF) != 0 : 1;
+      sol[r][COLS - 1] = in ? (lm[r] >> 8) != 0 : 1;
+    }
+  } else {
+    float lp[ROWS][COLS], lb[ROWS][COLS];
+    uint8_t lm[ROWS][COLS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
 #pragma unroll
       for (int c = 0; c < COLS; ++c) {
-        const int gx = gx0 + c;
-        p[r][c] = 0.0f;
-        bb[r][c] = 0.0f;
-        sol[r][c] = 1;
-        if (row_in && gx >= 0 && gx < W) {
-          const int idx = gy * W + gx;
-          p[r][c] = S::ld(pin, idx);
-          bb[r][c] = S::ld(b, idx);
-          sol[r][c] = mask[idx] != 0;
-        }
+        const int idx = min(max(gy0 + r, 0), H - 1) * W + min(max(gx0 + c, 0), W - 1);
+        lp[r][c] = S::ld(pin, idx);
+        lb[r][c] = S::ld(b, idx);
+        lm[r][c] = mask[idx];
       }
-    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+      for (int c = 0; c < COLS; ++c) {
+        const int gy = gy0 + r, gx = gx0 + c;
+        const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        p[r][c] = in ? lp[r][c] : 0.0f;
+        bb[r][c] = in ? lb[r][c] : 0.0f;
+        sol[r][c] = in ? (lm[r][c] != 0) : 1;
+      }
   }
 
   __shared__ float xch[2][2][WAVES][EX];
@@ -122,10 +147,13 @@ __device__ __forceinline__ void jacobi_tb_tile(
     for (int c = 0; c < COLS; ++c) {
       const int L = c > 0 ? sol[r][c - 1] : sl;
       const int R = c < COLS - 1 ? sol[r][c + 1] : sr;
+      // the rows beyond the tile's first and last count as fluid: their
+      // cells are outside the valid cone, and the waves holding them then
+      // take the fast path too
       const int B = r > 0 ? sol[r - 1][c]
-                          : (wave > 0 ? sxch[1][wave - 1][COLS * lane + c] : 1);
+                          : (wave > 0 ? sxch[1][wave - 1][COLS * lane + c] : 0);
       const int T = r < ROWS - 1 ? sol[r + 1][c]
-                                 : (wave < WAVES - 1 ? sxch[0][wave + 1][COLS * lane + c] : 1);
+                                 : (wave < WAVES - 1 ? sxch[0][wave + 1][COLS * lane + c] : 0);
       const int code = L | (R << 1) | (B << 2) | (T << 3) | (sol[r][c] << 4);
       cd[r][c] = code;
       any_nbr |= (code & 0xF) != 0;
@@ -177,21 +205,75 @@ __device__ __forceinline__ void jacobi_tb_tile(
     s0 = 1;
   }
 
-  for (int s = s0; s < k; ++s) {
-    const int buf = s & 1;
+  // the rows above and below a wave's strip come from its neighbours in the
+  // block; the first and last waves read their own published rows instead
+  // (tile rows outside every sweep's valid cone, so any finite value
+  // serves), which keeps the LDS reads unconditional
+  const int wave_up = wave > 0 ? wave - 1 : 0;
+  const int wave_dn = wave < WAVES - 1 ? wave + 1 : WAVES - 1;
+  auto exchange = [&](int buf, float(&prev)[COLS], float(&above)[COLS]) {
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       xch[buf][0][wave][tcol0 + c] = p[0][c];
       xch[buf][1][wave][tcol0 + c] = p[ROWS - 1][c];
     }
     __syncthreads();
-    float prev[COLS], above[COLS];
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
-      prev[c] = wave > 0 ? xch[buf][1][wave - 1][tcol0 + c] : p[0][c];
-      above[c] = wave < WAVES - 1 ? xch[buf][0][wave + 1][tcol0 + c] : p[ROWS - 1][c];
+      prev[c] = xch[buf][1][wave_up][tcol0 + c];
+      above[c] = xch[buf][0][wave_dn][tcol0 + c];
     }
-    const bool last = (s == k - 1) && red != nullptr;
+  };
+
+  // Sweeps in two phases: the neighbour sums of every cell, then the
+  // update in place (p is dead once the sums are formed, so no register
+  // copies).  SP: with neighbour replacement (a solid or out-of-grid
+  // neighbour somewhere in the wave); wave-uniform, one scalar branch.
+  // The last sweep of a launch that accumulates the residual is peeled.
+  int s = s0;
+  const int s_fast = red != nullptr ? k - 1 : k;
+  auto sweeps = [&](auto spc) {
+    constexpr bool SP = decltype(spc)::value;
+    for (; s < s_fast; ++s) {
+      float prev[COLS], above[COLS];
+      exchange(s & 1, prev, above);
+      float sum[ROWS][COLS];
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) {
+          const float pc = p[r][c];
+          // lane 0's left and lane 63's right neighbour read 0: tile
+          // columns outside the valid cone
+          float x1 = c > 0 ? p[r][c - 1] : lane_shr0(p[r][COLS - 1]);
+          float x2 = c < COLS - 1 ? p[r][c + 1] : lane_shl0(p[r][0]);
+          float y1 = r > 0 ? p[r - 1][c] : prev[c];
+          float y2 = r < ROWS - 1 ? p[r + 1][c] : above[c];
+          if (SP) {
+            const int code = cd[r][c];
+            x1 = (code & 1) ? pc : x1;
+            x2 = (code & 2) ? pc : x2;
+            y1 = (code & 4) ? pc : y1;
+            y2 = (code & 8) ? pc : y2;
+          }
+          sum[r][c] = ((x1 + x2) + y1) + y2;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) p[r][c] = (sum[r][c] - bb[r][c]) * 0.25f;
+    }
+  };
+  if (special)
+    sweeps(std::true_type());
+  else
+    sweeps(std::false_type());
+
+  if (red != nullptr && s < k) {
+    // the last sweep, accumulating the FP64 residual of its input iterate
+    float prev[COLS], above[COLS];
+    exchange(s & 1, prev, above);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       const float lft = lane_from_left(p[r][COLS - 1]);
@@ -200,29 +282,19 @@ __device__ __forceinline__ void jacobi_tb_tile(
 #pragma unroll
       for (int c = 0; c < COLS; ++c) {
         const float pc = p[r][c];
-        float x1 = c > 0 ? p[r][c - 1] : lft;
-        float x2 = c < COLS - 1 ? p[r][c + 1] : rgt;
-        float y1 = prev[c];
-        float y2 = r < ROWS - 1 ? p[r + 1][c] : above[c];
-        if (special) {
-          const int code = cd[r][c];
-          x1 = (code & 1) ? pc : x1;
-          x2 = (code & 2) ? pc : x2;
-          y1 = (code & 4) ? pc : y1;
-          y2 = (code & 8) ? pc : y2;
-        }
+        const int code = cd[r][c];
+        const float x1 = (code & 1) ? pc : (c > 0 ? p[r][c - 1] : lft);
+        const float x2 = (code & 2) ? pc : (c < COLS - 1 ? p[r][c + 1] : rgt);
+        const float y1 = (code & 4) ? pc : prev[c];
+        const float y2 = (code & 8) ? pc : (r < ROWS - 1 ? p[r + 1][c] : above[c]);
         np[c] = ((((x1 + x2) + y1) + y2) - bb[r][c]) * 0.25f;
-        if (last) {
-          const int trow = trow0 + r, tcol = tcol0 + c;
-          if (tcol >= k && tcol < EX - k && trow >= k && trow < EY - k &&
-              (cd[r][c] & 16) == 0) {
-            const double rr = (double)bb[r][c] -
-                              (((((double)x1 + (double)x2) + (double)y1) +
-                                (double)y2) -
-                               4.0 * (double)pc);
-            sr += rr;
-            srr += rr * rr;
-          }
+        const int trow = trow0 + r, tcol = tcol0 + c;
+        if (tcol >= k && tcol < EX - k && trow >= k && trow < EY - k && (code & 16) == 0) {
+          const double rr =
+              (double)bb[r][c] -
+              (((((double)x1 + (double)x2) + (double)y1) + (double)y2) - 4.0 * (double)pc);
+          sr += rr;
+          srr += rr * rr;
         }
       }
 #pragma unroll

@@ -55,27 +55,53 @@ struct Pub {
 
 enum PassKind { P_SETUP = 0, P_ITER = 1, P_RES = 2 };
 
+// Solid flags of a wave's register rows are packed one bit per row per
+// column: bit i of SLB[c] is set when the cell of register row i in column
+// c is solid or outside the grid.  For a pass, E[c] = (SLB[c] << 2) | ps[c]
+// extends that by the two rows above the wave's strip (bit 0: row -2,
+// bit 1: row -1, from the previous wave or the carry), so bit j of E is row
+// j - 2; after the pass the rows have moved down by one and SLB[c] becomes
+// (E[c] >> 1) masked to ROWS bits.
+template <int ROWS, int COLS>
+__device__ __forceinline__ bool extend_flags(int (&E)[COLS], const int (&SLB)[COLS],
+                                             const int (&ps)[COLS]) {
+  int any = 0;
+#pragma unroll
+  for (int c = 0; c < COLS; ++c) {
+    E[c] = (SLB[c] << 2) | ps[c];
+    any |= E[c];
+  }
+  return any != 0;
+}
+
+template <int ROWS, int COLS>
+__device__ __forceinline__ void shift_flags(int (&SLB)[COLS], const int (&E)[COLS]) {
+#pragma unroll
+  for (int c = 0; c < COLS; ++c) SLB[c] = (E[c] >> 1) & ((1 << ROWS) - 1);
+}
+
 // One Chebyshev pass over a wave's register rows, descending, in place.
 // V is the stencil operand (d for iterations, x for setup and residual),
 // (A, B) the other two arrays ((x, r) for iterations, (r, d) otherwise).
-// p0..p3 / ps: the previous wave's (or the carry's) last rows.
-template <int KIND, int ROWS, int COLS>
+// p0..p3: the previous wave's (or the carry's) last rows; E: the extended
+// solid flags, el / er those of the lanes to the left and right.  SP: with
+// neighbour replacement (some flag is set in the wave); without it, no
+// select is evaluated and the lane shifts fold into the first addition.
+template <int KIND, bool SP, int ROWS, int COLS>
 __device__ __forceinline__ void cheb_pass(float (&X)[ROWS][COLS], float (&R)[ROWS][COLS],
-                                          float (&D)[ROWS][COLS], int (&SL)[ROWS][COLS],
-                                          const float (&p0)[COLS], const float (&p1)[COLS],
-                                          const float (&p2)[COLS], const float (&p3)[COLS],
-                                          const int (&ps)[COLS], float theta, float cdd,
+                                          float (&D)[ROWS][COLS], const int (&E)[COLS], int el,
+                                          int er, const float (&p0)[COLS],
+                                          const float (&p1)[COLS], const float (&p2)[COLS],
+                                          const float (&p3)[COLS], float theta, float cdd,
                                           float crr, int row_new0, int gx0, int tcol0, int k,
                                           int Y0, int Y1, int W, int EX,
-                                          const float (&BR)[ROWS][COLS], double (&acc)[5],
-                                          bool special) {
+                                          const float (&BR)[ROWS][COLS], double (&acc)[5]) {
   float(&V)[ROWS][COLS] = KIND == P_ITER ? D : X;
   float(&Aa)[ROWS][COLS] = KIND == P_ITER ? X : R;
   float(&Bb)[ROWS][COLS] = KIND == P_ITER ? R : D;
 #pragma unroll
   for (int i = ROWS - 1; i >= 0; --i) {
     float vc[COLS], va[COLS], vb[COLS], ac[COLS], bc[COLS];
-    int sc[COLS], sa[COLS], sb[COLS];
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       vc[c] = i >= 1 ? V[i - 1][c] : p1[c];
@@ -83,48 +109,44 @@ __device__ __forceinline__ void cheb_pass(float (&X)[ROWS][COLS], float (&R)[ROW
       vb[c] = V[i][c];
       ac[c] = i >= 1 ? Aa[i - 1][c] : p2[c];
       bc[c] = i >= 1 ? Bb[i - 1][c] : p3[c];
-      sc[c] = i >= 1 ? SL[i - 1][c] : (ps[c] >> 1) & 1;
-      sa[c] = i >= 2 ? SL[i - 2][c] : (i == 1 ? (ps[c] >> 1) & 1 : ps[c] & 1);
-      sb[c] = SL[i][c];
-    }
-    const float lft = lane_from_left(vc[COLS - 1]);
-    const float rgt = lane_from_right(vc[0]);
-    int slf = 0, srg = 0;
-    if (special) {
-      slf = ilane_from_left(sc[COLS - 1]);
-      srg = ilane_from_right(sc[0]);
     }
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       const float v = vc[c];
-      float x1 = c > 0 ? vc[c - 1] : lft;
-      float x2 = c < COLS - 1 ? vc[c + 1] : rgt;
+      // the strip's outermost lanes read 0 across the wave edge: columns
+      // outside the ones the launch keeps
+      float x1 = c > 0 ? vc[c - 1] : jt::lane_shr0(vc[COLS - 1]);
+      float x2 = c < COLS - 1 ? vc[c + 1] : jt::lane_shl0(vc[0]);
       float y1 = va[c];
       float y2 = vb[c];
-      if (special) {
-        const int s1 = c > 0 ? sc[c - 1] : slf;
-        const int s2 = c < COLS - 1 ? sc[c + 1] : srg;
+      // register row j is bit j + 2 of E: the centre (row i - 1) is bit
+      // i + 1, y1 (row i - 2) bit i, y2 (row i) bit i + 2
+      const int sc = SP ? (E[c] >> (i + 1)) & 1 : 0;
+      if (SP) {
+        const int s1 = ((c > 0 ? E[c - 1] : el) >> (i + 1)) & 1;
+        const int s2 = ((c < COLS - 1 ? E[c + 1] : er) >> (i + 1)) & 1;
         x1 = s1 ? v : x1;
         x2 = s2 ? v : x2;
-        y1 = sa[c] ? v : y1;
-        y2 = sb[c] ? v : y2;
+        y1 = ((E[c] >> i) & 1) ? v : y1;
+        y2 = ((E[c] >> (i + 2)) & 1) ? v : y2;
       }
       if (KIND == P_SETUP) {
-        const float lv = (((x1 + x2) + y1) + y2) - 4.0f * v;
+        // 4 v is exact: the fused multiply-add rounds once, like the
+        // subtraction of the product
+        const float lv = __builtin_fmaf(-4.0f, v, ((x1 + x2) + y1) + y2);
         const float rn = lv - ac[c];  // r holds b before the setup
         X[i][c] = v;
         R[i][c] = rn;
         D[i][c] = rn / theta;
       } else if (KIND == P_ITER) {
-        const float lv = (((x1 + x2) + y1) + y2) - 4.0f * v;
+        const float lv = __builtin_fmaf(-4.0f, v, ((x1 + x2) + y1) + y2);
         const float rn = bc[c] + lv;
         X[i][c] = ac[c] + v;
         R[i][c] = rn;
         D[i][c] = cdd * v + crr * rn;
       } else {
         const int gy = row_new0 + i, gx = gx0 + c, tcol = tcol0 + c;
-        if (gy >= Y0 && gy < Y1 && tcol >= k && tcol < EX - k && gx >= 0 && gx < W &&
-            !sc[c]) {
+        if (gy >= Y0 && gy < Y1 && tcol >= k && tcol < EX - k && gx >= 0 && gx < W && !sc) {
           const double bd = (double)BR[i][c];  // b of the output row, loaded with the chunk
           const double rr =
               bd - (((((double)x1 + (double)x2) + (double)y1) + (double)y2) - 4.0 * (double)v);
@@ -138,12 +160,14 @@ __device__ __forceinline__ void cheb_pass(float (&X)[ROWS][COLS], float (&R)[ROW
         R[i][c] = ac[c];
         D[i][c] = bc[c];
       }
-      SL[i][c] = sc[c];
     }
   }
 }
 
-template <int ROWS, int WAVES, int COLS, bool RES>
+// VEC: float2 accesses (COLS == 2, even width and k: every lane's column
+// pair is aligned and wholly inside or outside the grid and the columns
+// the launch keeps).
+template <int ROWS, int WAVES, int COLS, bool RES, bool VEC>
 __global__ void __launch_bounds__(WAVES * 64)
     k_cheb_stream(const float* __restrict__ xin, const float* __restrict__ rin,
                   const float* __restrict__ din, const float* __restrict__ b,
@@ -164,75 +188,60 @@ __global__ void __launch_bounds__(WAVES * 64)
   const int Y1 = min(Y0 + seg, H);
   const int A = Y0 - k;
   const int nch = (Y1 - Y0 + 2 * k + S - 1) / S;
+  static_assert(!VEC || COLS == 2, "float2 accesses take two columns per lane");
   const float a_line = dapp != nullptr ? (float)line[0] : 0.0f;
-  const bool vec = COLS == 2 && (W % 2 == 0) && (k % 2 == 0);
 
   __shared__ Pub<EX> pub[2][WAVES];
   __shared__ Pub<EX> car[STREAM_KMAX];
 
   float X[ROWS][COLS], R[ROWS][COLS], D[ROWS][COLS];
-  int SL[ROWS][COLS];
-  // the next chunk's inputs, loaded while the current chunk's passes run
-  // (its loads stay in flight across the passes' LDS barriers); with RES
-  // also b of the rows the last pass writes (register rows k up)
-  float Xn[ROWS][COLS], Rn[ROWS][COLS], Dn[ROWS][COLS];
-  float BR[ROWS][COLS], BRn[ROWS][COLS];  // used by RES launches only
-  int SLn[ROWS][COLS];
+  int SLB[COLS];  // packed solid flags, bit i: register row i
+  float BR[ROWS][COLS];  // b of the rows the residual pass writes (RES)
+  // The next chunk's raw inputs, loaded from clamped in-grid addresses
+  // while the current chunk's passes run: nothing reads them before the
+  // next chunk starts, so the loads stay in flight across the passes'
+  // barriers.  Cells outside the grid are replaced when the chunk starts.
+  // With RES also b of the rows the last pass writes (register row i holds
+  // grid row - k after the k passes).
+  float Xr[ROWS][COLS], Rr[ROWS][COLS], Dr[ROWS][COLS];
+  float Br[RES ? ROWS : 1][RES ? COLS : 1];
+  uint32_t Mr[ROWS][VEC ? 1 : COLS];  // VEC: both mask bytes of the pair
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const float* rsrc = first ? b : rin;  // the first launch holds b in r
 
   auto load_chunk = [&](int ch) {
     const int row0 = A + ch * S + wave * ROWS;
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
-      const int gy = row0 + i;
-      const bool rin_grid = gy >= 0 && gy < H;
-      // b of the row the residual pass writes from register row i
-      const int gyr = gy - k;
-      const bool res_row = RES && gyr >= 0 && gyr < H;
-      // loads are unconditional, from a clamped in-grid address, and the
-      // values of cells outside the grid are replaced afterwards (a select
-      // on the loaded value keeps every load a plain global load)
-      const float* rsrc = first ? b : rin;
-      const float* dsrc = first ? xin : din;
-      if (vec) {
-        const bool in = rin_grid && gx0 >= 0 && gx0 < W;
-        const int idx = in ? gy * W + gx0 : 0;
+      const int cy = min(max(row0 + i, 0), H - 1);
+      const int cyr = min(max(row0 + i - k, 0), H - 1);
+      if (VEC) {
+        const int idx = cy * W + min(max(gx0, 0), W - 2);
         const float2 xv = *reinterpret_cast<const float2*>(xin + idx);
         const float2 rv = *reinterpret_cast<const float2*>(rsrc + idx);
-        const float2 dv = *reinterpret_cast<const float2*>(dsrc + idx);
-        const uint16_t mm = *reinterpret_cast<const uint16_t*>(mask + idx);
-        const bool dz = !in || first;
-        Xn[i][0] = in ? xv.x : 0.0f;
-        Xn[i][COLS - 1] = in ? xv.y : 0.0f;
-        Rn[i][0] = in ? rv.x : 0.0f;
-        Rn[i][COLS - 1] = in ? rv.y : 0.0f;
-        Dn[i][0] = dz ? 0.0f : dv.x;
-        Dn[i][COLS - 1] = dz ? 0.0f : dv.y;
-        SLn[i][0] = in ? (mm & 0xFF) != 0 : 1;
-        SLn[i][COLS - 1] = in ? (mm >> 8) != 0 : 1;
+        const float2 dv = *reinterpret_cast<const float2*>(din + idx);
+        Mr[i][0] = *reinterpret_cast<const uint16_t*>(mask + idx);
+        Xr[i][0] = xv.x;
+        Xr[i][COLS - 1] = xv.y;
+        Rr[i][0] = rv.x;
+        Rr[i][COLS - 1] = rv.y;
+        Dr[i][0] = dv.x;
+        Dr[i][COLS - 1] = dv.y;
         if (RES) {
-          const bool inr = res_row && gx0 >= 0 && gx0 < W;
-          const float2 bv = *reinterpret_cast<const float2*>(b + (inr ? gyr * W + gx0 : 0));
-          BRn[i][0] = inr ? bv.x : 0.0f;
-          BRn[i][COLS - 1] = inr ? bv.y : 0.0f;
+          const float2 bv =
+              *reinterpret_cast<const float2*>(b + cyr * W + min(max(gx0, 0), W - 2));
+          Br[RES ? i : 0][0] = bv.x;
+          Br[RES ? i : 0][RES ? COLS - 1 : 0] = bv.y;
         }
       } else {
 #pragma unroll
         for (int c = 0; c < COLS; ++c) {
-          const int gx = gx0 + c;
-          const bool in = rin_grid && gx >= 0 && gx < W;
-          const int idx = in ? gy * W + gx : 0;
-          const float xv = xin[idx], rv = rsrc[idx], dv = dsrc[idx];
-          const uint8_t mv = mask[idx];
-          Xn[i][c] = in ? xv : 0.0f;
-          Rn[i][c] = in ? rv : 0.0f;
-          Dn[i][c] = (in && !first) ? dv : 0.0f;
-          SLn[i][c] = in ? (mv != 0) : 1;
-          if (RES) {
-            const bool inr = res_row && gx >= 0 && gx < W;
-            const float bv = b[inr ? gyr * W + gx : 0];
-            BRn[i][c] = inr ? bv : 0.0f;
-          }
+          const int cx = min(max(gx0 + c, 0), W - 1);
+          Xr[i][c] = xin[cy * W + cx];
+          Rr[i][c] = rsrc[cy * W + cx];
+          Dr[i][c] = din[cy * W + cx];
+          Mr[i][VEC ? 0 : c] = mask[cy * W + cx];
+          if (RES) Br[RES ? i : 0][RES ? c : 0] = b[cyr * W + cx];
         }
       }
     }
@@ -241,27 +250,42 @@ __global__ void __launch_bounds__(WAVES * 64)
   load_chunk(0);
   for (int ch = 0; ch < nch; ++ch) {
     const int row0 = A + ch * S + wave * ROWS;  // level-0 row of register row 0
+    // the chunk's registers from its raw inputs
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) SLB[c] = 0;
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
-      const int gy = row0 + i;
-      const bool rin_grid = gy >= 0 && gy < H;
+      const int gy = row0 + i, gyr = gy - k;
 #pragma unroll
       for (int c = 0; c < COLS; ++c) {
-        X[i][c] = Xn[i][c];
-        R[i][c] = Rn[i][c];
-        D[i][c] = Dn[i][c];
-        SL[i][c] = SLn[i][c];
-        BR[i][c] = RES ? BRn[i][c] : 0.0f;
+        const int gx = gx0 + c;
+        const bool col_in = gx >= 0 && gx < W;
+        const bool in = col_in && gy >= 0 && gy < H;
+        const uint32_t m = VEC ? (c == 0 ? Mr[i][0] & 0xFF : Mr[i][0] >> 8) : Mr[i][VEC ? 0 : c];
+        X[i][c] = in ? Xr[i][c] : 0.0f;
+        R[i][c] = in ? Rr[i][c] : 0.0f;
+        D[i][c] = (in && !first) ? Dr[i][c] : 0.0f;
+        SLB[c] |= (in ? m != 0 : 1) << i;
+        BR[i][c] = (RES && col_in && gyr >= 0 && gyr < H) ? Br[RES ? i : 0][RES ? c : 0] : 0.0f;
       }
-      // a pending line-search step x += a dapp on the fluid cells
-      if (a_line != 0.0f) {
+    }
+    // a pending line-search step x += a dapp on the fluid cells (the first
+    // launch of a run); its loads are issued together, then applied
+    if (a_line != 0.0f) {
+      float av[ROWS][COLS];
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i)
+#pragma unroll
+        for (int c = 0; c < COLS; ++c)
+          av[i][c] = dapp[min(max(row0 + i, 0), H - 1) * W + min(max(gx0 + c, 0), W - 1)];
+#pragma unroll
+      for (int i = 0; i < ROWS; ++i)
 #pragma unroll
         for (int c = 0; c < COLS; ++c) {
-          const int gx = gx0 + c;
-          if (rin_grid && gx >= 0 && gx < W && !SL[i][c])
-            X[i][c] = X[i][c] + a_line * dapp[gy * W + gx];
+          const int gy = row0 + i, gx = gx0 + c;
+          if (gy >= 0 && gy < H && gx >= 0 && gx < W && !((SLB[c] >> i) & 1))
+            X[i][c] = X[i][c] + a_line * av[i][c];
         }
-      }
     }
     if (ch + 1 < nch) load_chunk(ch + 1);
 
@@ -293,7 +317,7 @@ __global__ void __launch_bounds__(WAVES * 64)
         const float v1 = it ? D[ROWS - 1][c] : X[ROWS - 1][c];
         const float a1 = it ? X[ROWS - 1][c] : R[ROWS - 1][c];
         const float b1 = it ? R[ROWS - 1][c] : D[ROWS - 1][c];
-        const uint8_t sb = (uint8_t)(SL[ROWS - 2][c] | (SL[ROWS - 1][c] << 1));
+        const uint8_t sb = (uint8_t)((SLB[c] >> (ROWS - 2)) & 3);
         pub[par][wave].v[0][tcol0 + c] = v2;
         pub[par][wave].v[1][tcol0 + c] = v1;
         pub[par][wave].v[2][tcol0 + c] = a1;
@@ -334,15 +358,18 @@ __global__ void __launch_bounds__(WAVES * 64)
       // wave (its own and the two above): no neighbour is replaced.  The
       // strip's outermost lanes read garbage across the wave edge either
       // way, which stays outside the columns the launch keeps.
-      int anys = ps[0] | (COLS > 1 ? ps[COLS - 1] : 0);
-#pragma unroll
-      for (int i = 0; i < ROWS; ++i)
-#pragma unroll
-        for (int c = 0; c < COLS; ++c) anys |= SL[i][c];
-      const bool special = __any(anys != 0);
-      cheb_pass<KIND>(X, R, D, SL, p0, p1, p2, p3, ps, theta, it ? cf.cd[s] : 0.0f,
-                      it ? cf.cr[s] : 0.0f, row0 - (s + 1), gx0, tcol0, k, Y0, Y1, W, EX, BR,
-                      acc, special);
+      int E[COLS];
+      const bool special = __any(extend_flags<ROWS, COLS>(E, SLB, ps));
+      const float cdd = it ? cf.cd[s] : 0.0f, crr = it ? cf.cr[s] : 0.0f;
+      if (special) {
+        const int el = jt::ilane_from_left(E[COLS - 1]), er = jt::ilane_from_right(E[0]);
+        cheb_pass<KIND, true>(X, R, D, E, el, er, p0, p1, p2, p3, theta, cdd, crr,
+                              row0 - (s + 1), gx0, tcol0, k, Y0, Y1, W, EX, BR, acc);
+      } else {
+        cheb_pass<KIND, false>(X, R, D, E, 0, 0, p0, p1, p2, p3, theta, cdd, crr,
+                               row0 - (s + 1), gx0, tcol0, k, Y0, Y1, W, EX, BR, acc);
+      }
+      shift_flags<ROWS, COLS>(SLB, E);
     };
     // the setup pass (first launch), the iterations, the residual pass (RES)
     int s = 0;
@@ -355,7 +382,7 @@ __global__ void __launch_bounds__(WAVES * 64)
     for (int i = 0; i < ROWS; ++i) {
       const int gy = row0 + i - k;
       if (gy < Y0 || gy >= Y1) continue;
-      if (vec) {
+      if (VEC) {
         if (tcol0 < k || tcol0 >= EX - k || gx0 < 0 || gx0 >= W) continue;
         const int idx = gy * W + gx0;
         *reinterpret_cast<float2*>(xout + idx) = make_float2(X[i][0], X[i][COLS - 1]);
@@ -408,14 +435,20 @@ hipError_t launch_cheb_stream_shape(const float* xin, const float* rin, const fl
                                     const float* dapp, const double* line, hipStream_t st) {
   constexpr int EX = 64 * COLS;
   const dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + seg - 1) / seg);
-  if (red)
-    k_cheb_stream<ROWS, WAVES, COLS, true><<<grid, WAVES * 64, 0, st>>>(
-        xin, rin, din, b, mask, xout, rout, dout, W, H, k, seg, first, theta, cf, red, dapp,
-        line);
+  const bool vec = COLS == 2 && W % 2 == 0 && k % 2 == 0;
+#define NX_CHEB_STREAM_GO(RESV, VECV)                                                   \
+  k_cheb_stream<ROWS, WAVES, COLS, RESV, VECV><<<grid, WAVES * 64, 0, st>>>(            \
+      xin, rin, din, b, mask, xout, rout, dout, W, H, k, seg, first, theta, cf, red, dapp, \
+      line)
+  if (red && vec)
+    NX_CHEB_STREAM_GO(true, COLS == 2);
+  else if (red)
+    NX_CHEB_STREAM_GO(true, false);
+  else if (vec)
+    NX_CHEB_STREAM_GO(false, COLS == 2);
   else
-    k_cheb_stream<ROWS, WAVES, COLS, false><<<grid, WAVES * 64, 0, st>>>(
-        xin, rin, din, b, mask, xout, rout, dout, W, H, k, seg, first, theta, cf, nullptr,
-        dapp, line);
+    NX_CHEB_STREAM_GO(false, false);
+#undef NX_CHEB_STREAM_GO
   return hipGetLastError();
 }
 
@@ -423,55 +456,45 @@ hipError_t launch_cheb_stream_shape(const float* xin, const float* rin, const fl
 // p' = ((((x1 + x2) + y1) + y2) - b) * 0.25 (shader.Poisson.comp:37) with the
 // centre row one register row up; b moves with p.  With RESID, the FP64
 // residual b - ((((x1 + x2) + y1) + y2) - 4 p) of the sweep's input iterate
-// is accumulated over the fluid cells of the written interior.
-template <bool RESID, int ROWS, int COLS>
+// is accumulated over the fluid cells of the written interior.  E, el, er
+// and SP as for cheb_pass.
+template <bool RESID, bool SP, int ROWS, int COLS>
 __device__ __forceinline__ void jacobi_pass(float (&P)[ROWS][COLS], float (&B)[ROWS][COLS],
-                                            int (&SL)[ROWS][COLS], const float (&p0)[COLS],
-                                            const float (&p1)[COLS], const float (&p2)[COLS],
-                                            const int (&ps)[COLS], int row_new0, int gx0,
+                                            const int (&E)[COLS], int el, int er,
+                                            const float (&p0)[COLS], const float (&p1)[COLS],
+                                            const float (&p2)[COLS], int row_new0, int gx0,
                                             int tcol0, int k, int Y0, int Y1, int W, int EX,
-                                            double& sr, double& srr, bool special) {
+                                            double& sr, double& srr) {
 #pragma unroll
   for (int i = ROWS - 1; i >= 0; --i) {
     float vc[COLS], va[COLS], vb[COLS], bc[COLS];
-    int sc[COLS], sa[COLS], sb[COLS];
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       vc[c] = i >= 1 ? P[i - 1][c] : p1[c];
       va[c] = i >= 2 ? P[i - 2][c] : (i == 1 ? p1[c] : p0[c]);
       vb[c] = P[i][c];
       bc[c] = i >= 1 ? B[i - 1][c] : p2[c];
-      sc[c] = i >= 1 ? SL[i - 1][c] : (ps[c] >> 1) & 1;
-      sa[c] = i >= 2 ? SL[i - 2][c] : (i == 1 ? (ps[c] >> 1) & 1 : ps[c] & 1);
-      sb[c] = SL[i][c];
-    }
-    const float lft = lane_from_left(vc[COLS - 1]);
-    const float rgt = lane_from_right(vc[0]);
-    int slf = 0, srg = 0;
-    if (special) {
-      slf = ilane_from_left(sc[COLS - 1]);
-      srg = ilane_from_right(sc[0]);
     }
 #pragma unroll
     for (int c = 0; c < COLS; ++c) {
       const float v = vc[c];
-      float x1 = c > 0 ? vc[c - 1] : lft;
-      float x2 = c < COLS - 1 ? vc[c + 1] : rgt;
+      float x1 = c > 0 ? vc[c - 1] : jt::lane_shr0(vc[COLS - 1]);
+      float x2 = c < COLS - 1 ? vc[c + 1] : jt::lane_shl0(vc[0]);
       float y1 = va[c];
       float y2 = vb[c];
-      if (special) {
-        const int s1 = c > 0 ? sc[c - 1] : slf;
-        const int s2 = c < COLS - 1 ? sc[c + 1] : srg;
+      const int sc = SP ? (E[c] >> (i + 1)) & 1 : 0;
+      if (SP) {
+        const int s1 = ((c > 0 ? E[c - 1] : el) >> (i + 1)) & 1;
+        const int s2 = ((c < COLS - 1 ? E[c + 1] : er) >> (i + 1)) & 1;
         x1 = s1 ? v : x1;
         x2 = s2 ? v : x2;
-        y1 = sa[c] ? v : y1;
-        y2 = sb[c] ? v : y2;
+        y1 = ((E[c] >> i) & 1) ? v : y1;
+        y2 = ((E[c] >> (i + 2)) & 1) ? v : y2;
       }
       P[i][c] = ((((x1 + x2) + y1) + y2) - bc[c]) * 0.25f;
       if (RESID) {
         const int gy = row_new0 + i, gx = gx0 + c, tcol = tcol0 + c;
-        if (gy >= Y0 && gy < Y1 && tcol >= k && tcol < EX - k && gx >= 0 && gx < W &&
-            !sc[c]) {
+        if (gy >= Y0 && gy < Y1 && tcol >= k && tcol < EX - k && gx >= 0 && gx < W && !sc) {
           const double rr = (double)bc[c] - (((((double)x1 + (double)x2) + (double)y1) +
                                               (double)y2) -
                                              4.0 * (double)v);
@@ -480,7 +503,6 @@ __device__ __forceinline__ void jacobi_pass(float (&P)[ROWS][COLS], float (&B)[R
         }
       }
       B[i][c] = bc[c];
-      SL[i][c] = sc[c];
     }
   }
 }
@@ -489,7 +511,7 @@ __device__ __forceinline__ void jacobi_pass(float (&P)[ROWS][COLS], float (&B)[R
 // also writes the FP64 residual sums of its input iterate,
 // red[2 blk + {sum r, sum r^2}] over the fluid cells the launch writes.
 // done (nullable): a device flag; when non-zero the launch returns at once.
-template <int ROWS, int WAVES, int COLS>
+template <int ROWS, int WAVES, int COLS, bool VEC>
 __global__ void __launch_bounds__(WAVES * 64)
     k_jacobi_stream(const float* __restrict__ pin, const float* __restrict__ b,
                     const uint8_t* __restrict__ mask, float* __restrict__ pout, int W, int H,
@@ -508,48 +530,40 @@ __global__ void __launch_bounds__(WAVES * 64)
   const int Y1 = min(Y0 + seg, H);
   const int A = Y0 - k;
   const int nch = (Y1 - Y0 + 2 * k + S - 1) / S;
-  const bool vec = COLS == 2 && (W % 2 == 0) && (k % 2 == 0);
+  static_assert(!VEC || COLS == 2, "float2 accesses take two columns per lane");
 
   __shared__ Pub<EX, 3> pub[2][WAVES];
   __shared__ Pub<EX, 3> car[STREAM_KMAX];
 
   float P[ROWS][COLS], B[ROWS][COLS];
-  int SL[ROWS][COLS];
-  // the next chunk's inputs, loaded while the current chunk's sweeps run
-  float Pn[ROWS][COLS], Bn[ROWS][COLS];
-  int SLn[ROWS][COLS];
+  int SLB[COLS];  // packed solid flags, bit i: register row i
+  // the next chunk's raw inputs (clamped in-grid addresses), loaded while
+  // the current chunk's sweeps run and taken when the next chunk starts
+  float Pr[ROWS][COLS], Br[ROWS][COLS];
+  uint32_t Mr[ROWS][VEC ? 1 : COLS];  // VEC: both mask bytes of the pair
   double sr = 0.0, srr = 0.0;
 
   auto load_chunk = [&](int ch) {
     const int row0 = A + ch * S + wave * ROWS;
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
-      const int gy = row0 + i;
-      const bool row_in = gy >= 0 && gy < H;
-      // unconditional loads from a clamped in-grid address, then selects
-      if (vec) {
-        const bool in = row_in && gx0 >= 0 && gx0 < W;
-        const int idx = in ? gy * W + gx0 : 0;
+      const int cy = min(max(row0 + i, 0), H - 1);
+      if (VEC) {
+        const int idx = cy * W + min(max(gx0, 0), W - 2);
         const float2 pv = *reinterpret_cast<const float2*>(pin + idx);
         const float2 bv = *reinterpret_cast<const float2*>(b + idx);
-        const uint16_t mm = *reinterpret_cast<const uint16_t*>(mask + idx);
-        Pn[i][0] = in ? pv.x : 0.0f;
-        Pn[i][COLS - 1] = in ? pv.y : 0.0f;
-        Bn[i][0] = in ? bv.x : 0.0f;
-        Bn[i][COLS - 1] = in ? bv.y : 0.0f;
-        SLn[i][0] = in ? (mm & 0xFF) != 0 : 1;
-        SLn[i][COLS - 1] = in ? (mm >> 8) != 0 : 1;
+        Mr[i][0] = *reinterpret_cast<const uint16_t*>(mask + idx);
+        Pr[i][0] = pv.x;
+        Pr[i][COLS - 1] = pv.y;
+        Br[i][0] = bv.x;
+        Br[i][COLS - 1] = bv.y;
       } else {
 #pragma unroll
         for (int c = 0; c < COLS; ++c) {
-          const int gx = gx0 + c;
-          const bool in = row_in && gx >= 0 && gx < W;
-          const int idx = in ? gy * W + gx : 0;
-          const float pv = pin[idx], bv = b[idx];
-          const uint8_t mv = mask[idx];
-          Pn[i][c] = in ? pv : 0.0f;
-          Bn[i][c] = in ? bv : 0.0f;
-          SLn[i][c] = in ? (mv != 0) : 1;
+          const int idx = cy * W + min(max(gx0 + c, 0), W - 1);
+          Pr[i][c] = pin[idx];
+          Br[i][c] = b[idx];
+          Mr[i][VEC ? 0 : c] = mask[idx];
         }
       }
     }
@@ -559,13 +573,20 @@ __global__ void __launch_bounds__(WAVES * 64)
   for (int ch = 0; ch < nch; ++ch) {
     const int row0 = A + ch * S + wave * ROWS;
 #pragma unroll
-    for (int i = 0; i < ROWS; ++i)
+    for (int c = 0; c < COLS; ++c) SLB[c] = 0;
+#pragma unroll
+    for (int i = 0; i < ROWS; ++i) {
+      const int gy = row0 + i;
 #pragma unroll
       for (int c = 0; c < COLS; ++c) {
-        P[i][c] = Pn[i][c];
-        B[i][c] = Bn[i][c];
-        SL[i][c] = SLn[i][c];
+        const int gx = gx0 + c;
+        const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+        const uint32_t m = VEC ? (c == 0 ? Mr[i][0] & 0xFF : Mr[i][0] >> 8) : Mr[i][VEC ? 0 : c];
+        P[i][c] = in ? Pr[i][c] : 0.0f;
+        B[i][c] = in ? Br[i][c] : 0.0f;
+        SLB[c] |= (in ? m != 0 : 1) << i;
       }
+    }
     if (ch + 1 < nch) load_chunk(ch + 1);
 
     for (int s = 0; s < k; ++s) {
@@ -584,7 +605,7 @@ __global__ void __launch_bounds__(WAVES * 64)
       }
 #pragma unroll
       for (int c = 0; c < COLS; ++c) {
-        const uint8_t sb = (uint8_t)(SL[ROWS - 2][c] | (SL[ROWS - 1][c] << 1));
+        const uint8_t sb = (uint8_t)((SLB[c] >> (ROWS - 2)) & 3);
         pub[par][wave].v[0][tcol0 + c] = P[ROWS - 2][c];
         pub[par][wave].v[1][tcol0 + c] = P[ROWS - 1][c];
         pub[par][wave].v[2][tcol0 + c] = B[ROWS - 1][c];
@@ -616,25 +637,28 @@ __global__ void __launch_bounds__(WAVES * 64)
         }
       }
       const int row_new0 = row0 - (s + 1);
-      int anys = ps[0] | (COLS > 1 ? ps[COLS - 1] : 0);
-#pragma unroll
-      for (int i = 0; i < ROWS; ++i)
-#pragma unroll
-        for (int c = 0; c < COLS; ++c) anys |= SL[i][c];
-      const bool special = __any(anys != 0);
-      if (red != nullptr && s == k - 1)
-        jacobi_pass<true>(P, B, SL, p0, p1, p2, ps, row_new0, gx0, tcol0, k, Y0, Y1, W, EX,
-                          sr, srr, special);
-      else
-        jacobi_pass<false>(P, B, SL, p0, p1, p2, ps, row_new0, gx0, tcol0, k, Y0, Y1, W, EX,
-                           sr, srr, special);
+      int E[COLS];
+      const bool special = __any(extend_flags<ROWS, COLS>(E, SLB, ps));
+      if (red != nullptr && s == k - 1) {
+        const int el = jt::ilane_from_left(E[COLS - 1]), er = jt::ilane_from_right(E[0]);
+        jacobi_pass<true, true>(P, B, E, el, er, p0, p1, p2, row_new0, gx0, tcol0, k, Y0, Y1,
+                                W, EX, sr, srr);
+      } else if (special) {
+        const int el = jt::ilane_from_left(E[COLS - 1]), er = jt::ilane_from_right(E[0]);
+        jacobi_pass<false, true>(P, B, E, el, er, p0, p1, p2, row_new0, gx0, tcol0, k, Y0, Y1,
+                                 W, EX, sr, srr);
+      } else {
+        jacobi_pass<false, false>(P, B, E, 0, 0, p0, p1, p2, row_new0, gx0, tcol0, k, Y0, Y1,
+                                  W, EX, sr, srr);
+      }
+      shift_flags<ROWS, COLS>(SLB, E);
     }
 
 #pragma unroll
     for (int i = 0; i < ROWS; ++i) {
       const int gy = row0 + i - k;
       if (gy < Y0 || gy >= Y1) continue;
-      if (vec) {
+      if (VEC) {
         if (tcol0 < k || tcol0 >= EX - k || gx0 < 0 || gx0 >= W) continue;
         *reinterpret_cast<float2*>(pout + gy * W + gx0) = make_float2(P[i][0], P[i][COLS - 1]);
         continue;
@@ -677,8 +701,12 @@ hipError_t launch_jacobi_stream_shape(const float* pin, const float* b, const ui
                                       const int* done, hipStream_t st) {
   constexpr int EX = 64 * COLS;
   const dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + seg - 1) / seg);
-  k_jacobi_stream<ROWS, WAVES, COLS><<<grid, WAVES * 64, 0, st>>>(pin, b, mask, pout, W, H, k,
-                                                                   seg, red, done);
+  if (COLS == 2 && W % 2 == 0 && k % 2 == 0)
+    k_jacobi_stream<ROWS, WAVES, COLS, COLS == 2><<<grid, WAVES * 64, 0, st>>>(
+        pin, b, mask, pout, W, H, k, seg, red, done);
+  else
+    k_jacobi_stream<ROWS, WAVES, COLS, false><<<grid, WAVES * 64, 0, st>>>(
+        pin, b, mask, pout, W, H, k, seg, red, done);
   return hipGetLastError();
 }
 

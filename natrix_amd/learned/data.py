@@ -10,12 +10,14 @@ right-hand side b, a solid mask and the pressure p the solver converged to.
 The right-hand sides are smooth random fields (sums of Gaussian blobs of
 random width and sign plus low Fourier modes), like the restricted
 divergence and residual fields the solver hands the network; masks are
-disks (the cylinder scenes), rectangles, or empty.  b is centred on its
-fluid mean (the operator's null space) before solving.
+disks (the cylinder scenes), rectangles, or empty.  b is centred on the
+fluid mean of every connected fluid region (the operator's null space)
+before solving, so every problem has a solution.
 """
 from __future__ import annotations
 
 import numpy as np
+from scipy import ndimage
 
 from ..batch import BatchedScenes, random_rect_obstacles
 
@@ -44,7 +46,16 @@ def random_problem(rng, n):
                 np.cos(np.pi * ky * (Y + 0.5) / n + ph[1])
     b[solid] = 0.0
     fluid = ~solid
-    b[fluid] -= b[fluid].mean()
+    # centred on every connected fluid region (4-connected, the stencil's
+    # neighbourhood): obstacles can cut the domain in pieces, and the
+    # operator's null space then holds one constant per piece, so a right-hand
+    # side centred only on the global mean has no solution
+    lab, count = ndimage.label(fluid)
+    if count > 0:
+        sums = ndimage.sum_labels(b, lab, np.arange(1, count + 1))
+        cells = ndimage.sum_labels(fluid, lab, np.arange(1, count + 1))
+        means = np.concatenate([[0.0], sums / cells])
+        b = b - means[lab]
     b /= max(np.sqrt((b[fluid] ** 2).mean()), 1e-12)
     return b.astype(np.float32), solid
 

@@ -133,13 +133,19 @@ def main(argv=None):
     t0 = time.time()
     b, solid, p, res = make_dataset(a.samples, a.size, tol=a.tol, max_iter=a.max_iter,
                                     log=log)
+    # a target the solver did not bring to the tolerance is not a solution:
+    # such samples are left out (and counted)
+    keep = res <= 10.0 * a.tol
+    dropped = int((~keep).sum())
+    if dropped:
+        b, solid, p = b[keep], solid[keep], p[keep]
     edges = [0.0] + [10.0 ** e for e in range(-12, 0)] + [float("inf")]
     hist, _ = np.histogram(res, bins=edges)
     residual_hist = {f"<={edges[i + 1]:g}": int(c) for i, c in enumerate(hist) if c}
     log(json.dumps({"dataset": a.samples, "size": a.size, "seconds": round(time.time() - t0, 1),
                     "residual_median": float(np.median(res)), "residual_max": float(res.max()),
                     "frac_at_tol": float((res <= a.tol).mean()),
-                    "residual_histogram": residual_hist}))
+                    "dropped_unconverged": dropped, "residual_histogram": residual_hist}))
     net, hist = train(b, solid, p, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log,
                       time_budget=a.time_budget)
     meta = {"samples": a.samples, "size": a.size, "epochs": a.epochs,
@@ -147,6 +153,7 @@ def main(argv=None):
                     "the FP64 solution",
             "data_tol": a.tol, "data_residual_median": float(np.median(res)),
             "data_residual_max": float(res.max()), "data_residual_histogram": residual_hist,
+            "data_dropped_unconverged": dropped,
             "history": hist, "params": net.n_params()}
     save(net, a.out, meta)
     log(json.dumps({"saved": a.out, "params": net.n_params()}))
