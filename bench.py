@@ -244,31 +244,10 @@ def config_256(steps=200):
             "cpu_steps_per_s": round(steps / t_cpu, 2), "gpu_steps_per_s": round(steps / t_gpu, 1)}
 
 
-BATCH_SIDES = (128, 256, 384, 512, 768, 1024)
-
-
 def make_batch(n_scenes=64, iters=60):
-    """64 scenes (seeds 3000..3063): side from {128..1024}, 1-5 Gaussian
-    splats (radius 0.02*N, |F| ~ U(50, 200) times dt, dye 1.0) applied every
-    step, rectangle obstacles covering U(0, 0.15) of the cells."""
-    from natrix_amd.batch import BatchedScenes, random_rect_obstacles
-    sides, solids, splats = [], [], []
-    for i in range(n_scenes):
-        rng = np.random.default_rng(3000 + i)
-        n = int(rng.choice(BATCH_SIDES))
-        sides.append(n)
-        solids.append(random_rect_obstacles(rng, n, n, rng.uniform(0, 0.15)))
-        for _ in range(int(rng.integers(1, 6))):
-            f = rng.uniform(50, 200) / 60.0
-            ang = rng.uniform(0, 2 * np.pi)
-            splats.append((i, rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9),
-                           f * np.cos(ang), f * np.sin(ang), 0.02 * n, 1.0))
-    b = BatchedScenes(sides, dye=True)
-    b.set_params(iterations=iters, vorticity=0.3, viscosity=0.0, bc_mode="free_slip")
-    for i, s in enumerate(solids):
-        b.set_obstacle_map(i, s)
-    b.set_step_splats(splats)
-    return b, sides, solids, splats
+    """The batched configuration's scenes (natrix_amd.batch.benchmark_scenes)."""
+    from natrix_amd.batch import benchmark_scenes
+    return benchmark_scenes(n_scenes, iters)
 
 
 def bench_batch(args, copy_bandwidth):

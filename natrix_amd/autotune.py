@@ -11,6 +11,11 @@ best tile depends on the mix of scene sizes, not on the total cell count,
 and is cached under ``batch_key(sizes)`` for ``BatchedScenes``.
 
     python -m natrix_amd.autotune 2048 1024 512   # tunes and updates tuned.json
+    python -m natrix_amd.autotune --batch 64      # the 64-scene benchmark batch
+
+The fine Chebyshev smoother of the learned cycle is tuned per grid as well
+(``tune_smoother``: tile and streaming launch shapes, timed on the
+smoothing call the cycle makes) and cached under "cheb".
 """
 from __future__ import annotations
 
@@ -193,11 +198,111 @@ def store_step_timed(W, H, tile, ms_per_step, dtype):
     os.replace(tmp, CACHE)
 
 
+# the fine Chebyshev smoother's launch shapes: tile kernels (seg = 0;
+# NX_CHEB_SHAPES in csrc/nx_cheb.hip) and streaming kernels (seg > 0;
+# NX_STREAM_SHAPES in csrc/nx_stream.hip)
+CHEB_SHAPES = [(2, 16, 2), (4, 8, 2), (4, 16, 2), (8, 8, 1), (8, 16, 1)]
+CHEB_KS = (4, 6, 8, 10, 12, 16)
+STREAM_SHAPES = [(4, 8, 2), (2, 16, 2), (4, 16, 2), (8, 8, 2)]
+STREAM_KS = (8, 12, 16)
+STREAM_SEGS = (48, 80, 112, 144, 208)
+
+
+def smoother_candidates():
+    out = []
+    for (r, w, c) in CHEB_SHAPES:
+        for k in CHEB_KS:
+            if 64 * c - 2 * k >= 8 and r * w - 2 * k >= 8:
+                out.append((k, r, w, c, 0))
+    for (r, w, c) in STREAM_SHAPES:
+        for k in STREAM_KS:
+            for seg in STREAM_SEGS:
+                out.append((k, r, w, c, seg))
+    return out
+
+
+def tune_smoother(W, H, iterations=72, lambda_min=0.006, calls=10, log=print):
+    """Time the fine Chebyshev smoother (nx_smooth_chebyshev, `iterations`
+    per call, as the learned cycle runs it) with every launch shape on a
+    W x H scene inside a step; returns (best shape, results)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from . import _lib
+    from .core.fluid_simulator import FluidSimulator
+    sim = FluidSimulator(W, H, bc_mode="inflow")
+    X = np.arange(W)[None, :]
+    Y = np.arange(H)[:, None]
+    sim.set_obstacle_map((X - 0.4 * W) ** 2 + (Y - 0.42 * H) ** 2 <= (0.1 * min(W, H)) ** 2)
+    for _ in range(3):
+        sim.update(1 / 60)
+    results = []
+    sim._push_params()
+    _lib.call("nx_step_begin", sim._alive(), 1 / 60)
+    try:
+        for cfg in smoother_candidates():
+            try:
+                sim.smoother_tile = cfg
+            except ValueError:
+                continue
+            _lib.call("nx_smooth_chebyshev", sim._alive(), iterations, lambda_min, None, None)
+            sim.set_timing(True)
+            sim.collect_timing()
+            for _ in range(calls):
+                _lib.call("nx_smooth_chebyshev", sim._alive(), iterations, lambda_min, None,
+                          None)
+            ms, launches = sim.collect_timing()["chebyshev"]
+            sim.set_timing(False)
+            us = 1e3 * ms / (calls * iterations)
+            results.append((us, cfg))
+            log(f"  {W}x{H} smoother {cfg}: {us:8.3f} us/iteration")
+    finally:
+        _lib.call("nx_step_finish", sim._alive())
+        sim.destroy()
+    results.sort()
+    return results[0][1], results
+
+
+def store_smoother(W, H, tile, us_per_iteration):
+    data = _load()
+    data.setdefault(ARCH, {}).setdefault("cheb", {})[f"{W}x{H}"] = {
+        "tile": list(tile), "us_per_iteration": round(us_per_iteration, 4)}
+    tmp = CACHE + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE)
+
+
 def main(argv):
-    sizes = [int(a) for a in argv] or [2048]
-    for n in sizes:
-        best, res = tune(n, n, log=lambda m: print(m, flush=True))
+    """python -m natrix_amd.autotune [--batch N] [--no-smoother] SIZE...:
+    the Jacobi tile and the Chebyshev smoother's launch shape of every
+    SIZE x SIZE grid, and with --batch N the Jacobi tile of the benchmark's
+    batch of N scenes."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="natrix_amd.autotune")
+    ap.add_argument("sizes", type=int, nargs="*")
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--no-smoother", action="store_true")
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    for n in a.sizes or ([] if a.batch else [2048]):
+        best, res = tune(n, n, log=log)
         store(n, n, best, res[0][0])
+        log(f"{n}x{n} jacobi: {best} {res[0][0]:.4f} us/sweep")
+        if not a.no_smoother:
+            best, res = tune_smoother(n, n, log=log)
+            store_smoother(n, n, best, res[0][0])
+            log(f"{n}x{n} smoother: {best} {res[0][0]:.4f} us/iteration")
+    if a.batch:
+        from .batch import benchmark_scenes
+        b, sides, _, _ = benchmark_scenes(a.batch)
+        try:
+            best, res = tune_batch(b, log=log)
+            store_batch(list(zip(sides, sides)), best, res[0][0])
+            log(f"batch of {a.batch}: {best} {res[0][0]:.4f} us/sweep")
+        finally:
+            b.destroy()
         print(f"best {n}x{n}: tile {best} {res[0][0]:.3f} us/sweep", flush=True)
 
 
