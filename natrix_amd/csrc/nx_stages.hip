@@ -5,8 +5,6 @@
 // nx_jacobi.hip.  All are memory-bound pointwise or 5-point-stencil kernels:
 // a 64x4 block puts one wavefront on 64 consecutive x of one row, so every
 // float2 access of a wavefront is one contiguous 512-byte segment.
-#include <type_traits>
-
 #include "nx_kernels.h"
 
 namespace nx {
@@ -39,80 +37,13 @@ __global__ void __launch_bounds__(256) k_bc(float2* u, int W, int H,
 }
 
 // ---------------------------------------------------------------- S2
-// ADV_ROWS cells per thread, a column of consecutive rows: the accesses of
-// all its cells are issued together at each of the two dependent stages
-// (centre velocity and mask, then the four bilinear corners), so a wave has
-// ADV_ROWS times the loads in flight of a one-cell-per-thread form.  The
-// wall treatment only changes cells of the grid's outer ring; a stage whose
-// wave reads no ring cell loads the velocity directly (one wave-uniform
-// test), the others evaluate it on every read.  Each cell evaluates
-// advect_cell's expressions in its order (a solid cell's result is replaced
-// by zero at the end instead of returning early).
-constexpr int ADV_ROWS = 4;
-
 __global__ void __launch_bounds__(256)
     k_advect(const float2* __restrict__ uin, float2* __restrict__ uout,
              const uint8_t* __restrict__ mask, int W, int H, float dt,
              float speed, float diss, BcParams bc) {
-  const int x = blockIdx.x * BX + threadIdx.x;
-  const int y0 = (blockIdx.y * BY + threadIdx.y) * ADV_ROWS;
-  const bool live = x < W && y0 < H;
-  // lanes past the grid work on a clamped cell and store nothing
-  const int xc = min(x, W - 1), yc0 = min(y0, H - 1);
-  auto on_ring = [&](int cx, int cy) { return cx == 0 || cy == 0 || cx == W - 1 || cy == H - 1; };
-  auto vel_at = [&](auto ringc, int cx, int cy) {
-    return decltype(ringc)::value ? load_bc(uin, cx, cy, W, H, bc) : uin[cy * W + cx];
-  };
-  uint8_t m[ADV_ROWS];
-  float2 vel[ADV_ROWS];
-  bool ring = false;
-#pragma unroll
-  for (int r = 0; r < ADV_ROWS; ++r) ring |= on_ring(xc, min(yc0 + r, H - 1));
-  auto stage1 = [&](auto ringc) {
-#pragma unroll
-    for (int r = 0; r < ADV_ROWS; ++r) {
-      const int y = min(yc0 + r, H - 1);
-      m[r] = mask[y * W + xc];
-      vel[r] = vel_at(ringc, xc, y);
-    }
-  };
-  if (__any(ring))
-    stage1(std::true_type());
-  else
-    stage1(std::false_type());
-  Bilin c[ADV_ROWS];
-  ring = false;
-#pragma unroll
-  for (int r = 0; r < ADV_ROWS; ++r) {
-    const int y = min(yc0 + r, H - 1);
-    const float fx = (float)xc - (vel[r].x * dt) * speed;
-    const float fy = (float)y - (vel[r].y * dt) * speed;
-    c[r] = bilinear_corners(fx, fy, W, H);
-    ring |= c[r].bx == 0 || c[r].by == 0 || c[r].tx == W - 1 || c[r].ty == H - 1;
-  }
-  float2 lt[ADV_ROWS], rt[ADV_ROWS], lb[ADV_ROWS], rb[ADV_ROWS];
-  auto stage2 = [&](auto ringc) {
-#pragma unroll
-    for (int r = 0; r < ADV_ROWS; ++r) {
-      lt[r] = vel_at(ringc, c[r].bx, c[r].ty);
-      rt[r] = vel_at(ringc, c[r].tx, c[r].ty);
-      lb[r] = vel_at(ringc, c[r].bx, c[r].by);
-      rb[r] = vel_at(ringc, c[r].tx, c[r].by);
-    }
-  };
-  if (__any(ring))
-    stage2(std::true_type());
-  else
-    stage2(std::false_type());
-  if (!live) return;
-#pragma unroll
-  for (int r = 0; r < ADV_ROWS; ++r) {
-    if (y0 + r >= H) break;
-    const float2 v = bilinear_mix<float2>(lt[r], rt[r], lb[r], rb[r], c[r].dx, c[r].dy);
-    uout[(y0 + r) * W + x] = m[r] != 0 ? make_float2(0.0f, 0.0f)
-                                       : make_float2(clampf(v.x * diss, -1.0f, 1.0f),
-                                                     clampf(v.y * diss, -1.0f, 1.0f));
-  }
+  int x, y;
+  if (!cell(x, y, W, H)) return;
+  uout[y * W + x] = advect_cell(uin, mask, x, y, W, H, dt, speed, diss, bc);
 }
 
 // ---------------------------------------------------------------- S3
@@ -365,9 +296,8 @@ hipError_t launch_bc(float2* u, int W, int H, BcParams bc, hipStream_t s) {
 hipError_t launch_advect(const float2* uin, float2* uout, const uint8_t* mask,
                          int W, int H, float dt, float speed, float diss,
                          BcParams bc, hipStream_t s) {
-  const dim3 grid((W + BX - 1) / BX, (H + BY * ADV_ROWS - 1) / (BY * ADV_ROWS), 1);
-  NX_LAUNCH(k_advect<<<grid, dim3(BX, BY), 0, s>>>(uin, uout, mask, W, H, dt, speed, diss,
-                                                     bc));
+  NX_LAUNCH(k_advect<<<grid2d(W, H), dim3(BX, BY), 0, s>>>(
+      uin, uout, mask, W, H, dt, speed, diss, bc));
 }
 hipError_t launch_curl(const float2* u, float* w, int W, int H,
                        hipStream_t s) {
