@@ -47,25 +47,38 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def committed_traffic(N, iters, tile, kernel="k_jacobi_tb", suffix="fixed"):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    of this grid, iteration count and tile (FETCH_SIZE x 2.0 + WRITE_SIZE x
-    1.0, the copy-probe calibration), or None when no summary matches."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{N}_{suffix}{iters}.md")
+def profile_traffic(name, **match):
+    """HBM bytes per launch from the committed rocprofv3 summary
+    profiles/<name>.md (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the copy-probe
+    calibration), when every header line `key: value` named in `match`
+    equals it (tile as a comma list), else (None, None)."""
+    path = os.path.join(ROOT, "profiles", f"{name}.md")
     if not os.path.exists(path):
         return None, None
     kv = {}
     for line in open(path):
         k, sep, v = line.partition(":")
-        if sep and k.strip() in ("traffic_bytes_per_launch", "grid", "iterations", "tile"):
-            kv[k.strip()] = v.strip()
+        if sep and (k.strip() in match or k.strip() == "traffic_bytes_per_launch"):
+            kv.setdefault(k.strip(), v.strip())
     try:
-        ok = (int(kv["grid"]) == N and int(kv["iterations"]) == iters and
-              [int(t) for t in kv["tile"].split(",")] == list(tile))
-        return (int(kv["traffic_bytes_per_launch"]) if ok else None,
-                os.path.relpath(path, ROOT) if ok else None)
+        for k, want in match.items():
+            got = kv[k]
+            if isinstance(want, (list, tuple)):
+                ok = [int(t) for t in got.split(",")] == [int(t) for t in want]
+            else:
+                ok = type(want)(got) == want
+            if not ok:
+                return None, None
+        return int(kv["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
     except (KeyError, ValueError):
         return None, None
+
+
+def committed_traffic(N, iters, tile, kernel="k_jacobi_tb", suffix="fixed"):
+    """HBM bytes per launch of `kernel` on an N x N grid at this iteration
+    count and tile, from profiles/<kernel>_<N>_<suffix><iters>.md."""
+    return profile_traffic(f"{kernel}_{N}_{suffix}{iters}", grid=N, iterations=iters,
+                           tile=list(tile))
 
 
 def roofline_entry(kernel, algo_bytes, launch_us, traffic, traffic_src, **extra):
@@ -275,8 +288,11 @@ def bench_batch(args, copy_bandwidth):
     roof = None
     if jac_n and jac_ms > 0:
         # compulsory bytes per launch: every scene's p, b, mask read, p written
+        traffic, tsrc = profile_traffic(f"kb_jacobi_batch{len(sides)}", scenes=len(sides),
+                                        cells=cells, iterations=iters,
+                                        tile=list(b.jacobi_tile))
         roof = roofline_entry(
-            "kb_jacobi", JACOBI_BYTES_PER_CELL_SWEEP * cells, 1e3 * jac_ms / jac_n, None, None,
+            "kb_jacobi", JACOBI_BYTES_PER_CELL_SWEEP * cells, 1e3 * jac_ms / jac_n, traffic, tsrc,
             sweeps_per_launch=round(tb_sweeps / jac_n, 3),
             per_sweep_GBps=round(JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps
                                  / (jac_ms * 1e-3) / 1e9, 1))
@@ -349,7 +365,8 @@ def bench_mixed(args):
     N = 4096 if args.size == 2048 else args.size
     iters = 120 if args.iters == 80 else args.iters
     sim, splats = mixed_scene(N, args.storage, iters)
-    mixed_k = sim.jacobi_tile[0]
+    mixed_tile = tuple(sim.jacobi_tile)
+    mixed_k = mixed_tile[0]
     log(f"bench mixed: {N}^2 {args.storage} I={iters} tile {sim.jacobi_tile}")
     if args.graph:
         sim.use_graph = True
@@ -387,8 +404,10 @@ def bench_mixed(args):
         k = mixed_k
         per_step_launches = -(-(iters - 1) // k) + 1
         algo = cells * (7 * (per_step_launches - 1) + 9) / per_step_launches
+        traffic, tsrc = profile_traffic(f"km_jacobi_{args.storage}_{N}", grid=N,
+                                        iterations=iters, tile=list(mixed_tile))
         roof = roofline_entry(
-            "km_jacobi (+ the correction launch)", algo, 1e3 * jac_ms / jac_n, None, None,
+            "km_jacobi (+ the correction launch)", algo, 1e3 * jac_ms / jac_n, traffic, tsrc,
             per_sweep_GBps=round(cells * (7 * (iters - 1) + 9) * args.steps
                                  / (jac_ms * 1e-3) / 1e9, 1))
     out = {"metric": f"sim steps/sec at {N}^2, {args.storage} storage",
@@ -661,8 +680,8 @@ def main():
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the copy-bandwidth probe (counter calibration)")
     ap.add_argument("--probe-variant", type=int, default=0,
-                    help="copy probe variant (0: float4 grid-stride, 5: float2, 6: float; "
-                         "-1: the best of all)")
+                    help="copy probe variant (0: float4 grid-stride, 5/6/7/8: 8/4/2/1-byte "
+                         "accesses; -1: the best of all)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 20 if args.workload == "learned" else 1000

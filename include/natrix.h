@@ -472,11 +472,11 @@ int nx_fp64_compare(const float* p, const double* x, const uint8_t* mask, int W,
 /* attainable bandwidth: a plain device copy of `bytes` (multiple of 16),
  * `iters` times; *gbps = (read + write bytes) / time.  nx_copy_bandwidth
  * runs the counter-calibration copy (one float4 per thread per grid-stride
- * iteration); nx_copy_bandwidth_variant runs variant 0-6 (0 = that copy,
+ * iteration); nx_copy_bandwidth_variant runs variant 0-8 (0 = that copy,
  * 1/2 = 4/8 float4 in flight per thread, 3 = 8 with non-temporal loads and
- * stores, 4 = one contiguous chunk per workgroup, 5/6 = 4 float2 / float
- * per thread) or, with variant -1, all of them and reports the best.  No
- * reference counterpart (measurement). */
+ * stores, 4 = one contiguous chunk per workgroup, 5/6/7/8 = 4 accesses of
+ * 8/4/2/1 bytes per thread) or, with variant -1, all of them and reports
+ * the best.  No reference counterpart (measurement). */
 int nx_copy_bandwidth(size_t bytes, int iters, double* gbps);
 int nx_copy_bandwidth_variant(size_t bytes, int iters, int variant, double* gbps);
 

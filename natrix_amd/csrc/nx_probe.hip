@@ -15,7 +15,9 @@
 //      (no grid stride: each workgroup streams its own region);
 //   5  float2 per lane (8-byte accesses, the stencil kernels' row loads),
 //      4 in flight per thread;
-//   6  float per lane (4-byte accesses), 4 in flight per thread.
+//   6  float per lane (4-byte accesses), 4 in flight per thread;
+//   7  2-byte accesses (the reduced-storage fields), 4 in flight;
+//   8  1-byte accesses (the obstacle masks), 4 in flight.
 // The narrower variants calibrate the traffic counters for kernels whose
 // loads are narrower than 16 bytes per lane.
 // nx_copy_bandwidth runs variant 0; nx_copy_bandwidth_variant(-1) runs all
@@ -116,7 +118,7 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-constexpr int N_VARIANTS = 7;
+constexpr int N_VARIANTS = 9;
 
 void launch_variant(int v, const float4* a, float4* b, size_t n, hipStream_t s) {
   const unsigned blocks = 256 * 32;  // 32 blocks per CU for the grid-stride forms
@@ -140,6 +142,14 @@ void launch_variant(int v, const float4* a, float4* b, size_t n, hipStream_t s) 
     case 6:
       k_copy_narrow<float, 4><<<blocks, 256, 0, s>>>(reinterpret_cast<const float*>(a),
                                                      reinterpret_cast<float*>(b), 4 * n);
+      break;
+    case 7:
+      k_copy_narrow<uint16_t, 4><<<blocks, 256, 0, s>>>(reinterpret_cast<const uint16_t*>(a),
+                                                        reinterpret_cast<uint16_t*>(b), 8 * n);
+      break;
+    case 8:
+      k_copy_narrow<uint8_t, 4><<<blocks, 256, 0, s>>>(reinterpret_cast<const uint8_t*>(a),
+                                                       reinterpret_cast<uint8_t*>(b), 16 * n);
       break;
     default: {
       // 8 workgroups per CU, each with one contiguous chunk
