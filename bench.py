@@ -456,6 +456,10 @@ def bench_learned(args, copy_bandwidth):
         sim.bc_mode = "inflow"
         sim.inflow = 1.0
         sim.set_obstacle_map(solid)
+        if args.tile:
+            sim.jacobi_tile = tuple(args.tile)
+        if args.smoother_tile:
+            sim.smoother_tile = tuple(args.smoother_tile)
         for _ in range(args.warmup if develop is None else develop):
             sim.update(dt)
         sim.synchronize()
@@ -650,6 +654,10 @@ def main():
     ap.add_argument("--check-every", type=int, default=1024)
     ap.add_argument("--tile", type=lambda s: [int(v) for v in s.split(",")],
                     default=None)
+    ap.add_argument("--smoother-tile", type=lambda s: [int(v) for v in s.split(",")],
+                    default=None,
+                    help="fine Chebyshev smoother launch shape k,rows,waves,cols[,seg] "
+                         "(learned workload; default: the autotuned shape)")
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")

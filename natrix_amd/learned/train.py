@@ -124,18 +124,25 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="FP64 conjugate-gradient iterations per training problem")
+    ap.add_argument("--targets", choices=["fp64", "jacobi"], default="fp64",
+                    help="fp64: FP64 conjugate-gradient solutions to --tol; jacobi: the "
+                         "batched float32 Jacobi solver run to its round-off floor")
+    ap.add_argument("--seed", type=int, default=0, help="initialisation and shuffling seed")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--time-budget", type=float, default=None,
                     help="stop training after this many seconds (weights are saved)")
     a = ap.parse_args(argv)
-    from .data import make_dataset
+    from .data import make_dataset, make_dataset_jacobi
     log = lambda m: print(m, flush=True)
     t0 = time.time()
-    b, solid, p, res = make_dataset(a.samples, a.size, tol=a.tol, max_iter=a.max_iter,
-                                    log=log)
+    if a.targets == "fp64":
+        b, solid, p, res = make_dataset(a.samples, a.size, tol=a.tol, max_iter=a.max_iter,
+                                        log=log)
+    else:
+        b, solid, p, res = make_dataset_jacobi(a.samples, a.size, tol=a.tol, log=log)
     # a target the solver did not bring to the tolerance is not a solution:
     # such samples are left out (and counted)
-    keep = res <= 10.0 * a.tol
+    keep = res <= 10.0 * a.tol if a.targets == "fp64" else np.isfinite(res)
     dropped = int((~keep).sum())
     if dropped:
         b, solid, p = b[keep], solid[keep], p[keep]
@@ -146,11 +153,13 @@ def main(argv=None):
                     "residual_median": float(np.median(res)), "residual_max": float(res.max()),
                     "frac_at_tol": float((res <= a.tol).mean()),
                     "dropped_unconverged": dropped, "residual_histogram": residual_hist}))
-    net, hist = train(b, solid, p, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log,
-                      time_budget=a.time_budget)
+    net, hist = train(b, solid, p, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed,
+                      log=log, time_budget=a.time_budget)
     meta = {"samples": a.samples, "size": a.size, "epochs": a.epochs,
-            "data": "FP64 conjugate gradients (nx_fp64_solve_batch), float32 rounding of "
-                    "the FP64 solution",
+            "data": ("FP64 conjugate gradients (nx_fp64_solve_batch), float32 rounding of "
+                     "the FP64 solution" if a.targets == "fp64" else
+                     "batched float32 Jacobi from zero to its round-off floor"),
+            "seed": a.seed,
             "data_tol": a.tol, "data_residual_median": float(np.median(res)),
             "data_residual_max": float(res.max()), "data_residual_histogram": residual_hist,
             "data_dropped_unconverged": dropped,

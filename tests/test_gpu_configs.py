@@ -76,7 +76,7 @@ def test_single_scene_configuration_2048(gpu):
 
 def test_single_scene_graph_replay_equals_per_shader_path_2048(gpu):
     """The fused step replayed from a HIP graph and the one-kernel-per-shader
-    path, 20 steps each at the full configuration: bitwise equal."""
+    path, 100 steps each at the full configuration: bitwise equal."""
     N = 2048
     solid = cylinder_2048()
     out = []
@@ -88,7 +88,7 @@ def test_single_scene_graph_replay_equals_per_shader_path_2048(gpu):
         sim.set_obstacle_map(solid)
         if fused:
             sim.use_graph = True
-        for _ in range(20):
+        for _ in range(100):
             sim.update(1 / 60)
         out.append((sim.download("velocity"), sim.download("pressure")))
         sim.destroy()
