@@ -550,6 +550,8 @@ int nx_create_ex(int width, int height, int storage, nx_sim** out) {
   ok &= hipMalloc(&s->state, 4 * sizeof(int)) == hipSuccess;
   ok &= hipHostMalloc(&s->host_state, 8 * sizeof(int), 0) == hipSuccess;
   ok &= hipHostMalloc(&s->host_out, 8 * sizeof(double), 0) == hipSuccess;
+  ok &= ok && hipHostGetDevicePointer(reinterpret_cast<void**>(&s->host_out_dev), s->host_out,
+                                      0) == hipSuccess;
   ok &= hipEventCreateWithFlags(&s->poll_ev[0], hipEventDisableTiming) == hipSuccess;
   ok &= hipEventCreateWithFlags(&s->poll_ev[1], hipEventDisableTiming) == hipSuccess;
   if (!ok) return cleanup(fail(NX_ERR_HIP, "device allocation failed"));

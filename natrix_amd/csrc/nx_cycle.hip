@@ -694,13 +694,7 @@ int nx_smooth(nx_sim* s, int sweeps, float omega, double* out) {
   NX_HIP(launch_sums(s->p[s->pcur], s->div, s->mask[s->mcur], s->W, s->H, s->red5,
                      s->stream));
   NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,
-                       nullptr, 0, 0, s->stream));
-  s->host_out[1] = 0.0;
-  NX_HIP(hipMemcpyAsync(s->host_out, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
-                        s->stream));
-  if (s->line)
-    NX_HIP(hipMemcpyAsync(s->host_out + 1, s->line, sizeof(double), hipMemcpyDeviceToHost,
-                          s->stream));
+                       nullptr, 0, 0, s->stream, s->host_out_dev, s->line));
   NX_HIP(hipStreamSynchronize(s->stream));
   const double rel = s->host_out[0], alpha = s->host_out[1];
   s->stats.sweeps = s->step_sweeps;
@@ -777,14 +771,10 @@ static int smooth_cheb_impl(nx_sim* s, int iterations, float lambda_min, const i
     done += its;
   }
   s->step_sweeps += iterations;
+  // the residual and the line-search step land in pinned host memory,
+  // written by the reduction itself
   NX_HIP(launch_reduce(s->cheb_red, last_blocks, 5, 0, s->sums, -1.0, nullptr, 0, 0,
-                       s->stream));
-  s->host_out[1] = 0.0;
-  NX_HIP(hipMemcpyAsync(s->host_out, s->sums + 5, sizeof(double), hipMemcpyDeviceToHost,
-                        s->stream));
-  if (s->line)
-    NX_HIP(hipMemcpyAsync(s->host_out + 1, s->line, sizeof(double), hipMemcpyDeviceToHost,
-                          s->stream));
+                       s->stream, s->host_out_dev, s->line));
   NX_HIP(hipStreamSynchronize(s->stream));
   const double rel = s->host_out[0], alpha = s->host_out[1];
   s->stats.sweeps = s->step_sweeps;
@@ -990,12 +980,10 @@ int nx_refine_residual(nx_sim* s, double* rel) {
       s->ph, s->pl, s->b0, s->mask[s->mcur], s->div, s->p[0], s->red5, s->W, s->H);
   NX_HIP(hipGetLastError());
   NX_HIP(launch_reduce(s->red5, residual_blocks(s->W, s->H), 5, 0, s->sums, -1.0,
-                       nullptr, 0, 0, s->stream));
-  NX_HIP(hipMemcpyAsync(s->host_out, s->sums, 6 * sizeof(double), hipMemcpyDeviceToHost,
-                        s->stream));
+                       nullptr, 0, 0, s->stream, s->host_out_dev));
   NX_HIP(hipStreamSynchronize(s->stream));
   s->pcur = 0;
-  *rel = s->host_out[5];
+  *rel = s->host_out[0];
   return NX_OK;
 }
 

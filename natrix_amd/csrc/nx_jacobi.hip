@@ -107,7 +107,8 @@ __device__ double rel_from(const double* s) {
 __global__ void __launch_bounds__(1024)
     k_reduce(const double* __restrict__ part, int nblocks, int width, int off,
              double* __restrict__ sums, double tol, int* __restrict__ state,
-             int sweeps_now, int final_idx) {
+             int sweeps_now, int final_idx, double* __restrict__ mirror,
+             const double* __restrict__ extra) {
   // one pass over the records: thread t adds records t, t + 1024, ... in
   // order, then the 16 wave sums are added in wave order (deterministic)
   double a[5] = {0, 0, 0, 0, 0};
@@ -130,6 +131,10 @@ __global__ void __launch_bounds__(1024)
     }
     const double rel = rel_from(sums);
     sums[5] = rel;
+    if (mirror != nullptr) {
+      mirror[0] = rel;
+      mirror[1] = extra != nullptr ? extra[0] : 0.0;
+    }
     if (state != nullptr) {
       if (state[0] == 0 && rel <= tol) {
         state[0] = 1;
@@ -241,9 +246,10 @@ hipError_t launch_sums(const float* p, const float* b, const uint8_t* mask,
 
 hipError_t launch_reduce(const double* part, int nblocks, int width, int off,
                          double* sums, double tol, int* state, int sweeps_now,
-                         int final_idx, hipStream_t s) {
+                         int final_idx, hipStream_t s, double* mirror,
+                         const double* extra) {
   k_reduce<<<1, 1024, 0, s>>>(part, nblocks, width, off, sums, tol, state,
-                              sweeps_now, final_idx);
+                              sweeps_now, final_idx, mirror, extra);
   return hipGetLastError();
 }
 

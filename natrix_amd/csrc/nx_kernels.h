@@ -168,9 +168,14 @@ hipError_t launch_sums(const float* p, const float* b, const uint8_t* mask,
 // sums[off..off+width), then sums[5] = relative residual from sums[0..4].
 // With `state` (tolerance-driven solve): the first time rel <= tol it sets
 // state[0]=1, state[1]=sweeps_now, state[2]=final_idx; state[3]=sweeps_now.
+// With `mirror` (a device pointer to pinned host memory): mirror[0] = the
+// relative residual and mirror[1] = extra[0] (0 without `extra`), written
+// by the kernel, so the host reads them after synchronising the stream
+// without a copy of its own.
 hipError_t launch_reduce(const double* part, int nblocks, int width, int off,
                          double* sums, double tol, int* state, int sweeps_now,
-                         int final_idx, hipStream_t s);
+                         int final_idx, hipStream_t s, double* mirror = nullptr,
+                         const double* extra = nullptr);
 
 // b -= fluid mean (sums[2]/sums[4]) and, if p != nullptr, p = first sweep
 // of the centred system, on fluid cells only.

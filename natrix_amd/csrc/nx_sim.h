@@ -60,6 +60,7 @@ struct nx_sim {
   int* state = nullptr;    // 4 ints
   int* host_state = nullptr;  // pinned, 2 slots x 4 ints
   double* host_out = nullptr;  // pinned, 8 doubles: per-call scalar read-backs
+  double* host_out_dev = nullptr;  // the same memory as the device addresses it
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
 
   nx_params prm;

@@ -55,6 +55,25 @@ def test_advect(scene, speed, diss):
     assert_bitwise(sim.download("velocity"), ref, "advect")
 
 
+def test_advect_long_back_traces(gpu):
+    """Velocities up to 3 per component (as unclamped Gaussian splats leave
+    them) back-trace up to 25 cells at speed 500, past the grid's edge from
+    many cells (corners clamped into the grid): bitwise equal to the
+    oracle."""
+    for (H, W) in [(37, 53), (130, 97), (96, 200)]:
+        rng = np.random.default_rng(H * 7 + W)
+        u = random_velocity(rng, H, W, 3.0)
+        m = blob_mask(rng, H, W)
+        sim = make_sim(H, W)
+        sim.upload("velocity", u)
+        sim.upload("mask", m)
+        sim.speed = 500.0
+        sim.stage("advect", DT)
+        ref = F.advect_velocity(u, obs_from_mask(m), DT, 500.0, 1.0)
+        assert_bitwise(sim.download("velocity"), ref, f"advect long back-traces {H}x{W}")
+        sim.destroy()
+
+
 def test_curl_and_confine(scene):
     sim, u, m, _ = scene
     sim.vorticity = 0.3
