@@ -203,7 +203,7 @@ def store_step_timed(W, H, tile, ms_per_step, dtype):
 # NX_STREAM_SHAPES in csrc/nx_stream.hip)
 CHEB_SHAPES = [(2, 16, 2), (4, 8, 2), (4, 16, 2), (8, 8, 1), (8, 16, 1)]
 CHEB_KS = (4, 6, 8, 10, 12, 16)
-STREAM_SHAPES = [(4, 8, 2), (2, 16, 2), (4, 16, 2), (8, 8, 2)]
+STREAM_SHAPES = [(4, 8, 2), (2, 16, 2), (4, 16, 2), (8, 8, 2), (8, 4, 2)]
 STREAM_KS = (8, 12, 16)
 STREAM_SEGS = (48, 80, 112, 144, 208)
 

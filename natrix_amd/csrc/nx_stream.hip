@@ -192,7 +192,9 @@ __global__ void __launch_bounds__(WAVES * 64)
   const float a_line = dapp != nullptr ? (float)line[0] : 0.0f;
 
   __shared__ Pub<EX> pub[2][WAVES];
-  __shared__ Pub<EX> car[STREAM_KMAX];
+  // the per-pass carry, k entries in dynamic LDS (sized by the launch)
+  extern __shared__ __align__(16) unsigned char nx_stream_carry[];
+  Pub<EX>* car = reinterpret_cast<Pub<EX>*>(nx_stream_carry);
 
   float X[ROWS][COLS], R[ROWS][COLS], D[ROWS][COLS];
   int SLB[COLS];  // packed solid flags, bit i: register row i
@@ -436,8 +438,9 @@ hipError_t launch_cheb_stream_shape(const float* xin, const float* rin, const fl
   constexpr int EX = 64 * COLS;
   const dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + seg - 1) / seg);
   const bool vec = COLS == 2 && W % 2 == 0 && k % 2 == 0;
+  const size_t carry = (size_t)k * sizeof(Pub<EX>);
 #define NX_CHEB_STREAM_GO(RESV, VECV)                                                   \
-  k_cheb_stream<ROWS, WAVES, COLS, RESV, VECV><<<grid, WAVES * 64, 0, st>>>(            \
+  k_cheb_stream<ROWS, WAVES, COLS, RESV, VECV><<<grid, WAVES * 64, carry, st>>>(        \
       xin, rin, din, b, mask, xout, rout, dout, W, H, k, seg, first, theta, cf, red, dapp, \
       line)
   if (red && vec)
@@ -533,7 +536,8 @@ __global__ void __launch_bounds__(WAVES * 64)
   static_assert(!VEC || COLS == 2, "float2 accesses take two columns per lane");
 
   __shared__ Pub<EX, 3> pub[2][WAVES];
-  __shared__ Pub<EX, 3> car[STREAM_KMAX];
+  extern __shared__ __align__(16) unsigned char nx_stream_carry[];
+  Pub<EX, 3>* car = reinterpret_cast<Pub<EX, 3>*>(nx_stream_carry);
 
   float P[ROWS][COLS], B[ROWS][COLS];
   int SLB[COLS];  // packed solid flags, bit i: register row i
@@ -701,11 +705,12 @@ hipError_t launch_jacobi_stream_shape(const float* pin, const float* b, const ui
                                       const int* done, hipStream_t st) {
   constexpr int EX = 64 * COLS;
   const dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + seg - 1) / seg);
+  const size_t carry = (size_t)k * sizeof(Pub<EX, 3>);
   if (COLS == 2 && W % 2 == 0 && k % 2 == 0)
-    k_jacobi_stream<ROWS, WAVES, COLS, COLS == 2><<<grid, WAVES * 64, 0, st>>>(
+    k_jacobi_stream<ROWS, WAVES, COLS, COLS == 2><<<grid, WAVES * 64, carry, st>>>(
         pin, b, mask, pout, W, H, k, seg, red, done);
   else
-    k_jacobi_stream<ROWS, WAVES, COLS, false><<<grid, WAVES * 64, 0, st>>>(
+    k_jacobi_stream<ROWS, WAVES, COLS, false><<<grid, WAVES * 64, carry, st>>>(
         pin, b, mask, pout, W, H, k, seg, red, done);
   return hipGetLastError();
 }
@@ -717,7 +722,8 @@ hipError_t launch_jacobi_stream_shape(const float* pin, const float* b, const ui
   X(4, 8, 2)                \
   X(2, 16, 2)               \
   X(4, 16, 2)               \
-  X(8, 8, 2)
+  X(8, 8, 2)                \
+  X(8, 4, 2)
 
 bool stream_cfg_supported(JacobiCfg c) {
   bool shape = false;

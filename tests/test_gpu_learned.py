@@ -289,7 +289,8 @@ def test_fine_chebyshev_matches_float32_statement(gpu, rng, H, W, tile, iters):
                                        ((8, 4, 8, 2, 7), 17), ((5, 4, 8, 2, 40), 23),
                                        ((16, 2, 16, 2, 48), 40), ((8, 4, 16, 2, 80), 8),
                                        ((7, 8, 8, 2, 33), 1), ((16, 8, 8, 2, 1000), 50),
-                                       ((2, 4, 8, 2, 16), 3)])
+                                       ((2, 4, 8, 2, 16), 3), ((8, 8, 4, 2, 80), 20),
+                                       ((6, 8, 4, 2, 33), 13)])
 def test_fine_chebyshev_streaming_matches_float32_statement(gpu, rng, H, W, cfg, iters):
     """The streaming smoother (a workgroup walks a column strip over `seg`
     output rows, carrying the rows above each chunk per pass) equals the

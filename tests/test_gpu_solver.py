@@ -45,7 +45,7 @@ def test_jacobi_blocked_bitwise(gpu, rows, waves, cols):
         sim.destroy()
 
 
-STREAM_TILES = [(4, 8, 2), (2, 16, 2), (4, 16, 2), (8, 8, 2)]
+STREAM_TILES = [(4, 8, 2), (2, 16, 2), (4, 16, 2), (8, 8, 2), (8, 4, 2)]
 
 
 @pytest.mark.parametrize("rows,waves,cols", STREAM_TILES)
