@@ -377,23 +377,58 @@ __global__ void __launch_bounds__(1024)
   const int band0 = blockIdx.x * CB_BAND;
   const int y0 = band0 - (CB_T + 1) + CB_ROWS * t.wave;
   const bool first = s0 == 0;
-  auto solid = [&](int x, int y) {
-    return x < 0 || y < 0 || x >= Wc || y >= Hc || mc[y * Wc + x] != 0;
-  };
-  float x[CB_ROWS][2], r[CB_ROWS][2], d[CB_ROWS][2], lv[CB_ROWS][2];
+  // Every load is unconditional, from a clamped in-grid address, issued
+  // before the first use: x, r (b at the first launch) and d of the lane's
+  // cells, and the mask of its two columns over the rows y0 - 1 .. y0 +
+  // CB_ROWS; the neighbours' flags to the left and right come from the
+  // adjacent lanes, and a neighbour outside the grid counts as solid.
+  const float* rsrc = first ? rhs : rin;
+  float lx[CB_ROWS][2], lr[CB_ROWS][2], ld[CB_ROWS][2];
+  uint8_t lm[CB_ROWS + 2][2];
 #pragma unroll
-  for (int i = 0; i < CB_ROWS; ++i)
+  for (int c = 0; c < 2; ++c) {
+    const int cx = min(t.x0 + c, Wc - 1);
+#pragma unroll
+    for (int i = 0; i < CB_ROWS; ++i) {
+      const int pos = min(max(y0 + i, 0), Hc - 1) * Wc + cx;
+      lx[i][c] = xin[pos];
+      lr[i][c] = rsrc[pos];
+      ld[i][c] = din[pos];
+    }
+#pragma unroll
+    for (int i = 0; i < CB_ROWS + 2; ++i) lm[i][c] = mc[min(max(y0 - 1 + i, 0), Hc - 1) * Wc + cx];
+  }
+  // solid flags of the lane's two columns on rows y0 - 1 + i
+  int sf[CB_ROWS + 2][2];
+#pragma unroll
+  for (int i = 0; i < CB_ROWS + 2; ++i)
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      const int gx = t.x0 + c, gy = y0 + i;
-      const bool in = gx < Wc && gy >= 0 && gy < Hc;
-      const int pos = in ? gy * Wc + gx : 0;
-      x[i][c] = in ? xin[pos] : 0.0f;
-      r[i][c] = in ? (first ? rhs[pos] : rin[pos]) : 0.0f;
-      d[i][c] = (in && !first) ? din[pos] : 0.0f;
-      t.code[i][c] = (solid(gx - 1, gy) ? 1 : 0) | (solid(gx + 1, gy) ? 2 : 0) |
-                     (solid(gx, gy - 1) ? 4 : 0) | (solid(gx, gy + 1) ? 8 : 0);
+      const int gy = y0 - 1 + i;
+      sf[i][c] = (t.x0 + c < Wc && gy >= 0 && gy < Hc) ? lm[i][c] != 0 : 1;
     }
+  float x[CB_ROWS][2], r[CB_ROWS][2], d[CB_ROWS][2], lv[CB_ROWS][2];
+#pragma unroll
+  for (int i = 0; i < CB_ROWS; ++i) {
+    const int gy = y0 + i;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const bool in = t.x0 + c < Wc && gy >= 0 && gy < Hc;
+      x[i][c] = in ? lx[i][c] : 0.0f;
+      r[i][c] = in ? lr[i][c] : 0.0f;
+      d[i][c] = (in && !first) ? ld[i][c] : 0.0f;
+    }
+    // column x0 - 1 is the left lane's second column, x0 + 2 the right
+    // lane's first; beyond the grid's first column and past its last: solid.
+    // The lane shifts run with every lane active (a lane shift under a
+    // branch would read disabled lanes), the selects come after.
+    const int lnb = jt::ilane_from_left(sf[i + 1][1]);
+    const int rnb = jt::ilane_from_right(sf[i + 1][0]);
+    const int left = t.x0 == 0 ? 1 : lnb;
+    const int right = t.x0 + 2 >= Wc ? 1 : rnb;
+    t.code[i][0] = left | (sf[i + 1][1] << 1) | (sf[i][0] << 2) | (sf[i + 2][0] << 3);
+    t.code[i][1] = sf[i + 1][0] | (right << 1) | (sf[i][1] << 2) | (sf[i + 2][1] << 3);
+  }
   if (first) {
     t.lap(x, lv);
 #pragma unroll
