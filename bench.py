@@ -132,7 +132,7 @@ def cheb_launch_plan(iterations, k):
 
 
 def baseline_formula(cells, K, ms_per_step):
-    """The step-level figure of the survey's bandwidth model:
+    """The step-level figure of the path's bandwidth model:
     N^2 (104 + 13 K) bytes per step (each array once per stage, K Jacobi /
     smoother passes each 13 B) over the step time.  Temporal blocking keeps
     iterates on chip between passes, so this ratio to the HBM peak is not
