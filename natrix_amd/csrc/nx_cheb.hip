@@ -24,8 +24,10 @@
 // k-pass dependency cone stayed inside the tile is written back.  The first
 // launch of a run spends its first pass on the setup, the last one (RES)
 // its last pass on the FP64 residual of the result.  Every pass evaluates
-// each cell's expressions in a fixed order without contraction, so the
-// result is bitwise independent of the tile shape and of k.
+// each cell's expressions in a fixed order without contraction (the -4 v
+// of L v is one fused multiply-add, which rounds like the subtraction
+// because 4 v is exact), so the result is bitwise independent of the tile
+// shape and of k.
 #include <type_traits>
 
 #include "../../include/natrix.h"
@@ -47,6 +49,7 @@ using jt::ilane_from_right;
 // and writes red[5 blk + {sum r, sum r^2, sum b, sum b^2, fluid cells}].
 template <int ROWS, int WAVES, int COLS, bool RES>
 __global__ void __launch_bounds__(WAVES * 64)
+    __attribute__((amdgpu_waves_per_eu(jt::cheb_waves_per_eu<ROWS * COLS, RES>())))
     k_cheb_tb(const float* __restrict__ xin, const float* __restrict__ rin,
               const float* __restrict__ din, const float* __restrict__ b,
               const uint8_t* __restrict__ mask, float* __restrict__ xout,
@@ -58,8 +61,12 @@ __global__ void __launch_bounds__(WAVES * 64)
   // wave-uniform: a scalar register, so branches on it are scalar branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int VX = EX - 2 * k, VY = EY - 2 * k;
-  const int gx0 = blockIdx.x * VX - k + COLS * lane;
-  const int gy0 = blockIdx.y * VY - k + wave * ROWS;
+  // 1D grid of gx * gy tiles in XCD order (xcd_tile); the partial-sum
+  // record is the tile's row-major index
+  const int gx = (W + VX - 1) / VX, gy = (H + VY - 1) / VY;
+  const int tile = xcd_tile((int)blockIdx.x, gx * gy);
+  const int gx0 = (tile % gx) * VX - k + COLS * lane;
+  const int gy0 = (tile / gx) * VY - k + wave * ROWS;
   const int tcol0 = COLS * lane, trow0 = wave * ROWS;
 
   // float2 accesses when every lane's column pair is aligned and wholly
@@ -328,11 +335,10 @@ __global__ void __launch_bounds__(WAVES * 64)
       for (int j = 0; j < 5; ++j) part[wave][j] = acc[j];
     __syncthreads();
     if (threadIdx.x == 0) {
-      const int bid = blockIdx.y * gridDim.x + blockIdx.x;
       for (int j = 0; j < 5; ++j) {
         double t = 0.0;
         for (int w = 0; w < WAVES; ++w) t += part[w][j];
-        red[5 * bid + j] = t;
+        red[5 * tile + j] = t;
       }
     }
   }
@@ -371,7 +377,7 @@ hipError_t launch_cheb(const float* xin, const float* rin, const float* din, con
                        int H, int k, int first, float theta, const ChebCoef& cf, double* red,
                        const float* dapp, const double* line, hipStream_t st) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
-  const dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + EY - 2 * k - 1) / (EY - 2 * k));
+  const dim3 grid(((W + EX - 2 * k - 1) / (EX - 2 * k)) * ((H + EY - 2 * k - 1) / (EY - 2 * k)));
   if (red)
     k_cheb_tb<ROWS, WAVES, COLS, true><<<grid, WAVES * 64, 0, st>>>(
         xin, rin, din, b, mask, xout, rout, dout, W, H, k, first, theta, cf, red, dapp, line);

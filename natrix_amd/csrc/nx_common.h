@@ -224,4 +224,34 @@ struct Splat {  // Gaussian splat: u += (ax, ay) * w, dye += dye * w
   float px, py, ax, ay, radius, dye;
 };
 
+// The chip dispatches the workgroups of a launch to its 8 XCDs round-robin
+// (workgroup i to XCD i mod 8), and each XCD has its own L2.  Kernels whose
+// blocks read overlapping halos launch a 1D grid of T blocks and take block
+// (tile) xcd_tile(i, T) in workgroup i, so that XCD x works through the
+// contiguous row-major run of tiles [x q + min(x, r), ...) of length
+// q + (x < r) (q = T / 8, r = T mod 8) in order: neighbouring tiles then run
+// on one XCD close together in time, and the rows and columns they share
+// are read from that XCD's L2 instead of from memory.  A bijection of
+// [0, T) for every T.
+__device__ __forceinline__ int xcd_tile(int i, int T) {
+  const int x = i & 7, j = i >> 3, q = T >> 3, r = T & 7;
+  return x * q + min(x, r) + j;
+}
+
+// A 1D grid of the 64 x 4 cell blocks that cover a W x H grid (one
+// wavefront per 64 consecutive x of a row), and the cell of the calling
+// thread when workgroup i takes block xcd_tile(i, blocks): the rows a
+// stencil shares with the blocks above and below are then read once per XCD.
+inline dim3 cell_grid(int W, int H) {
+  return dim3((unsigned)(((W + 63) / 64) * ((H + 3) / 4)), 1, 1);
+}
+
+__device__ __forceinline__ bool cell_of(int& x, int& y, int W, int H) {
+  const int gx = (W + 63) >> 6;
+  const int t = xcd_tile((int)blockIdx.x, gx * ((H + 3) >> 2));
+  x = (t % gx) * 64 + (int)threadIdx.x;
+  y = (t / gx) * 4 + (int)threadIdx.y;
+  return x < W && y < H;
+}
+
 }  // namespace nx

@@ -54,35 +54,46 @@ __global__ void __launch_bounds__(256)
   rcnt[y * Wc + X] = (float)fluid;
 }
 
-// Row streaming for the residual-type kernels below: wave w of a 64 x
-// RES_ROWS block walks rows y0 .. y0 + RES_ROWS/4 - 1 of column x = 64 bx +
-// lane, holding the rows above and below in registers; the left and right
-// neighbours come from the adjacent lanes by DPP, lanes 0 and 63 load theirs
-// (x - 1, x + 1) and the grid's edges use the cell itself, as the clamped
-// index does.  Lanes past the last column read column W - 1 and write
-// nothing, so every lane runs the same code.
-struct RowNbr {
-  float l, r;
-  int ml, mr;
-};
+// Column strips for the residual-type kernels below: wave w of a 64 x
+// RES_ROWS block owns rows y0 .. y0 + RS - 1 (RS = RES_ROWS / 4) of column
+// x = 64 bx + lane.  Every load of a strip is issued before any is used
+// (one wait per wave, not one per row): rows y0 - 1 .. y0 + RS of the column
+// (clamped to the grid, so an edge row reads the row itself, as the clamped
+// index does) and, per row, the value one column beyond the lane's 64-wide
+// segment (x - 1 for lane 0, x + 1 for the others, clamped).  The left and
+// right neighbours then come from the adjacent lanes by DPP, lanes 0 and 63
+// take the loaded values and the grid's right edge the cell itself.  Lanes
+// past the last column read column W - 1 and write nothing, and rows past
+// the last read row H - 1 and are skipped, so every lane runs the same code.
+constexpr int RS = RES_ROWS / 4;
 
-__device__ __forceinline__ RowNbr row_nbr(const float* __restrict__ p,
-                                          const uint8_t* __restrict__ mask, int row, int x,
-                                          int W, int lane, float pc, int mcv) {
-  RowNbr n;
-  n.l = jt::lane_from_left(pc);
-  n.r = jt::lane_from_right(pc);
-  n.ml = jt::ilane_from_left(mcv);
-  n.mr = jt::ilane_from_right(mcv);
-  if (lane == 0) {
-    n.l = x > 0 ? p[row + x - 1] : pc;
-    n.ml = x > 0 ? mask[row + x - 1] : mcv;
-  }
-  if (lane == 63 || x + 1 >= W) {
-    n.r = x + 1 < W ? p[row + x + 1] : pc;
-    n.mr = x + 1 < W ? mask[row + x + 1] : mcv;
-  }
-  return n;
+template <typename T, typename V>
+__device__ __forceinline__ void strip_load(const T* __restrict__ a, V (&col)[RS + 2],
+                                           V (&edge)[RS], int y0, int xc, int ex, int W,
+                                           int H) {
+#pragma unroll
+  for (int i = 0; i < RS + 2; ++i) col[i] = (V)a[min(max(y0 - 1 + i, 0), H - 1) * W + xc];
+#pragma unroll
+  for (int i = 0; i < RS; ++i) edge[i] = (V)a[min(y0 + i, H - 1) * W + ex];
+}
+
+// neighbours of a row's cell c with the row's loaded edge value e; the
+// shifts run with every lane active (no branch around them)
+__device__ __forceinline__ float nbr_left(float c, float e, int lane) {
+  const float s = jt::lane_from_left(c);
+  return lane == 0 ? e : s;
+}
+__device__ __forceinline__ float nbr_right(float c, float e, int lane, bool last) {
+  const float s = jt::lane_from_right(c);
+  return last ? c : (lane == 63 ? e : s);
+}
+__device__ __forceinline__ int inbr_left(int c, int e, int lane) {
+  const int s = jt::ilane_from_left(c);
+  return lane == 0 ? e : s;
+}
+__device__ __forceinline__ int inbr_right(int c, int e, int lane, bool last) {
+  const int s = jt::ilane_from_right(c);
+  return last ? c : (lane == 63 ? e : s);
 }
 
 // The same row records when f is a power of two dividing 64: lane l owns
@@ -95,26 +106,31 @@ __global__ void __launch_bounds__(256)
                         float* __restrict__ rcnt, int W, int H, int f, int Wc) {
   const int lane = threadIdx.x, wave = threadIdx.y;
   const int x = blockIdx.x * 64 + lane, xc = min(x, W - 1);
-  const int y0 = blockIdx.y * RES_ROWS + wave * (RES_ROWS / 4);
+  const int ex = lane == 0 ? max(x - 1, 0) : min(x + 1, W - 1);
+  const int y0 = blockIdx.y * RES_ROWS + wave * RS;
   if (y0 >= H) return;  // wave-uniform
-  const int y1 = min(y0 + RES_ROWS / 4, H);
-  float pc = p[y0 * W + xc];
-  int mcv = mask[y0 * W + xc];
-  float pu = y0 > 0 ? p[(y0 - 1) * W + xc] : pc;
-  int mu = y0 > 0 ? mask[(y0 - 1) * W + xc] : mcv;
-  for (int y = y0; y < y1; ++y) {
-    const int row = y * W;
-    const bool dn = y + 1 < H;
-    const float pd = dn ? p[row + W + xc] : pc;
-    const int md = dn ? mask[row + W + xc] : mcv;
-    const RowNbr n = row_nbr(p, mask, row, x, W, lane, pc, mcv);
+  const bool last = x + 1 >= W;
+  float pv[RS + 2], pe[RS], bv[RS];
+  int mv[RS + 2], me[RS];
+  strip_load(p, pv, pe, y0, xc, ex, W, H);
+  strip_load(mask, mv, me, y0, xc, ex, W, H);
+#pragma unroll
+  for (int i = 0; i < RS; ++i) bv[i] = b[min(y0 + i, H - 1) * W + xc];
+#pragma unroll
+  for (int i = 0; i < RS; ++i) {
+    const int y = y0 + i;
+    if (y >= H) break;  // wave-uniform
+    const float pc = pv[i + 1];
+    const int mcv = mv[i + 1];
+    const float l = nbr_left(pc, pe[i], lane), r = nbr_right(pc, pe[i], lane, last);
+    const int ml = inbr_left(mcv, me[i], lane), mr = inbr_right(mcv, me[i], lane, last);
     float v = 0.0f, c = 0.0f;
     if (x < W && mcv == 0) {
-      const float xl = n.ml ? pc : n.l;
-      const float xr = n.mr ? pc : n.r;
-      const float yb = mu ? pc : pu;
-      const float yt = md ? pc : pd;
-      v = b[row + x] - ((((xl + xr) + yb) + yt) - 4.0f * pc);
+      const float xl = ml ? pc : l;
+      const float xr = mr ? pc : r;
+      const float yb = mv[i] ? pc : pv[i];
+      const float yt = mv[i + 2] ? pc : pv[i + 2];
+      v = bv[i] - ((((xl + xr) + yb) + yt) - 4.0f * pc);
       c = 1.0f;
     }
     for (int off = f >> 1; off > 0; off >>= 1) {
@@ -125,10 +141,6 @@ __global__ void __launch_bounds__(256)
       rsum[y * Wc + x / f] = v;
       rcnt[y * Wc + x / f] = c;
     }
-    pu = pc;
-    mu = mcv;
-    pc = pd;
-    mcv = md;
   }
 }
 
@@ -167,27 +179,31 @@ __global__ void __launch_bounds__(256)
   const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
   if (x >= W || y >= H) return;
   const int pos = y * W + x;
+  // the coarse indices depend on the cell's position only, so the mask,
+  // the four coarse masks and corrections are loaded together and the
+  // fine mask selects at the end (a solid cell gets no correction)
+  const float cx = ((float)x - 0.5f * (float)(f - 1)) / (float)f;
+  const float cy = ((float)y - 0.5f * (float)(f - 1)) / (float)f;
+  const float fx = fminf(fmaxf(cx, 0.0f), (float)(Wc - 1));
+  const float fy = fminf(fmaxf(cy, 0.0f), (float)(Hc - 1));
+  const int x0 = (int)fx, y0 = (int)fy;
+  const int x1 = min(x0 + 1, Wc - 1), y1 = min(y0 + 1, Hc - 1);
+  const float tx = fx - (float)x0, ty = fy - (float)y0;
+  const int i00 = y0 * Wc + x0, i01 = y0 * Wc + x1, i10 = y1 * Wc + x0, i11 = y1 * Wc + x1;
+  const uint8_t m = mask[pos];
+  const uint8_t m00 = mc[i00], m01 = mc[i01], m10 = mc[i10], m11 = mc[i11];
+  const float e00 = ec[i00], e01 = ec[i01], e10 = ec[i10], e11 = ec[i11];
+  const float pold = ADD ? p[pos] : 0.0f;
+  const float w00 = m00 ? 0.0f : (1.0f - tx) * (1.0f - ty);
+  const float w01 = m01 ? 0.0f : tx * (1.0f - ty);
+  const float w10 = m10 ? 0.0f : (1.0f - tx) * ty;
+  const float w11 = m11 ? 0.0f : tx * ty;
+  const float wsum = ((w00 + w01) + w10) + w11;
   float v = 0.0f;
-  if (mask[pos] == 0) {
-    const float cx = ((float)x - 0.5f * (float)(f - 1)) / (float)f;
-    const float cy = ((float)y - 0.5f * (float)(f - 1)) / (float)f;
-    const float fx = fminf(fmaxf(cx, 0.0f), (float)(Wc - 1));
-    const float fy = fminf(fmaxf(cy, 0.0f), (float)(Hc - 1));
-    const int x0 = (int)fx, y0 = (int)fy;
-    const int x1 = min(x0 + 1, Wc - 1), y1 = min(y0 + 1, Hc - 1);
-    const float tx = fx - (float)x0, ty = fy - (float)y0;
-    const int i00 = y0 * Wc + x0, i01 = y0 * Wc + x1, i10 = y1 * Wc + x0,
-              i11 = y1 * Wc + x1;
-    const float w00 = mc[i00] ? 0.0f : (1.0f - tx) * (1.0f - ty);
-    const float w01 = mc[i01] ? 0.0f : tx * (1.0f - ty);
-    const float w10 = mc[i10] ? 0.0f : (1.0f - tx) * ty;
-    const float w11 = mc[i11] ? 0.0f : tx * ty;
-    const float wsum = ((w00 + w01) + w10) + w11;
-    if (wsum > 0.0f)
-      v = (((w00 * ec[i00] + w01 * ec[i01]) + w10 * ec[i10]) + w11 * ec[i11]) / wsum;
-  }
+  if (m == 0 && wsum > 0.0f)
+    v = (((w00 * e00 + w01 * e01) + w10 * e10) + w11 * e11) / wsum;
   if (ADD) {
-    if (mask[pos] == 0) p[pos] = p[pos] + scale * v;
+    if (m == 0) p[pos] = pold + scale * v;
   } else {
     p[pos] = scale * v;
   }
@@ -515,47 +531,44 @@ __global__ void __launch_bounds__(256)
     k_line_terms(const float* __restrict__ p, const float* __restrict__ b,
                  const uint8_t* __restrict__ mask, const float* __restrict__ d,
                  double* __restrict__ red, int W, int H) {
-  // rows stream as in k_restrict_rows_seg; a solid neighbour is left out
-  // (its difference to the centre is 0), a clamped one is the cell itself
+  // column strips as in k_restrict_rows_seg; a solid neighbour is left
+  // out (its difference to the centre is 0), a clamped one is the cell itself
   const int lane = threadIdx.x, wave = threadIdx.y;
   const int x = blockIdx.x * 64 + lane, xc = min(x, W - 1);
-  const int y0 = blockIdx.y * RES_ROWS + wave * (RES_ROWS / 4);
-  const int y1 = min(y0 + RES_ROWS / 4, H);
+  const int ex = lane == 0 ? max(x - 1, 0) : min(x + 1, W - 1);
+  const int y0 = blockIdx.y * RES_ROWS + wave * RS;
   double v0 = 0.0, v1 = 0.0;
-  if (y0 < H) {
-    float pc = p[y0 * W + xc], dc = d[y0 * W + xc];
-    int mcv = mask[y0 * W + xc];
-    float pu = y0 > 0 ? p[(y0 - 1) * W + xc] : pc;
-    float du = y0 > 0 ? d[(y0 - 1) * W + xc] : dc;
-    int mu = y0 > 0 ? mask[(y0 - 1) * W + xc] : mcv;
-    for (int y = y0; y < y1; ++y) {
-      const int row = y * W;
-      const bool dn = y + 1 < H;
-      const float pd = dn ? p[row + W + xc] : pc;
-      const float dd = dn ? d[row + W + xc] : dc;
-      const int md = dn ? mask[row + W + xc] : mcv;
-      const RowNbr np = row_nbr(p, mask, row, x, W, lane, pc, mcv);
-      const RowNbr nd = row_nbr(d, mask, row, x, W, lane, dc, mcv);
+  if (y0 < H) {  // wave-uniform
+    const bool last = x + 1 >= W;
+    float pv[RS + 2], pe[RS], dv[RS + 2], de[RS], bv[RS];
+    int mv[RS + 2], me[RS];
+    strip_load(p, pv, pe, y0, xc, ex, W, H);
+    strip_load(d, dv, de, y0, xc, ex, W, H);
+    strip_load(mask, mv, me, y0, xc, ex, W, H);
+#pragma unroll
+    for (int i = 0; i < RS; ++i) bv[i] = b[min(y0 + i, H - 1) * W + xc];
+#pragma unroll
+    for (int i = 0; i < RS; ++i) {
+      if (y0 + i >= H) break;  // wave-uniform
+      const float pc = pv[i + 1], dc = dv[i + 1];
+      const int mcv = mv[i + 1];
+      const float pl = nbr_left(pc, pe[i], lane), pr = nbr_right(pc, pe[i], lane, last);
+      const float dl = nbr_left(dc, de[i], lane), dr = nbr_right(dc, de[i], lane, last);
+      const int ml = inbr_left(mcv, me[i], lane), mr = inbr_right(mcv, me[i], lane, last);
       if (x < W && mcv == 0) {
         const double pcd = (double)pc, dcd = (double)dc;
-        const float pv[4] = {np.l, np.r, pu, pd}, dv[4] = {nd.l, nd.r, du, dd};
-        const int sv[4] = {np.ml, np.mr, mu, md};
+        const float pn[4] = {pl, pr, pv[i], pv[i + 2]}, dn[4] = {dl, dr, dv[i], dv[i + 2]};
+        const int sv[4] = {ml, mr, mv[i], mv[i + 2]};
         double lp = 0.0, ld = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           if (sv[j]) continue;
-          lp += (double)pv[j] - pcd;
-          ld += (double)dv[j] - dcd;
+          lp += (double)pn[j] - pcd;
+          ld += (double)dn[j] - dcd;
         }
-        v0 += dcd * ((double)b[row + x] - lp);
+        v0 += dcd * ((double)bv[i] - lp);
         v1 += dcd * ld;
       }
-      pu = pc;
-      du = dc;
-      mu = mcv;
-      pc = pd;
-      dc = dd;
-      mcv = md;
     }
   }
   __shared__ double part[4][2];
@@ -632,34 +645,38 @@ __global__ void __launch_bounds__(256)
                       const float* __restrict__ b0, const uint8_t* __restrict__ mask,
                       float* __restrict__ r_out, float* __restrict__ p1,
                       double* __restrict__ red, int W, int H) {
-  // rows stream as in k_restrict_rows_seg, with the pair (ph, pl) per cell
+  // column strips as in k_restrict_rows_seg, with the pair (ph, pl) per cell
   const int lane = threadIdx.x, wave = threadIdx.y;
   const int x = blockIdx.x * 64 + lane, xc = min(x, W - 1);
-  const int y0 = blockIdx.y * RES_ROWS + wave * (RES_ROWS / 4);
-  const int y1 = min(y0 + RES_ROWS / 4, H);
+  const int ex = lane == 0 ? max(x - 1, 0) : min(x + 1, W - 1);
+  const int y0 = blockIdx.y * RES_ROWS + wave * RS;
   double v[5] = {0, 0, 0, 0, 0};
-  if (y0 < H) {
-    float hc = ph[y0 * W + xc], lc = pl[y0 * W + xc];
-    int mcv = mask[y0 * W + xc];
-    float hu = y0 > 0 ? ph[(y0 - 1) * W + xc] : hc;
-    float lu = y0 > 0 ? pl[(y0 - 1) * W + xc] : lc;
-    int mu = y0 > 0 ? mask[(y0 - 1) * W + xc] : mcv;
-    for (int y = y0; y < y1; ++y) {
-      const int row = y * W;
-      const bool dn = y + 1 < H;
-      const float hd = dn ? ph[row + W + xc] : hc;
-      const float ld = dn ? pl[row + W + xc] : lc;
-      const int md = dn ? mask[row + W + xc] : mcv;
-      const RowNbr nh = row_nbr(ph, mask, row, x, W, lane, hc, mcv);
-      const RowNbr nl = row_nbr(pl, mask, row, x, W, lane, lc, mcv);
+  if (y0 < H) {  // wave-uniform
+    const bool last = x + 1 >= W;
+    float hv[RS + 2], he[RS], lv[RS + 2], le[RS], bv[RS];
+    int mv[RS + 2], me[RS];
+    strip_load(ph, hv, he, y0, xc, ex, W, H);
+    strip_load(pl, lv, le, y0, xc, ex, W, H);
+    strip_load(mask, mv, me, y0, xc, ex, W, H);
+#pragma unroll
+    for (int i = 0; i < RS; ++i) bv[i] = b0[min(y0 + i, H - 1) * W + xc];
+#pragma unroll
+    for (int i = 0; i < RS; ++i) {
+      const int y = y0 + i;
+      if (y >= H) break;  // wave-uniform
+      const float hc = hv[i + 1], lc = lv[i + 1];
+      const int mcv = mv[i + 1];
+      const float hl = nbr_left(hc, he[i], lane), hr = nbr_right(hc, he[i], lane, last);
+      const float ll = nbr_left(lc, le[i], lane), lr = nbr_right(lc, le[i], lane, last);
+      const int ml = inbr_left(mcv, me[i], lane), mr = inbr_right(mcv, me[i], lane, last);
       if (x < W) {
-        const int pos = row + x;
+        const int pos = y * W + x;
         const double pc = (double)hc + (double)lc;
-        const double x1 = nh.ml ? pc : (double)nh.l + (double)nl.l;
-        const double x2 = nh.mr ? pc : (double)nh.r + (double)nl.r;
-        const double y1v = mu ? pc : (double)hu + (double)lu;
-        const double y2v = md ? pc : (double)hd + (double)ld;
-        const double bd = (double)b0[pos];
+        const double x1 = ml ? pc : (double)hl + (double)ll;
+        const double x2 = mr ? pc : (double)hr + (double)lr;
+        const double y1v = mv[i] ? pc : (double)hv[i] + (double)lv[i];
+        const double y2v = mv[i + 2] ? pc : (double)hv[i + 2] + (double)lv[i + 2];
+        const double bd = (double)bv[i];
         const double rr = bd - ((((x1 + x2) + y1v) + y2v) - 4.0 * pc);
         const float rf = mcv ? 0.0f : (float)rr;
         r_out[pos] = rf;
@@ -672,12 +689,6 @@ __global__ void __launch_bounds__(256)
           v[4] += 1.0;
         }
       }
-      hu = hc;
-      lu = lc;
-      mu = mcv;
-      hc = hd;
-      lc = ld;
-      mcv = md;
     }
   }
   __shared__ double part[4][5];

@@ -29,9 +29,11 @@ __global__ void __launch_bounds__(256)
                      float2* __restrict__ uout, float* __restrict__ div,
                      float* __restrict__ p1, int W, int H, float dt,
                      float scale, int viscous, float alpha, float rbeta) {
-  fused::fused_vort_div_tile((int)blockIdx.x * TX, (int)blockIdx.y * TY, uin,
-                             mask, uout, div, p1, W, H, dt, scale, viscous,
-                             alpha, rbeta);
+  // 1D grid of TX x TY tiles in XCD order
+  const int gx = (W + TX - 1) / TX;
+  const int t = xcd_tile((int)blockIdx.x, gx * ((H + TY - 1) / TY));
+  fused::fused_vort_div_tile((t % gx) * TX, (t / gx) * TY, uin, mask, uout, div, p1, W, H,
+                             dt, scale, viscous, alpha, rbeta);
 }
 
 }  // namespace
@@ -40,7 +42,7 @@ hipError_t launch_fused_vort_div(const float2* uin, const uint8_t* mask,
                                  float2* uout, float* div, float* p1, int W,
                                  int H, float dt, float scale, int viscous,
                                  float alpha, float rbeta, hipStream_t s) {
-  dim3 grid((W + TX - 1) / TX, (H + TY - 1) / TY, 1);
+  const dim3 grid(((W + TX - 1) / TX) * ((H + TY - 1) / TY), 1, 1);
   k_fused_vort_div<<<grid, dim3(64, 4), 0, s>>>(uin, mask, uout, div, p1, W,
                                                 H, dt, scale, viscous, alpha,
                                                 rbeta);

@@ -30,14 +30,19 @@ using jt::wave_sum;
 
 template <int ROWS, int WAVES, int COLS>
 __global__ void __launch_bounds__(WAVES * 64)
+    __attribute__((amdgpu_waves_per_eu(jt::jacobi_waves_per_eu<ROWS * COLS>())))
     k_jacobi_tb(const float* __restrict__ pin, const float* __restrict__ b,
                 const uint8_t* __restrict__ mask, float* __restrict__ pout,
                 int W, int H, int k, double* __restrict__ red,
                 const int* __restrict__ done) {
   if (done != nullptr && *done != 0) return;  // block-uniform early exit
-  jt::jacobi_tb_tile<ROWS, WAVES, COLS>(
-      (int)blockIdx.x, (int)blockIdx.y, pin, b, mask, pout, W, H, k, red,
-      (int)(blockIdx.y * gridDim.x + blockIdx.x));
+  // 1D grid of gx * gy tiles in XCD order; the partial-sum record is the
+  // tile's row-major index
+  const int gx = (W + 64 * COLS - 2 * k - 1) / (64 * COLS - 2 * k);
+  const int gy = (H + ROWS * WAVES - 2 * k - 1) / (ROWS * WAVES - 2 * k);
+  const int t = xcd_tile((int)blockIdx.x, gx * gy);
+  jt::jacobi_tb_tile<ROWS, WAVES, COLS>(t % gx, t / gx, pin, b, mask, pout, W, H, k, red,
+                                        t);
 }
 
 // Exact FP64 residual of p (or of a zero guess when p == nullptr) plus the
@@ -172,7 +177,7 @@ hipError_t launch_tb(const float* pin, const float* b, const uint8_t* mask,
                      const int* done, hipStream_t s) {
   constexpr int EX = 64 * COLS, EY = ROWS * WAVES;
   const int VX = EX - 2 * k, VY = EY - 2 * k;
-  dim3 grid((W + VX - 1) / VX, (H + VY - 1) / VY, 1);
+  const dim3 grid(((W + VX - 1) / VX) * ((H + VY - 1) / VY), 1, 1);
   k_jacobi_tb<ROWS, WAVES, COLS><<<grid, WAVES * 64, 0, s>>>(
       pin, b, mask, pout, W, H, k, red, done);
   return hipGetLastError();

@@ -8,6 +8,20 @@
 namespace nx {
 namespace jt {
 
+// Waves per SIMD asked of the compiler for a tile kernel of CELLS cells
+// per lane (amdgpu_waves_per_eu; 1 leaves it free).  Left free, a Jacobi
+// tile of 8 cells per lane takes 62 VGPRs and 106 SGPRs, which caps it at 7
+// waves per SIMD, 3 blocks of 8 waves per CU; asked for 8, the compiler
+// keeps the SGPRs that do not fit in VGPR lanes and a 4th block fits.  A
+// Chebyshev tile of 8 cells per lane takes 83 VGPRs (5 waves, 2 blocks of 8
+// waves); asked for 6 it fits in 80 with one spilled VGPR, and a 3rd block
+// fits.  Its residual variant and the larger tiles would spill heavily, so
+// they stay free.
+template <int CELLS>
+constexpr int jacobi_waves_per_eu() { return CELLS <= 8 ? 8 : 1; }
+template <int CELLS, bool RES>
+constexpr int cheb_waves_per_eu() { return CELLS == 8 && !RES ? 6 : 1; }
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;

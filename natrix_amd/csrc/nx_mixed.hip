@@ -18,16 +18,16 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
-inline dim3 grid2d(int W, int H) {
-  return dim3((W + BX - 1) / BX, (H + BY - 1) / BY, 1);
-}
+// blocks of BX x BY cells in XCD order (cell_grid / cell_of)
+inline dim3 grid2d(int W, int H) { return cell_grid(W, H); }
 
 template <class S>
 __global__ void __launch_bounds__(256)
     km_advect(const typename S::s2* __restrict__ uin, typename S::s2* __restrict__ uout,
               const uint8_t* __restrict__ mask, int W, int H, float dt,
               float speed, float diss, BcParams bc) {
-  const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+  int x, y;
+  cell_of(x, y, W, H);
   if (x >= W || y >= H) return;
   S::st2(uout, y * W + x,
          advect_cell<S>(uin, mask, x, y, W, H, dt, speed, diss, bc));
@@ -40,21 +40,25 @@ __global__ void __launch_bounds__(256)
              typename S::s1* __restrict__ div, typename S::s1* __restrict__ p1,
              int W, int H, float dt, float scale, int viscous, float alpha,
              float rbeta) {
-  fused::fused_vort_div_tile<S>((int)blockIdx.x * fused::TX,
-                                (int)blockIdx.y * fused::TY, uin, mask, uout,
-                                div, p1, W, H, dt, scale, viscous, alpha,
-                                rbeta);
+  const int gx = (W + fused::TX - 1) / fused::TX;
+  const int t = xcd_tile((int)blockIdx.x, gx * ((H + fused::TY - 1) / fused::TY));
+  fused::fused_vort_div_tile<S>((t % gx) * fused::TX, (t / gx) * fused::TY, uin, mask, uout,
+                                div, p1, W, H, dt, scale, viscous, alpha, rbeta);
 }
 
 template <int ROWS, int WAVES, int COLS, class S, bool CORR>
 __global__ void __launch_bounds__(WAVES * 64)
+    __attribute__((amdgpu_waves_per_eu(jt::jacobi_waves_per_eu<ROWS * COLS>())))
     km_jacobi(const typename S::s1* __restrict__ pin,
               const typename S::s1* __restrict__ b,
               const uint8_t* __restrict__ mask, typename S::s1* __restrict__ pout,
               float* __restrict__ pout32, int W, int H, int k) {
-  jt::jacobi_tb_tile<ROWS, WAVES, COLS, S, CORR>(
-      (int)blockIdx.x, (int)blockIdx.y, pin, b, mask, pout, W, H, k, nullptr,
-      0, pout32);
+  // 1D grid of gx * gy tiles in XCD order
+  const int gx = (W + 64 * COLS - 2 * k - 1) / (64 * COLS - 2 * k);
+  const int gy = (H + ROWS * WAVES - 2 * k - 1) / (ROWS * WAVES - 2 * k);
+  const int t = xcd_tile((int)blockIdx.x, gx * gy);
+  jt::jacobi_tb_tile<ROWS, WAVES, COLS, S, CORR>(t % gx, t / gx, pin, b, mask, pout, W, H,
+                                                 k, nullptr, 0, pout32);
 }
 
 template <class S>
@@ -62,7 +66,8 @@ __global__ void __launch_bounds__(256)
     km_subgrad(const typename S::s2* __restrict__ uin, const float* __restrict__ p,
                const uint8_t* __restrict__ mask, typename S::s2* __restrict__ uout,
                uint8_t* __restrict__ mask_next, int W, int H) {
-  const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+  int x, y;
+  cell_of(x, y, W, H);
   if (x >= W || y >= H) return;
   const int pos = y * W + x;
   S::st2(uout, pos, subgrad_cell<S>(uin, p, mask, x, y, W, H));
@@ -76,7 +81,8 @@ __global__ void __launch_bounds__(256)
                   const typename S::s2* __restrict__ u,
                   const uint8_t* __restrict__ mask, int Wp, int Hp, int Wv,
                   int Hv, float dt, float speed, float diss) {
-  const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+  int x, y;
+  cell_of(x, y, Wp, Hp);
   if (x >= Wp || y >= Hp) return;
   S::st(qout, y * Wp + x,
         advect_dye_cell<S>(qin, u, mask, x, y, Wp, Hp, Wv, Hv, dt, speed, diss));
@@ -91,7 +97,8 @@ template <class S>
 __global__ void __launch_bounds__(256)
     km_splats(typename S::s2* __restrict__ u, typename S::s1* __restrict__ q,
               int W, int H, SplatBatchM sb) {
-  const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+  int x, y;
+  cell_of(x, y, W, H);
   if (x >= W || y >= H) return;
   const int pos = y * W + x;
   float2 v = S::ld2(u, pos);
@@ -121,7 +128,8 @@ hipError_t mixed_tb(const void* pin, const void* b, const uint8_t* mask,
                     void* pout, float* pout32, int W, int H, int k, bool corr,
                     hipStream_t s) {
   constexpr int EX = 64 * CL, EY = R * WV;
-  dim3 grid((W + EX - 2 * k - 1) / (EX - 2 * k), (H + EY - 2 * k - 1) / (EY - 2 * k), 1);
+  const dim3 grid(((W + EX - 2 * k - 1) / (EX - 2 * k)) * ((H + EY - 2 * k - 1) / (EY - 2 * k)),
+                  1, 1);
   if (corr)
     km_jacobi<R, WV, CL, S, true><<<grid, WV * 64, 0, s>>>(
         (const typename S::s1*)pin, (const typename S::s1*)b, mask,
@@ -172,7 +180,7 @@ template <class S>
 static hipError_t fused_m(const void* uin, const uint8_t* mask, void* uout, void* div,
                           void* p1, int W, int H, float dt, float scale,
                           int viscous, float alpha, float rbeta, hipStream_t s) {
-  dim3 grid((W + fused::TX - 1) / fused::TX, (H + fused::TY - 1) / fused::TY, 1);
+  const dim3 grid(((W + fused::TX - 1) / fused::TX) * ((H + fused::TY - 1) / fused::TY), 1, 1);
   km_fused<S><<<grid, dim3(64, 4), 0, s>>>(
       (const typename S::s2*)uin, mask, (typename S::s2*)uout,
       (typename S::s1*)div, (typename S::s1*)p1, W, H, dt, scale, viscous, alpha,

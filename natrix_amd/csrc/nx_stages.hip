@@ -13,14 +13,11 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
-inline dim3 grid2d(int W, int H) {
-  return dim3((W + BX - 1) / BX, (H + BY - 1) / BY, 1);
-}
+// blocks of BX x BY cells in XCD order (cell_grid / cell_of)
+inline dim3 grid2d(int W, int H) { return cell_grid(W, H); }
 
 __device__ __forceinline__ bool cell(int& x, int& y, int W, int H) {
-  x = blockIdx.x * BX + threadIdx.x;
-  y = blockIdx.y * BY + threadIdx.y;
-  return x < W && y < H;
+  return cell_of(x, y, W, H);
 }
 
 // ---------------------------------------------------------------- S1
