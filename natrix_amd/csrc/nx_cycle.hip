@@ -364,121 +364,150 @@ __global__ void __launch_bounds__(256)
   pout[pos] = p + w * (jac - p);
 }
 
-// The Chebyshev smoothing of k_coarse_smooth<true> spread over row bands,
-// one workgroup per band of CB_BAND rows: a launch runs CB_T iterations
-// (plus, at the first launch, the initial residual) on its band extended by
-// CB_T + 1 rows on each side, whose outer rows go stale one row per stencil,
-// and stores the band's x, r, d.  Launch boundaries are the only global
-// synchronisation.  Every cell sees the operations of the single-workgroup
-// kernel in the same order; theta and the coefficients of the launch's
-// iterations (cf) come from the host, computed by the same float32
-// operations (IEEE single on both sides).
-constexpr int CB_ROWS = 2, CB_EY = CB_ROWS * CJ_WAVES, CB_T = 8;
-constexpr int CB_BAND = CB_EY - 2 * (CB_T + 1);
+// The Chebyshev smoothing of k_coarse_smooth<true> spread over 2D tiles:
+// a workgroup of CT_WAVES waves holds a 64 x CT_EY tile (lane l column l,
+// wave w rows CT_ROWS w .. CT_ROWS w + CT_ROWS - 1), and a launch runs up
+// to CB_T iterations on it (plus, at the first launch, the initial
+// residual).  A tile overlaps its neighbours by CT_HALO = CB_T + 1 cells on
+// every side, whose values go stale one cell per stencil, and stores its
+// CT_VX x CT_VY interior: at 128 x 128, 4 x 22 workgroups of 1024 threads,
+// 2 cells per lane (the shape and CB_T timed fastest on the 2048^2 learned
+// step among 8 to 16 iterations per launch and 4 to 16 waves).  Launch
+// boundaries are the only global synchronisation.  Every cell sees the
+// operations of the single-workgroup kernel in the same order, so x is
+// bitwise the same; wavefronts whose cells have no solid or out-of-grid
+// neighbour (the inner tiles away from obstacles) take a select-free
+// stencil with the same sums.  Theta and the
+// coefficients of the launch's iterations (cf) come from the host, computed
+// by the same float32 operations (IEEE single on both sides).
+constexpr int CB_T = 12, CT_ROWS = 2, CT_WAVES = 16, CT_HALO = CB_T + 1;
+constexpr int CT_EY = CT_ROWS * CT_WAVES, CT_VX = 64 - 2 * CT_HALO, CT_VY = CT_EY - 2 * CT_HALO;
+static_assert(CT_VX > 0 && CT_VY > 0, "coarse tile smaller than its halo");
 
-__global__ void __launch_bounds__(1024)
-    k_coarse_cheb_band(const float* __restrict__ xin, const float* __restrict__ rin,
+__global__ void __launch_bounds__(64 * CT_WAVES)
+    k_coarse_cheb_tile(const float* __restrict__ xin, const float* __restrict__ rin,
                        const float* __restrict__ din, float* __restrict__ xout,
                        float* __restrict__ rout, float* __restrict__ dout,
                        const float* __restrict__ rhs, const uint8_t* __restrict__ mc,
                        int Wc, int Hc, int s0, int T, float theta, ChebCoef cf) {
-  __shared__ float top[2][CJ_WAVES][CJ_MAX], bot[2][CJ_WAVES][CJ_MAX];
-  CoarseTile<CB_ROWS> t;
-  t.top = top;
-  t.bot = bot;
-  t.lane = threadIdx.x & 63;
-  t.wave = threadIdx.x >> 6;
-  t.x0 = 2 * t.lane;
-  t.buf = 0;
-  const int band0 = blockIdx.x * CB_BAND;
-  const int y0 = band0 - (CB_T + 1) + CB_ROWS * t.wave;
+  constexpr int R = CT_ROWS;
+  const int lane = threadIdx.x & 63;
+  // wave-uniform: a scalar register, so branches on it are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gx = (int)blockIdx.x * CT_VX - CT_HALO + lane;
+  const int gy0 = (int)blockIdx.y * CT_VY - CT_HALO + wave * R;
   const bool first = s0 == 0;
   // Every load is unconditional, from a clamped in-grid address, issued
   // before the first use: x, r (b at the first launch) and d of the lane's
-  // cells, and the mask of its two columns over the rows y0 - 1 .. y0 +
-  // CB_ROWS; the neighbours' flags to the left and right come from the
-  // adjacent lanes, and a neighbour outside the grid counts as solid.
+  // cells, and the mask of its column over the rows gy0 - 1 .. gy0 + R; the
+  // neighbours' flags to the left and right come from the adjacent lanes,
+  // and a cell outside the grid counts as solid.
   const float* rsrc = first ? rhs : rin;
-  float lx[CB_ROWS][2], lr[CB_ROWS][2], ld[CB_ROWS][2];
-  uint8_t lm[CB_ROWS + 2][2];
+  const int cx = min(max(gx, 0), Wc - 1);
+  float lx[R], lr[R], ld[R];
+  uint8_t lm[R + 2];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int cx = min(t.x0 + c, Wc - 1);
-#pragma unroll
-    for (int i = 0; i < CB_ROWS; ++i) {
-      const int pos = min(max(y0 + i, 0), Hc - 1) * Wc + cx;
-      lx[i][c] = xin[pos];
-      lr[i][c] = rsrc[pos];
-      ld[i][c] = din[pos];
-    }
-#pragma unroll
-    for (int i = 0; i < CB_ROWS + 2; ++i) lm[i][c] = mc[min(max(y0 - 1 + i, 0), Hc - 1) * Wc + cx];
+  for (int i = 0; i < R; ++i) {
+    const int pos = min(max(gy0 + i, 0), Hc - 1) * Wc + cx;
+    lx[i] = xin[pos];
+    lr[i] = rsrc[pos];
+    ld[i] = din[pos];
   }
-  // solid flags of the lane's two columns on rows y0 - 1 + i
-  int sf[CB_ROWS + 2][2];
 #pragma unroll
-  for (int i = 0; i < CB_ROWS + 2; ++i)
+  for (int i = 0; i < R + 2; ++i) lm[i] = mc[min(max(gy0 - 1 + i, 0), Hc - 1) * Wc + cx];
+  const bool col_in = gx >= 0 && gx < Wc;
+  int sf[R + 2];  // solid flags of rows gy0 - 1 + i
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int gy = y0 - 1 + i;
-      sf[i][c] = (t.x0 + c < Wc && gy >= 0 && gy < Hc) ? lm[i][c] != 0 : 1;
-    }
-  float x[CB_ROWS][2], r[CB_ROWS][2], d[CB_ROWS][2], lv[CB_ROWS][2];
-#pragma unroll
-  for (int i = 0; i < CB_ROWS; ++i) {
-    const int gy = y0 + i;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const bool in = t.x0 + c < Wc && gy >= 0 && gy < Hc;
-      x[i][c] = in ? lx[i][c] : 0.0f;
-      r[i][c] = in ? lr[i][c] : 0.0f;
-      d[i][c] = (in && !first) ? ld[i][c] : 0.0f;
-    }
-    // column x0 - 1 is the left lane's second column, x0 + 2 the right
-    // lane's first; beyond the grid's first column and past its last: solid.
-    // The lane shifts run with every lane active (a lane shift under a
-    // branch would read disabled lanes), the selects come after.
-    const int lnb = jt::ilane_from_left(sf[i + 1][1]);
-    const int rnb = jt::ilane_from_right(sf[i + 1][0]);
-    const int left = t.x0 == 0 ? 1 : lnb;
-    const int right = t.x0 + 2 >= Wc ? 1 : rnb;
-    t.code[i][0] = left | (sf[i + 1][1] << 1) | (sf[i][0] << 2) | (sf[i + 2][0] << 3);
-    t.code[i][1] = sf[i + 1][0] | (right << 1) | (sf[i][1] << 2) | (sf[i + 2][1] << 3);
+  for (int i = 0; i < R + 2; ++i) {
+    const int gy = gy0 - 1 + i;
+    sf[i] = (col_in && gy >= 0 && gy < Hc) ? lm[i] != 0 : 1;
   }
-  if (first) {
-    t.lap(x, lv);
+  float x[R], r[R], d[R], lv[R];
+  int code[R];  // bit 0: left, 1: right, 2: up (y-1), 3: down (y+1) solid
+  bool any = false;
 #pragma unroll
-    for (int i = 0; i < CB_ROWS; ++i)
+  for (int i = 0; i < R; ++i) {
+    const int gy = gy0 + i;
+    const bool in = col_in && gy >= 0 && gy < Hc;
+    x[i] = in ? lx[i] : 0.0f;
+    r[i] = in ? lr[i] : 0.0f;
+    d[i] = (in && !first) ? ld[i] : 0.0f;
+    // the lane shifts run with every lane active; lanes 0 and 63 keep their
+    // own flag, which only affects halo cells
+    const int lnb = jt::ilane_from_left(sf[i + 1]);
+    const int rnb = jt::ilane_from_right(sf[i + 1]);
+    code[i] = lnb | (rnb << 1) | (sf[i] << 2) | (sf[i + 2] << 3);
+    any |= code[i] != 0;
+  }
+  const bool special = __any(any);  // wave-uniform
+
+  __shared__ float top[2][CT_WAVES][64], bot[2][CT_WAVES][64];
+  int buf = 0;
+  // out = L v: ((l + r) + u) + d - 4 c, a solid or out-of-grid neighbour
+  // replaced by the centre; every thread of the block calls it
+  auto lap = [&](const float (&v)[R], float (&out)[R]) {
+    top[buf][wave][lane] = v[0];
+    bot[buf][wave][lane] = v[R - 1];
+    __syncthreads();
+    // the rows beyond the tile's first and last are halo: any finite value
+    const float up0 = bot[buf][wave > 0 ? wave - 1 : wave][lane];
+    const float dn0 = top[buf][wave < CT_WAVES - 1 ? wave + 1 : wave][lane];
+    buf ^= 1;
+    if (special) {
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        r[i][c] = lv[i][c] - r[i][c];
-        d[i][c] = r[i][c] / theta;
+      for (int i = 0; i < R; ++i) {
+        const float c = v[i];
+        const float fl = jt::lane_from_left(c), fr = jt::lane_from_right(c);
+        const float U = i > 0 ? v[i - 1] : up0;
+        const float D = i < R - 1 ? v[i + 1] : dn0;
+        const int cd = code[i];
+        const float l = (cd & 1) ? c : fl;
+        const float rr = (cd & 2) ? c : fr;
+        const float u = (cd & 4) ? c : U;
+        const float dd = (cd & 8) ? c : D;
+        out[i] = __builtin_fmaf(-4.0f, c, ((l + rr) + u) + dd);
       }
+    } else {
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const float c = v[i];
+        const float U = i > 0 ? v[i - 1] : up0;
+        const float D = i < R - 1 ? v[i + 1] : dn0;
+        // zero-filled shifts at lanes 0 and 63 (halo cells), folded into
+        // the additions as DPP operands
+        out[i] = __builtin_fmaf(-4.0f, c, ((jt::lane_shr0(c) + jt::lane_shl0(c)) + U) + D);
+      }
+    }
+  };
+  if (first) {
+    lap(x, lv);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      r[i] = lv[i] - r[i];
+      d[i] = r[i] / theta;
+    }
   }
   for (int it = 0; it < T; ++it) {
-    t.lap(d, lv);
+    lap(d, lv);
     const float cd = cf.cd[it], cr = cf.cr[it];
 #pragma unroll
-    for (int i = 0; i < CB_ROWS; ++i)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        x[i][c] = x[i][c] + d[i][c];
-        r[i][c] = r[i][c] + lv[i][c];
-        d[i][c] = cd * d[i][c] + cr * r[i][c];
-      }
-  }
-#pragma unroll
-  for (int i = 0; i < CB_ROWS; ++i)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int gx = t.x0 + c, gy = y0 + i;
-      if (gx < Wc && gy >= band0 && gy < band0 + CB_BAND && gy < Hc) {
-        const int pos = gy * Wc + gx;
-        xout[pos] = x[i][c];
-        rout[pos] = r[i][c];
-        dout[pos] = d[i][c];
-      }
+    for (int i = 0; i < R; ++i) {
+      x[i] = x[i] + d[i];
+      r[i] = r[i] + lv[i];
+      d[i] = cd * d[i] + cr * r[i];
     }
+  }
+  const bool col_out = lane >= CT_HALO && lane < 64 - CT_HALO && gx < Wc;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int ty = wave * R + i, gy = gy0 + i;
+    if (col_out && ty >= CT_HALO && ty < CT_EY - CT_HALO && gy < Hc) {
+      const int pos = gy * Wc + gx;
+      xout[pos] = x[i];
+      rout[pos] = r[i];
+      dout[pos] = d[i];
+    }
+  }
 }
 
 namespace nx {
@@ -486,12 +515,12 @@ int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* 
                          const uint8_t* mc, int iterations, float lambda_min, float* ws) {
   if (iterations == 0) return NX_OK;
   if (lambda_min > 0.0f && iterations > CB_T) {
-    // banded: x, r, d ping-pong between the two halves of ws (6 Wc Hc
+    // tiled: x, r, d ping-pong between the two halves of ws (6 Wc Hc
     // floats); the last launch writes x to e (read only by the first)
     if (!ws) return set_error(NX_ERR_ARG, "coarse smoother workspace missing");
     const size_t n = (size_t)Wc * Hc;
     float* set[2][3] = {{ws, ws + n, ws + 2 * n}, {ws + 3 * n, ws + 4 * n, ws + 5 * n}};
-    const int nb = (Hc + CB_BAND - 1) / CB_BAND;
+    const dim3 grid((Wc + CT_VX - 1) / CT_VX, (Hc + CT_VY - 1) / CT_VY);
     ChebRecurrence rec(lambda_min);
     int done = 0, cur = 0;
     while (done < iterations) {
@@ -501,9 +530,10 @@ int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* 
       float** out = set[done == 0 ? 0 : 1 - cur];
       ChebCoef cf{};
       rec.next(0, T, cf);
-      k_coarse_cheb_band<<<nb, 1024, 0, st>>>(xin, set[cur][1], set[cur][2],
-                                              last ? e : out[0], out[1], out[2], rc, mc,
-                                              Wc, Hc, done, T, rec.theta, cf);
+      k_coarse_cheb_tile<<<grid, 64 * CT_WAVES, 0, st>>>(xin, set[cur][1], set[cur][2],
+                                                         last ? e : out[0], out[1], out[2],
+                                                         rc, mc, Wc, Hc, done, T, rec.theta,
+                                                         cf);
       if (hipGetLastError() != hipSuccess)
         return set_error(NX_ERR_HIP, "coarse smoother launch failed");
       cur = done == 0 ? 0 : 1 - cur;

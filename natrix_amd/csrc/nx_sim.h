@@ -76,7 +76,7 @@ struct nx_sim {
   float* b0 = nullptr;
   float* dcorr = nullptr;   // prolonged coarse correction (line search)
   float* rrow = nullptr;    // restriction row records (2 x H x Wc)
-  float* cws = nullptr;     // banded coarse smoother: x, r, d twice
+  float* cws = nullptr;     // tiled coarse smoother: x, r, d twice
   float* fcheb = nullptr;   // fine Chebyshev smoother: r, d twice (4 W H)
   double* cheb_red = nullptr;  // its residual records, 5 per block
   size_t cheb_red_cap = 0;

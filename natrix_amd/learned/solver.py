@@ -12,7 +12,7 @@ One learned cycle:
      the coarse problem is in coarse grid units) and scale it to unit RMS;
   2. run the U-Net and scale back: the coarse correction's first guess;
   3. Chebyshev-accelerated Jacobi on the coarse problem from that guess
-     (banded over a few workgroups, 8 iterations per launch): removes the
+     (tiled over 88 workgroups, 12 iterations per launch): removes the
      network's error at short coarse wavelengths, where its relative error
      is largest and which the fine sweeps would need tens of thousands of
      sweeps to undo;

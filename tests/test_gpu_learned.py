@@ -40,12 +40,12 @@ def dev(sim, name):
 
 
 @pytest.mark.parametrize("Wc,Hc", [(128, 128), (100, 72), (7, 128), (128, 13)])
-@pytest.mark.parametrize("cheb,iters", [(False, 25), (True, 8), (True, 9), (True, 40),
+@pytest.mark.parametrize("cheb,iters", [(False, 25), (True, 12), (True, 13), (True, 40),
                                         (True, 100)])
 def test_coarse_smoother_matches_float64_statement(gpu, rng, Wc, Hc, cheb, iters):
-    """Chebyshev with more than 8 iterations runs banded (10 workgroups of
-    14 rows, 8 iterations per launch, a partial last launch for 9 and 100);
-    8 or fewer run in one resident workgroup."""
+    """Chebyshev with more than 12 iterations runs tiled (64 x 32 tiles with
+    a 13-cell halo, 12 iterations per launch, a partial last launch for 13,
+    40 and 100); 12 or fewer run in one resident workgroup."""
     from natrix_amd.learned.solver import chebyshev_lmin
     sim = in_step_sim(rng, 64, 64)
     solid = rng.random((Hc, Wc)) < 0.1
