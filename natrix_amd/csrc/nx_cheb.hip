@@ -7,6 +7,7 @@
 // reads the centre value):
 //   setup:      r = L x - b,  d = r / theta
 //   iteration:  x += d,  r += L d,  d = (rho' rho) d + (2 rho' / delta) r
+//               (one fused multiply-add: fma(cd, d, cr r))
 // with theta = (8 + lmin) / 2, delta = (8 - lmin) / 2, sigma = theta /
 // delta, rho_0 = 1 / sigma, rho' = 1 / (2 sigma - rho) (the scalar
 // recurrence runs on the host; a launch takes its passes' coefficients).
@@ -24,10 +25,12 @@
 // k-pass dependency cone stayed inside the tile is written back.  The first
 // launch of a run spends its first pass on the setup, the last one (RES)
 // its last pass on the FP64 residual of the result.  Every pass evaluates
-// each cell's expressions in a fixed order without contraction (the -4 v
-// of L v is one fused multiply-add, which rounds like the subtraction
-// because 4 v is exact), so the result is bitwise independent of the tile
-// shape and of k.
+// each cell's expressions in a fixed order: L v = (((u + l) + r) + d) - 4 v
+// with u, d the cells above and below and l, r the lane-shifted ones, so
+// that each shift is the DPP operand of an addition (no separate move); the
+// -4 v is one fused multiply-add, which rounds like the subtraction because
+// 4 v is exact; and the d update is the one fused multiply-add written
+// above.  The result is bitwise independent of the tile shape and of k.
 #include <type_traits>
 
 #include "../../include/natrix.h"
@@ -242,12 +245,12 @@ __global__ void __launch_bounds__(WAVES * 64)
         }
       }
     };
-    // lv = L v = ((((x1 + x2) + y1) + y2) - 4 v): 4 v is exact, so the
+    // lv = L v = ((((y1 + x1) + x2) + y2) - 4 v): 4 v is exact, so the
     // fused multiply-add rounds once, like the subtraction
     float lv[ROWS][COLS];
     auto lap = [&](const float (&v)[ROWS][COLS], int buf) {
       stencil(v, buf, [&](int i, int c, float vc, float x1, float x2, float y1, float y2) {
-        lv[i][c] = __builtin_fmaf(-4.0f, vc, ((x1 + x2) + y1) + y2);
+        lv[i][c] = __builtin_fmaf(-4.0f, vc, ((y1 + x1) + x2) + y2);
       });
     };
 
@@ -263,18 +266,26 @@ __global__ void __launch_bounds__(WAVES * 64)
       s = 1;
     }
     const int iter_end = RES ? k - 1 : k;
-    for (; s < iter_end; ++s) {
-      lap(d, s & 1);
-      const float cdd = cf.cd[s], crr = cf.cr[s];
+    auto iterate = [&](int it) {
+      lap(d, it & 1);
+      const float cdd = cf.cd[it], crr = cf.cr[it];
 #pragma unroll
       for (int i = 0; i < ROWS; ++i)
 #pragma unroll
         for (int c = 0; c < COLS; ++c) {
           x[i][c] = x[i][c] + d[i][c];
           r[i][c] = r[i][c] + lv[i][c];
-          d[i][c] = cdd * d[i][c] + crr * r[i][c];
+          d[i][c] = __builtin_fmaf(cdd, d[i][c], crr * r[i][c]);
         }
+    };
+    // two iterations per trip: the fused multiply-add leaves the new d in
+    // the product's register, and with two copies of the body the register
+    // allocator alternates the roles instead of copying d back every pass
+    for (; s + 1 < iter_end; s += 2) {
+      iterate(s);
+      iterate(s + 1);
     }
+    if (s < iter_end) iterate(s++);
   };
   if (special)
     run(std::true_type());

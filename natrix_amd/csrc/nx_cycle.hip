@@ -230,7 +230,10 @@ struct CoarseTile {
   int code[NR][2];  // bit 0: left, 1: right, 2: up (y-1), 3: down (y+1)
   int lane, wave, x0, buf;
 
-  // out = L v for this lane's cells (all threads of the block call it)
+  // out = L v for this lane's cells (all threads of the block call it),
+  // summed ((l + r) + u) + d - 4 c (the Jacobi sweep's order) or, CHEB,
+  // ((u + l) + r) + d - 4 c (the Chebyshev kernels' order)
+  template <bool CHEB = false>
   __device__ __forceinline__ void lap(const float (&v)[NR][2],
                                       float (&out)[NR][2]) {
     top[buf][wave][x0] = v[0][0];
@@ -260,7 +263,7 @@ struct CoarseTile {
         R = (cd & 2) ? ctr : R;
         U = (cd & 4) ? ctr : U;
         D = (cd & 8) ? ctr : D;
-        out[i][c] = (((L + R) + U) + D) - 4.0f * ctr;
+        out[i][c] = CHEB ? (((U + L) + R) + D) - 4.0f * ctr : (((L + R) + U) + D) - 4.0f * ctr;
       }
     buf ^= 1;
   }
@@ -308,7 +311,7 @@ __global__ void __launch_bounds__(1024)
     const float sigma = theta / delta;
     float rho = 1.0f / sigma;
     float d[CJ_ROWS][2];
-    t.lap(x, lv);
+    t.lap<true>(x, lv);
 #pragma unroll
     for (int i = 0; i < CJ_ROWS; ++i)
 #pragma unroll
@@ -317,7 +320,7 @@ __global__ void __launch_bounds__(1024)
         d[i][c] = r[i][c] / theta;
       }
     for (int s = 0; s < iters; ++s) {
-      t.lap(d, lv);
+      t.lap<true>(d, lv);
       const float rho1 = 1.0f / (2.0f * sigma - rho);
       const float cd = rho1 * rho, cr = 2.0f * rho1 / delta;
 #pragma unroll
@@ -326,7 +329,7 @@ __global__ void __launch_bounds__(1024)
         for (int c = 0; c < 2; ++c) {
           x[i][c] = x[i][c] + d[i][c];
           r[i][c] = r[i][c] + lv[i][c];  // q -= A d
-          d[i][c] = cd * d[i][c] + cr * r[i][c];
+          d[i][c] = __builtin_fmaf(cd, d[i][c], cr * r[i][c]);
         }
       rho = rho1;
     }
@@ -443,7 +446,7 @@ __global__ void __launch_bounds__(64 * CT_WAVES)
 
   __shared__ float top[2][CT_WAVES][64], bot[2][CT_WAVES][64];
   int buf = 0;
-  // out = L v: ((l + r) + u) + d - 4 c, a solid or out-of-grid neighbour
+  // out = L v: ((u + l) + r) + d - 4 c, a solid or out-of-grid neighbour
   // replaced by the centre; every thread of the block calls it
   auto lap = [&](const float (&v)[R], float (&out)[R]) {
     top[buf][wave][lane] = v[0];
@@ -465,7 +468,7 @@ __global__ void __launch_bounds__(64 * CT_WAVES)
         const float rr = (cd & 2) ? c : fr;
         const float u = (cd & 4) ? c : U;
         const float dd = (cd & 8) ? c : D;
-        out[i] = __builtin_fmaf(-4.0f, c, ((l + rr) + u) + dd);
+        out[i] = __builtin_fmaf(-4.0f, c, ((u + l) + rr) + dd);
       }
     } else {
 #pragma unroll
@@ -475,7 +478,7 @@ __global__ void __launch_bounds__(64 * CT_WAVES)
         const float D = i < R - 1 ? v[i + 1] : dn0;
         // zero-filled shifts at lanes 0 and 63 (halo cells), folded into
         // the additions as DPP operands
-        out[i] = __builtin_fmaf(-4.0f, c, ((jt::lane_shr0(c) + jt::lane_shl0(c)) + U) + D);
+        out[i] = __builtin_fmaf(-4.0f, c, ((U + jt::lane_shr0(c)) + jt::lane_shl0(c)) + D);
       }
     }
   };
@@ -494,7 +497,7 @@ __global__ void __launch_bounds__(64 * CT_WAVES)
     for (int i = 0; i < R; ++i) {
       x[i] = x[i] + d[i];
       r[i] = r[i] + lv[i];
-      d[i] = cd * d[i] + cr * r[i];
+      d[i] = __builtin_fmaf(cd, d[i], cr * r[i]);
     }
   }
   const bool col_out = lane >= CT_HALO && lane < 64 - CT_HALO && gx < Wc;

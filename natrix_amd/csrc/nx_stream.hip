@@ -133,17 +133,17 @@ __device__ __forceinline__ void cheb_pass(float (&X)[ROWS][COLS], float (&R)[ROW
       if (KIND == P_SETUP) {
         // 4 v is exact: the fused multiply-add rounds once, like the
         // subtraction of the product
-        const float lv = __builtin_fmaf(-4.0f, v, ((x1 + x2) + y1) + y2);
+        const float lv = __builtin_fmaf(-4.0f, v, ((y1 + x1) + x2) + y2);
         const float rn = lv - ac[c];  // r holds b before the setup
         X[i][c] = v;
         R[i][c] = rn;
         D[i][c] = rn / theta;
       } else if (KIND == P_ITER) {
-        const float lv = __builtin_fmaf(-4.0f, v, ((x1 + x2) + y1) + y2);
+        const float lv = __builtin_fmaf(-4.0f, v, ((y1 + x1) + x2) + y2);
         const float rn = bc[c] + lv;
         X[i][c] = ac[c] + v;
         R[i][c] = rn;
-        D[i][c] = cdd * v + crr * rn;
+        D[i][c] = __builtin_fmaf(cdd, v, crr * rn);
       } else {
         const int gy = row_new0 + i, gx = gx0 + c, tcol = tcol0 + c;
         if (gy >= Y0 && gy < Y1 && tcol >= k && tcol < EX - k && gx >= 0 && gx < W && !sc) {

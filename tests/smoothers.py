@@ -108,24 +108,59 @@ def jacobi32(x, rhs, solid, iters):
     return x
 
 
+def fma32(a, b, c):
+    """float32 fused multiply-add round(a b + c), elementwise: a b is exact
+    in float64 (24 + 24 significant bits), the float64 sum is rounded to odd
+    (its TwoSum error moves an even result one ulp towards the exact sum),
+    and rounding that to float32 is then correctly rounded (53 >= 24 + 2
+    bits; Boldo and Melquiond, "Emulation of FMA and correctly rounded sums:
+    proved algorithms using rounding to odd", IEEE Trans. Comput. 2008)."""
+    p = np.asarray(a, np.float64) * np.asarray(b, np.float64)
+    c = np.asarray(c, np.float64)
+    s = p + c
+    t = s - p
+    e = (p - (s - t)) + (c - t)
+    even = (s.view(np.int64) & 1) == 0
+    odd = (e != 0) & even & np.isfinite(s)
+    s = np.where(odd, np.nextafter(s, np.where(e > 0, np.inf, -np.inf)), s)
+    return s.astype(np.float32)
+
+
+def lapc32(v, solid):
+    """float32 L v in the Chebyshev kernels' summation order
+    (((u + l) + r) + d) - 4c: both lane-shifted neighbours enter as the
+    shifted operand of an addition."""
+    f32 = np.float32
+    H, W = v.shape
+    P = np.pad(v, 1, mode="edge")
+    S = np.pad(solid, 1, mode="constant", constant_values=True)
+    nb = []
+    for dy, dx in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+        n = P[1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+        s = S[1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+        nb.append(np.where(s, v, n))
+    return (((nb[2] + nb[0]) + nb[1]) + nb[3]) - f32(4.0) * v
+
+
 def cheb32(x, rhs, solid, iters, lmin):
-    """The coarse Chebyshev iteration's float32 operations, in the kernels'
-    order (no fused multiply-add)."""
+    """The fine and coarse Chebyshev iterations' float32 operations, in the
+    kernels' order: L v as lapc32 (the -4 v is one rounding, 4 v being
+    exact) and d <- fma(cd, d, cr r)."""
     f32 = np.float32
     lmin = f32(lmin)
     theta = f32(0.5) * (f32(8.0) + lmin)
     delta = f32(0.5) * (f32(8.0) - lmin)
     sigma = theta / delta
     rho = f32(1.0) / sigma
-    r = lap32(x, solid) - rhs
+    r = lapc32(x, solid) - rhs
     d = r / theta
     for _ in range(iters):
-        ld = lap32(d, solid)
+        ld = lapc32(d, solid)
         rho1 = f32(1.0) / (f32(2.0) * sigma - rho)
         cd = rho1 * rho
         cr = f32(2.0) * rho1 / delta
         x = x + d
         r = r + ld
-        d = cd * d + cr * r
+        d = fma32(cd, d, cr * r)
         rho = rho1
     return x

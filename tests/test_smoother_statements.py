@@ -8,7 +8,9 @@ T_k((theta - lam) / delta) / T_k(theta / delta) v."""
 import numpy as np
 import pytest
 
-from smoothers import cheb_ref, jacobi_ref, lap
+from fractions import Fraction
+
+from smoothers import cheb_ref, fma32, jacobi_ref, lap
 
 
 def mode(W, H, m, n):
@@ -85,3 +87,57 @@ def test_prolongation_reproduces_constant_and_linear_fields():
     mc[2, 3] = True
     out = prolong_ref(np.where(mc, 99.0, 2.5), mc, f, H, W, solid)
     assert np.allclose(out, 2.5)
+
+
+def round_f32(q):
+    """A rational rounded to float32, to nearest with ties to even."""
+    if q == 0:
+        return np.float32(0.0)
+    neg = q < 0
+    q = -q if neg else q
+    e = q.numerator.bit_length() - q.denominator.bit_length()
+    if Fraction(2) ** e > q:
+        e -= 1
+    if Fraction(2) ** (e + 1) <= q:
+        e += 1
+    e = max(e, -126)  # subnormals share the smallest exponent
+    scale = Fraction(2) ** (23 - e)
+    m = q * scale
+    n = m.numerator // m.denominator
+    rem = m - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    v = np.float32(float(Fraction(n) / scale))
+    return -v if neg else v
+
+
+def test_fma32_is_the_correctly_rounded_fused_multiply_add(rng):
+    """fma32 against exact rational arithmetic: random operands over many
+    binades, near-cancellations a b ~ -c, subnormal results, and products
+    that land exactly halfway between two float32 values, where a tiny c
+    decides the rounding (float64 evaluation rounds those twice and errs)."""
+    a = (rng.standard_normal(3000) * 2.0 ** rng.integers(-30, 30, 3000)).astype(np.float32)
+    b = (rng.standard_normal(3000) * 2.0 ** rng.integers(-30, 30, 3000)).astype(np.float32)
+    c = (rng.standard_normal(3000) * 2.0 ** rng.integers(-60, 60, 3000)).astype(np.float32)
+    c[:500] = (-(a[:500].astype(np.float64) * b[:500]) * (1 + 1e-7 * rng.standard_normal(500))).astype(np.float32)
+    a[500:600] = np.float32(2.0 ** -70)
+    b[500:600] = (rng.standard_normal(100) * 2.0 ** -70).astype(np.float32)
+    u = np.float32(1 + 2.0 ** -12)
+    ties = [(u, u, np.float32(s * 2.0 ** -80)) for s in (1, -1)] + \
+        [(u, u, np.float32(0.0)), (np.float32(1 + 2.0 ** -12), np.float32(-(1 + 2.0 ** -12)),
+                                   np.float32(1.0))]
+    a = np.concatenate([a, [t[0] for t in ties]]).astype(np.float32)
+    b = np.concatenate([b, [t[1] for t in ties]]).astype(np.float32)
+    c = np.concatenate([c, [t[2] for t in ties]]).astype(np.float32)
+    # the rounding reference itself: numpy's float64 -> float32 conversion
+    # is correctly rounded
+    xs = rng.standard_normal(2000) * 2.0 ** rng.integers(-140, 100, 2000)
+    assert np.array_equal(np.array([round_f32(Fraction(float(x))) for x in xs], np.float32),
+                          xs.astype(np.float32))
+    got = fma32(a, b, c)
+    want = np.array([round_f32(Fraction(float(x)) * Fraction(float(y)) + Fraction(float(z)))
+                     for x, y, z in zip(a, b, c)], np.float32)
+    assert np.array_equal(got, want)
+    # the tie cases are ones where rounding the float64 sum directly errs
+    naive = (a[-4:-2].astype(np.float64) * b[-4:-2] + c[-4:-2]).astype(np.float32)
+    assert not np.array_equal(naive, want[-4:-2])
