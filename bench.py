@@ -47,18 +47,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def profile_traffic(name, **match):
-    """HBM bytes per launch from the committed rocprofv3 summary
-    profiles/<name>.md (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the copy-probe
-    calibration), when every header line `key: value` named in `match`
-    equals it (tile as a comma list), else (None, None)."""
+def profile_header(name, **match):
+    """The `key: value` header lines of the committed rocprofv3 summary
+    profiles/<name>.md when every key named in `match` equals it (tile as a
+    comma list), else None."""
     path = os.path.join(ROOT, "profiles", f"{name}.md")
     if not os.path.exists(path):
-        return None, None
+        return None
     kv = {}
     for line in open(path):
         k, sep, v = line.partition(":")
-        if sep and (k.strip() in match or k.strip() == "traffic_bytes_per_launch"):
+        if sep and k.strip() and " " not in k.strip():
             kv.setdefault(k.strip(), v.strip())
     try:
         for k, want in match.items():
@@ -68,10 +67,41 @@ def profile_traffic(name, **match):
             else:
                 ok = type(want)(got) == want
             if not ok:
-                return None, None
-        return int(kv["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+                return None
     except (KeyError, ValueError):
+        return None
+    kv["_source"] = os.path.relpath(path, ROOT)
+    return kv
+
+
+def profile_traffic(name, **match):
+    """HBM bytes per launch from the committed rocprofv3 summary
+    profiles/<name>.md (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the copy-probe
+    calibration) when its header matches, else (None, None)."""
+    kv = profile_header(name, **match)
+    try:
+        return int(kv["traffic_bytes_per_launch"]), kv["_source"]
+    except (TypeError, KeyError, ValueError):
         return None, None
+
+
+def valu_entry(name, launch_us, **match):
+    """VALU issue figure of a launch from the committed summary's
+    SQ_INSTS_VALU count (wave instructions per launch): the time the chip
+    needs just to issue them, each wave64 VALU instruction taking 4 cycles
+    of one of the 1024 SIMDs (256 CUs x 4) at the 2.4 GHz peak engine
+    clock, as a fraction of the measured launch time; None without a
+    matching summary."""
+    kv = profile_header(name, **match)
+    try:
+        n = float(kv["valu_instructions_per_launch"])
+    except (TypeError, KeyError, ValueError):
+        return None
+    issue_us = n * 4 / (1024 * 2.4e9) * 1e6
+    return {"instructions_per_launch": int(n), "issue_bound_us": round(issue_us, 3),
+            "frac": round(issue_us / launch_us, 4),
+            "peak": "1024 SIMDs x 2.4 GHz, 4 cycles per wave64 VALU instruction",
+            "source": kv["_source"]}
 
 
 def committed_traffic(N, iters, tile, kernel="k_jacobi_tb", suffix="fixed"):
@@ -554,6 +584,8 @@ def bench_learned(args, copy_bandwidth):
     traffic_c, tsrc_c = committed_traffic(N, n_iter, tile, kname, "iters")
     roof_cheb = roofline_entry(
         kname, algo_call / len(plan), 1e3 * ch_ms / ch_n, traffic_c, tsrc_c,
+        valu=valu_entry(f"{kname}_{N}_iters{n_iter}", 1e3 * ch_ms / ch_n, grid=N,
+                        iterations=n_iter, tile=list(tile)),
         iterations_per_launch=round(n_iter / len(plan), 3),
         algorithmic_bytes_note="per call of 72 iterations: the first launch reads x, b, mask "
                                "and writes x, r, d (21 B/cell), the middle ones read x, r, d, "
@@ -585,6 +617,8 @@ def bench_learned(args, copy_bandwidth):
     traffic, tsrc = committed_traffic(N, args.iters, jtile, jname)
     roof = roofline_entry(
         jname, JACOBI_BYTES_PER_CELL_SWEEP * cells, 1e3 * jac_ms / jac_n, traffic, tsrc,
+        valu=valu_entry(f"{jname}_{N}_fixed{args.iters}", 1e3 * jac_ms / jac_n, grid=N,
+                        iterations=args.iters, tile=list(jtile)),
         sweeps_per_launch=round(tb_sweeps / jac_n, 3),
         algorithmic_bytes_note="13 B per cell per launch: read p, b, mask, write p",
         per_sweep_GBps=round(JACOBI_BYTES_PER_CELL_SWEEP * cells * tb_sweeps

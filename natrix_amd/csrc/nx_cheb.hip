@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(WAVES * 64)
   // 1D grid of gx * gy tiles in XCD order (xcd_tile); the partial-sum
   // record is the tile's row-major index
   const int gx = (W + VX - 1) / VX, gy = (H + VY - 1) / VY;
-  const int tile = xcd_tile((int)blockIdx.x, gx * gy);
+  const int tile = xcd_tile<XCD_RUNS>((int)blockIdx.x, gx * gy);
   const int gx0 = (tile % gx) * VX - k + COLS * lane;
   const int gy0 = (tile / gx) * VY - k + wave * ROWS;
   const int tcol0 = COLS * lane, trow0 = wave * ROWS;

@@ -227,24 +227,33 @@ struct Splat {  // Gaussian splat: u += (ax, ay) * w, dye += dye * w
 // The chip dispatches the workgroups of a launch to its 8 XCDs round-robin
 // (workgroup i to XCD i mod 8), and each XCD has its own L2.  Kernels whose
 // blocks read overlapping halos launch a 1D grid of T blocks and take block
-// (tile) xcd_tile(i, T, m) in workgroup i: the row-major tiles are cut into
-// 8 m runs of nearly equal length (the first T mod 8 m one tile longer) and
+// (tile) xcd_tile<M>(i, T) in workgroup i: the row-major tiles are cut into
+// 8 M runs of nearly equal length (the first T mod 8 M one tile longer) and
 // XCD x works through runs x, x + 8, x + 16, ... in order.  Neighbouring
 // tiles of a run then run on one XCD close together in time, and the rows
 // and columns they share are read from that XCD's L2 instead of from
-// memory; m > 1 spreads each XCD's tiles over the whole grid, so that a
+// memory; M > 1 spreads each XCD's tiles over the whole grid, so that a
 // region of costlier tiles (obstacle boundaries) is shared by all XCDs
 // instead of delaying one.  Each XCD gets as many tiles as the dispatch
-// gives it workgroups, so this is a bijection of [0, T) for every T and m.
-// XCD_RUNS = 4 timed fastest on the 2048^2 Chebyshev tile (41.3 us per
-// launch against 42.8 for one contiguous run per XCD and 45.9 for plain
-// round-robin order; 2 and 8 within 0.1 us of 4).
+// gives it workgroups, so this is a bijection of [0, T) for every T and M.
+// M = 1 takes shifts only; M > 1 (a power of two) two integer divisions
+// per workgroup, which the many-workgroup stage kernels leave out (their
+// cost showed at 65536 workgroups of 256 threads), while the tile kernels
+// use XCD_RUNS = 4, timed fastest on the 2048^2 Chebyshev tile (41.3 us per
+// launch against 42.8 for M = 1 and 45.9 for plain round-robin order; 2
+// and 8 within 0.1 us of 4).
 constexpr int XCD_RUNS = 4;
-__device__ __forceinline__ int xcd_tile(int i, int T, int m = XCD_RUNS) {
+template <int M = 1>
+__device__ __forceinline__ int xcd_tile(int i, int T) {
+  static_assert(M >= 1 && (M & (M - 1)) == 0, "runs per XCD must be a power of two");
   const int x = i & 7, j = i >> 3;
-  const int q = T / (8 * m), rem = T % (8 * m);
+  if (M == 1) {
+    const int q = T >> 3, r = T & 7;
+    return x * q + min(x, r) + j;
+  }
+  const int q = T / (8 * M), rem = T & (8 * M - 1);
   // runs x + 8 k with k < a have q + 1 tiles, the later ones q
-  const int a = min(m, max(0, (rem - x + 7) >> 3));
+  const int a = min(M, max(0, (rem - x + 7) >> 3));
   int k, off;
   if (j < a * (q + 1)) {
     k = j / (q + 1);

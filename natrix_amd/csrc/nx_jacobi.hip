@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(WAVES * 64)
   // tile's row-major index
   const int gx = (W + 64 * COLS - 2 * k - 1) / (64 * COLS - 2 * k);
   const int gy = (H + ROWS * WAVES - 2 * k - 1) / (ROWS * WAVES - 2 * k);
-  const int t = xcd_tile((int)blockIdx.x, gx * gy);
+  const int t = xcd_tile<XCD_RUNS>((int)blockIdx.x, gx * gy);
   jt::jacobi_tb_tile<ROWS, WAVES, COLS>(t % gx, t / gx, pin, b, mask, pout, W, H, k, red,
                                         t);
 }

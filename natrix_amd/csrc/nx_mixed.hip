@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(WAVES * 64)
   // 1D grid of gx * gy tiles in XCD order
   const int gx = (W + 64 * COLS - 2 * k - 1) / (64 * COLS - 2 * k);
   const int gy = (H + ROWS * WAVES - 2 * k - 1) / (ROWS * WAVES - 2 * k);
-  const int t = xcd_tile((int)blockIdx.x, gx * gy);
+  const int t = xcd_tile<XCD_RUNS>((int)blockIdx.x, gx * gy);
   jt::jacobi_tb_tile<ROWS, WAVES, COLS, S, CORR>(t % gx, t / gx, pin, b, mask, pout, W, H,
                                                  k, nullptr, 0, pout32);
 }
