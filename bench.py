@@ -495,9 +495,9 @@ def bench_learned(args, copy_bandwidth):
         sim.synchronize()
         return sim
 
-    def timed_learned(sim, steps):
+    def timed_learned(sim, steps, g=guess):
         for _ in range(2):  # first-call costs (allocator, code objects) off the clock
-            solve_step(sim, guess, dt, tol=args.tol, smooth=CHEB_SMOOTH)
+            solve_step(sim, g, dt, tol=args.tol, smooth=CHEB_SMOOTH)
         sim.synchronize()
         torch.cuda.synchronize()
         # no per-launch events in this timed region (they cost ~10 % of a
@@ -505,7 +505,7 @@ def bench_learned(args, copy_bandwidth):
         sweeps, outer, conv = [], [], []
         t0 = time.perf_counter()
         for _ in range(steps):
-            st = solve_step(sim, guess, dt, tol=args.tol, smooth=CHEB_SMOOTH)
+            st = solve_step(sim, g, dt, tol=args.tol, smooth=CHEB_SMOOTH)
             sweeps.append(st["sweeps"])
             outer.append(st["outer"])
             conv.append(st["converged"] and st["residual"] <= args.tol)
@@ -536,6 +536,20 @@ def bench_learned(args, copy_bandwidth):
         developed["flow"] = (f"steps {args.develop + 3}..{args.develop + 2 + args.developed_steps}"
                              f" ({args.develop} fixed-iteration steps of development)")
         log(f"bench learned, developed flow: {developed}")
+
+    classical = None
+    if args.classical_coarse > 0:
+        # the same two-grid cycle with the network off: the coarse problem
+        # is smoothed from zero with more Chebyshev iterations instead (the
+        # method without the learned guess, at its best measured setting)
+        g0 = LearnedGuess(net, coarse=meta.get("size", 128),
+                          coarse_iters=args.classical_coarse, use_network=False)
+        sim = fresh()
+        classical = timed_learned(sim, args.steps, g0)
+        sim.destroy()
+        classical["coarse_iterations"] = args.classical_coarse
+        classical["flow"] = learned["flow"]
+        log(f"bench two-grid without the network: {classical}")
 
     plain = None
     if not args.no_plain:
@@ -664,7 +678,8 @@ def bench_learned(args, copy_bandwidth):
                       "grid": N, "solver": "learned two-grid cycle (U-Net bf16 MFMA guess, "
                                            "Chebyshev coarse and fine smoothing), FP64 refinement",
                       "weights": os.path.basename(args.weights or DEFAULT_WEIGHTS)},
-           "learned": learned, "learned_developed_flow": developed, "plain": plain,
+           "learned": learned, "learned_developed_flow": developed,
+           "two_grid_without_network": classical, "plain": plain,
            "speedup_learned_vs_plain": round(speedup, 1) if speedup else None,
            "speedup_is_lower_bound": bool(plain and plain["rate_is_upper_bound"]),
            "plain_fixed_iterations": fixed, "copy_probe_GBps": round(copy_gbs, 1),
@@ -715,6 +730,9 @@ def main():
     ap.add_argument("--develop", type=int, default=500,
                     help="learned workload: also time learned steps after this many "
                          "fixed-iteration steps of development (0: skip)")
+    ap.add_argument("--classical-coarse", type=int, default=350,
+                    help="learned workload: also time the cycle without the network, "
+                         "with this many coarse Chebyshev iterations (0: skip)")
     ap.add_argument("--developed-steps", type=int, default=10,
                     help="learned workload: timed steps of the developed-flow leg")
     ap.add_argument("--fixed-steps", type=int, default=100,
