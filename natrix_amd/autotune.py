@@ -28,7 +28,7 @@ CACHE = os.path.join(HERE, "tuned.json")
 ARCH = "gfx950"
 
 # (rows, waves, cols) instantiated in csrc/nx_jacobi.hip (NX_TB_SHAPES)
-SHAPES = [(4, 8, 1), (8, 8, 1), (8, 16, 1), (16, 8, 1), (2, 8, 2), (2, 16, 2),
+SHAPES = [(4, 8, 1), (4, 16, 1), (8, 8, 1), (8, 16, 1), (16, 8, 1), (2, 8, 2), (2, 16, 2),
           (4, 4, 2), (4, 8, 2), (4, 16, 2), (8, 4, 2), (8, 8, 2)]
 KS = (2, 4, 6, 8, 10, 12, 16)
 DEFAULT = (6, 4, 8, 2)
@@ -201,7 +201,7 @@ def store_step_timed(W, H, tile, ms_per_step, dtype):
 # the fine Chebyshev smoother's launch shapes: tile kernels (seg = 0;
 # NX_CHEB_SHAPES in csrc/nx_cheb.hip) and streaming kernels (seg > 0;
 # NX_STREAM_SHAPES in csrc/nx_stream.hip)
-CHEB_SHAPES = [(2, 16, 2), (4, 8, 2), (4, 16, 2), (8, 8, 1), (8, 16, 1)]
+CHEB_SHAPES = [(2, 16, 2), (4, 8, 2), (4, 16, 2), (4, 16, 1), (8, 8, 1), (8, 16, 1)]
 CHEB_KS = (4, 6, 8, 10, 12, 16)
 STREAM_SHAPES = [(4, 8, 2), (2, 16, 2), (4, 16, 2), (8, 8, 2), (8, 4, 2)]
 STREAM_KS = (8, 12, 16)

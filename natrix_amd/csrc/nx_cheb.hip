@@ -406,6 +406,7 @@ hipError_t launch_cheb(const float* xin, const float* rin, const float* din, con
   X(2, 16, 2)             \
   X(4, 8, 2)              \
   X(4, 16, 2)             \
+  X(4, 16, 1)             \
   X(8, 8, 1)              \
   X(8, 16, 1)
 

@@ -189,6 +189,7 @@ hipError_t launch_tb(const float* pin, const float* b, const uint8_t* mask,
 // lane).  The table is the autotuner's search space.
 #define NX_TB_SHAPES(X) \
   X(4, 8, 1)            \
+  X(4, 16, 1)           \
   X(8, 8, 1)            \
   X(8, 16, 1)           \
   X(16, 8, 1)           \

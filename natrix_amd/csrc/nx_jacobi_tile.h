@@ -16,11 +16,12 @@ namespace jt {
 // Chebyshev tile of 8 cells per lane takes 83 VGPRs (5 waves, 2 blocks of 8
 // waves); asked for 6 it fits in 80 with one spilled VGPR, and a 3rd block
 // fits.  Its residual variant and the larger tiles would spill heavily, so
-// they stay free.
+// they stay free.  Tiles of 4 cells per lane ask for 8, so that two blocks
+// of 16 waves fit.
 template <int CELLS>
 constexpr int jacobi_waves_per_eu() { return CELLS <= 8 ? 8 : 1; }
 template <int CELLS, bool RES>
-constexpr int cheb_waves_per_eu() { return CELLS == 8 && !RES ? 6 : 1; }
+constexpr int cheb_waves_per_eu() { return CELLS <= 4 ? 8 : (CELLS == 8 && !RES ? 6 : 1); }
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

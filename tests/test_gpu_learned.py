@@ -250,10 +250,11 @@ def _tile(t):
                                         ((8, 2, 16, 2), 8), ((8, 2, 16, 2), 40),
                                         ((5, 4, 8, 2), 40), ((3, 8, 8, 1), 17),
                                         ((12, 2, 16, 2), 30), (None, 40),
-                                        ((10, 8, 16, 1), 25), ((6, 4, 16, 2), 33)])
+                                        ((10, 8, 16, 1), 25), ((6, 4, 16, 2), 33),
+                                        ((9, 4, 16, 1), 29)])
 def test_fine_chebyshev_matches_float32_statement(gpu, rng, H, W, tile, iters):
     """nx_smooth_chebyshev equals the float32 statement of the Chebyshev
-    iteration (cheb32: the kernels' operation order, no contraction)
+    iteration (cheb32: the kernels' operation order and fused multiply-adds)
     bitwise, for every tile shape, odd and even passes per launch (float2
     and scalar accesses) and the default tile (None); the reported residual
     is the FP64 relative residual of the result."""

@@ -18,7 +18,7 @@ from tests.helpers import (assert_bitwise, assert_close, blob_mask, f32,
 
 pytestmark = pytest.mark.gpu
 
-TILES = [(4, 8, 1), (8, 8, 1), (8, 16, 1), (16, 8, 1), (2, 8, 2), (2, 16, 2),
+TILES = [(4, 8, 1), (4, 16, 1), (8, 8, 1), (8, 16, 1), (16, 8, 1), (2, 8, 2), (2, 16, 2),
          (4, 4, 2), (4, 8, 2), (4, 16, 2), (8, 4, 2), (8, 8, 2)]
 
 
