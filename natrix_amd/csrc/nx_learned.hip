@@ -79,14 +79,24 @@ namespace {
 
 int fail(int code, const char* msg) { return nx::set_error(code, msg); }
 
-// Same rule as natrix_amd.learned.unet.conv_splits.
+// Same rule as natrix_amd.learned.unet.conv_splits.  A split count whose
+// splits do not fit one workgroup's waves (at most 16 splits of at most 9
+// K steps: the in-workgroup split-K convolution) is replaced by the
+// smallest one that does, when there is one.
 int conv_splits(long long m, int cin, int cout) {
   if (cin % 16) return 1;
   const long long base = ((m + 255) / 256) * (cout / (cout % 64 == 0 ? 64 : 32));
   if (base >= 256) return 1;
-  const long long a = (9 * cin / 16) / 8, b = (256 + base - 1) / base;
-  const long long s = a < b ? a : b;
-  return s < 1 ? 1 : (int)s;
+  const int steps = 9 * cin / 16;
+  const long long a = steps / 8, b = (256 + base - 1) / base;
+  long long s = a < b ? a : b;
+  if (s <= 1) return 1;
+  const long long per = (steps + s - 1) / s, used = (steps + per - 1) / per;
+  if (used > 16 || per > 9) {
+    const long long s2 = (steps + 8) / 9;
+    if (s2 <= 16) s = s2;
+  }
+  return (int)s;
 }
 
 __device__ __forceinline__ float block_sum_1024(float v, float* sh) {

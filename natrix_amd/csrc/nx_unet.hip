@@ -233,6 +233,93 @@ __global__ void __launch_bounds__(256)
   y[i] = tobf(v);
 }
 
+// In-workgroup split-K (Cin % 16 == 0): a workgroup of S = used waves
+// computes one 32 x 32 output tile (32 pixels, 32 output channels); wave z
+// runs K steps [z per, min(z per + per, steps)) of 16 channels of one tap,
+// exactly the steps split z of k_conv3x3 runs, with every fragment load of
+// its (at most KS_MAX) steps issued before its first MFMA.  The S partial
+// tiles are then summed in LDS in split order from 0, the bias added, ReLU
+// applied and the tile stored in bf16: the operations of the split kernel's
+// partials summed by k_conv_finish, in the same order (one MFMA computes
+// each output element's 16-term dot product the same way whatever else is
+// in its tile), so the result is bitwise the same, with no float32 partial
+// leaving the chip and no second launch.  Dynamic LDS: S x 4 KiB.
+constexpr int KS_MAX = 9, KS_WAVES = 16;
+
+template <bool WT, bool CAT>
+__global__ void __launch_bounds__(64 * KS_WAVES)
+    k_conv3x3_ks(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+                 const float* __restrict__ bias, uint16_t* __restrict__ y, int N, int H,
+                 int W, int Cin, int Cout, int relu, int per, int steps,
+                 const uint16_t* __restrict__ x2, int C1) {
+  extern __shared__ float ks_part[];  // [S][32 rows][32 columns]
+  const int lane = threadIdx.x & 63;
+  const int z = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int S = blockDim.x >> 6;
+  const int r = lane & 31, kb = lane >> 5;
+  const long long M = (long long)N * H * W;
+  const long long m_base = (long long)blockIdx.x * 32;
+  const int n_base = blockIdx.y * 32;
+  const long long m = m_base + r;
+  const bool valid = m < M;
+  const int hw = H * W;
+  const long long mm = valid ? m : 0;
+  const long long img = mm / hw;
+  const int rem = (int)(mm - img * hw);
+  const int py = rem / W, px = rem - py * W;
+  const long long pimg = img * hw, pimgh = img * (long long)(H >> 1) * (W >> 1);
+  const int cps = Cin >> 4;
+  const int ks0 = z * per, nst = min(per, steps - ks0);  // >= 1: z < used
+  uint4 a[KS_MAX], b[KS_MAX];
+  uint32_t msk[KS_MAX];
+#pragma unroll
+  for (int t = 0; t < KS_MAX; ++t) {
+    if (t >= nst) continue;  // wave-uniform; every index stays static
+    const int ks = ks0 + t;
+    const int tap = ks / cps, sblk = ks - tap * cps;
+    const int ci = sblk * 16 + kb * 8;
+    const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+    const bool ok = valid && yy >= 0 && yy < H && xx >= 0 && xx < W;
+    const bool up = CAT && ci < C1;
+    long long off;
+    if (!CAT)
+      off = ok ? (pimg + (long long)yy * W + xx) * Cin + ci : 0;
+    else if (up)
+      off = ok ? (pimgh + (long long)(yy >> 1) * (W >> 1) + (xx >> 1)) * C1 + ci : 0;
+    else
+      off = ok ? (pimg + (long long)yy * W + xx) * (Cin - C1) + (ci - C1) : 0;
+    a[t] = *reinterpret_cast<const uint4*>((CAT && !up ? x2 : x) + off);
+    msk[t] = ok ? 0xFFFFFFFFu : 0u;
+    b[t] = *reinterpret_cast<const uint4*>(
+        WT ? w + ((long long)(tap * cps + sblk) * Cout + n_base + r) * 16 + kb * 8
+           : w + ((long long)(n_base + r) * 9 + tap) * Cin + ci);
+  }
+  f32x16 acc;
+  for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+  for (int t = 0; t < KS_MAX; ++t) {
+    if (t >= nst) continue;
+    const bf16x8 af = __builtin_bit_cast(
+        bf16x8, make_uint4(a[t].x & msk[t], a[t].y & msk[t], a[t].z & msk[t], a[t].w & msk[t]));
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, __builtin_bit_cast(bf16x8, b[t]), acc, 0,
+                                                  0, 0);
+  }
+  // lane l holds column l mod 32 and rows 8 (e / 4) + 4 (l / 32) + e mod 4
+  float* mine = ks_part + z * 1024;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) mine[(8 * (e >> 2) + 4 * kb + (e & 3)) * 32 + r] = acc[e];
+  __syncthreads();
+  for (int o = threadIdx.x; o < 1024; o += 64 * S) {
+    const int row = o >> 5, col = o & 31;
+    float v = 0.0f;
+    for (int q = 0; q < S; ++q) v += ks_part[q * 1024 + o];
+    v += bias ? bias[n_base + col] : 0.0f;
+    if (relu) v = fmaxf(v, 0.0f);
+    const long long mo = m_base + row;
+    if (mo < M) y[mo * Cout + n_base + col] = tobf(v);
+  }
+}
+
 // w[co][tap][ci] -> w'[ci][8 - tap][co]
 __global__ void k_flip_weights(const uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
                                int Cin, int Cout) {
@@ -472,6 +559,21 @@ static int conv_fwd(const void* x, const void* w, const float* bias, void* y, in
   const uint16_t* ws = (const uint16_t*)w;
   uint16_t* ys = (uint16_t*)y;
   const uint16_t* x2s = (const uint16_t*)x2;
+  if (part && used <= KS_WAVES && per <= KS_MAX) {
+    // every split fits one wave of a workgroup: in-workgroup split-K
+    const dim3 grid((unsigned)((M + 31) / 32), cout / 32);
+    const size_t lds = (size_t)used * 1024 * sizeof(float);
+    if (x2s)
+      k_conv3x3_ks<true, true><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd, cin,
+                                                           cout, relu, per, steps, x2s, c1);
+    else if (tiled)
+      k_conv3x3_ks<true, false><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd, cin,
+                                                            cout, relu, per, steps, nullptr, 0);
+    else
+      k_conv3x3_ks<false, false><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd, cin,
+                                                             cout, relu, per, steps, nullptr, 0);
+    return 0;
+  }
   if (x2s) {
     // decoder convolution on the upsampled + concatenated input (tiled weights)
     if (cout % 64 == 0) {

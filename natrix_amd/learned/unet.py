@@ -44,13 +44,26 @@ def conv_splits(m, cin, cout):
     per compute unit (~256) for small images with many channels, at least 8
     K steps of 16 channels each (fewer, longer splits: every split writes
     and the finish pass re-reads a float32 partial of the whole output); 1
-    (no split) when the image alone fills the chip."""
+    (no split) when the image alone fills the chip.  A count whose splits do
+    not fit one workgroup's waves (at most 16 splits of at most 9 K steps:
+    the in-workgroup split-K convolution, no finish pass) is replaced by
+    the smallest one that does, when there is one."""
     if cin % 16:
         return 1
     base = -(-m // 256) * (cout // (64 if cout % 64 == 0 else 32))
     if base >= 256:
         return 1
-    return max(1, min((9 * cin // 16) // 8, -(-256 // base)))
+    steps = 9 * cin // 16
+    s = max(1, min(steps // 8, -(-256 // base)))
+    if s == 1:
+        return 1
+    per = -(-steps // s)
+    used = -(-steps // per)
+    if used > 16 or per > 9:
+        s2 = -(-steps // 9)
+        if s2 <= 16:
+            s = s2
+    return s
 
 
 class Conv3x3Fn(torch.autograd.Function):

@@ -44,11 +44,19 @@ def test_conv3x3_forward(gpu, n, h, w, ci, co):
 @pytest.mark.parametrize("n,h,w,ci,co,splits", [(1, 16, 16, 256, 256, 36),
                                                 (1, 32, 32, 384, 128, 54),
                                                 (1, 8, 8, 64, 64, 5),
-                                                (2, 12, 20, 32, 96, 18)])
+                                                (2, 12, 20, 32, 96, 18),
+                                                (1, 16, 16, 256, 256, 16),
+                                                (1, 32, 32, 128, 128, 9),
+                                                (1, 64, 64, 192, 64, 13),
+                                                (1, 10, 14, 64, 64, 4)])
 def test_conv3x3_forward_split_k(gpu, n, h, w, ci, co, splits):
-    """K split over launches with float32 partials summed in split order:
-    the same float64 oracle as the single launch, and within bf16 rounding
-    of it; repeated calls are bitwise equal (the sum's order is fixed)."""
+    """K split with float32 partials summed in split order: over launches
+    and a finish pass, or, when every split fits one wave (at most 16
+    splits of at most 9 K steps: 5, 16, 9, 13 here, 140 pixels in the last
+    case, not a multiple of the 32-pixel tile), over the waves of one
+    workgroup summed in LDS.  The same float64 oracle as the single launch,
+    and within bf16 rounding of it; repeated calls are bitwise equal (the
+    sum's order is fixed)."""
     import ctypes as C
     from natrix_amd import _lib
     g = torch.Generator().manual_seed(ci * co + splits)
