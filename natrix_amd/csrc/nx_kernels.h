@@ -144,6 +144,8 @@ hipError_t launch_cheb_tb(hipStream_t st, JacobiCfg c, const float* xin, const f
 // contiguous block per K step and output-channel group); the same results
 // as nx_conv3x3_fwd / nx_conv3x3_fwd_split with n = 1.
 int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t s);
+int head_fwd_scaled(const void* x, const float* w, const float* bias, float* out,
+                    long long pixels, int c, const float* scal, hipStream_t s);
 // With x2: the input is concat(upsample2(x) (c1 channels, h/2 x wd/2), x2
 // (cin - c1 channels, h x wd)), read in the fragment loads (decoder layers).
 int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
@@ -151,8 +153,8 @@ int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y,
                       const void* x2 = nullptr, int c1 = 0);
 
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
-// sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 8
-// Chebyshev iterations run banded over several workgroups and need
+// sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 12
+// Chebyshev iterations run on 2D tiles over many workgroups and need
 // ws = 6 Wc Hc floats.
 int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* rc,
                          const uint8_t* mc, int iterations, float lambda_min, float* ws);

@@ -13,7 +13,7 @@
 //      splits them, so nx_unet_forward equals the Python forward bitwise
 //      (the decoder convolutions read the upsampled, concatenated input in
 //      their fragment loads: the same values as the Python path's copies);
-//   4. k_scale_output: e = s x head output;
+//   4. e = s x head output (applied by the head kernel);
 //   5. Chebyshev smoothing of the coarse problem from e (nx_coarse_smooth);
 //   6. k_center_output (one block): e -= fluid mean of e, 0 on solid cells;
 //   7. interpolation and the energy line search (nx_prolong_line_search).
@@ -81,8 +81,9 @@ int fail(int code, const char* msg) { return nx::set_error(code, msg); }
 
 // Same rule as natrix_amd.learned.unet.conv_splits.  A split count whose
 // splits do not fit one workgroup's waves (at most 16 splits of at most 9
-// K steps: the in-workgroup split-K convolution) is replaced by the
-// smallest one that does, when there is one.
+// K steps, one load phase of the in-workgroup split-K convolution) is
+// replaced by the smallest one that does, else by 16 splits when those
+// take at most 18 steps (two load phases).
 int conv_splits(long long m, int cin, int cout) {
   if (cin % 16) return 1;
   const long long base = ((m + 255) / 256) * (cout / (cout % 64 == 0 ? 64 : 32));
@@ -94,7 +95,10 @@ int conv_splits(long long m, int cin, int cout) {
   const long long per = (steps + s - 1) / s, used = (steps + per - 1) / per;
   if (used > 16 || per > 9) {
     const long long s2 = (steps + 8) / 9;
-    if (s2 <= 16) s = s2;
+    if (s2 <= 16)
+      s = s2;
+    else if ((steps + 15) / 16 <= 18)
+      s = 16;
   }
   return (int)s;
 }
@@ -147,11 +151,6 @@ __global__ void __launch_bounds__(1024)
     const uint32_t lo = nx::f32_to_bf16(v[k] / s) | (nx::f32_to_bf16(solid[k] ? 1.0f : 0.0f) << 16);
     reinterpret_cast<uint4*>(x)[i] = make_uint4(lo, 0u, 0u, 0u);
   }
-}
-
-__global__ void k_scale_output(float* __restrict__ e, const float* __restrict__ scal, int n) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) e[i] = e[i] * scal[0];
 }
 
 __global__ void k_zero_output(float* __restrict__ e, int n) {
@@ -210,7 +209,9 @@ int conv_cat(nx_unet* u, int i, const void* lo, const void* skip, int c1, void* 
                                u->cout[i], 1, s, skip, c1);
 }
 
-int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s) {
+// scal != nullptr: the head's output multiplied by scal[0] (the rescaling)
+int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s,
+            const float* scal = nullptr) {
   const int H = u->H, W = u->W, c = u->base;
 #define NX_TRY_U(expr)          \
   do {                          \
@@ -234,7 +235,11 @@ int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s) {
   NX_TRY_U(conv(u, 11, u->s1, u->d1, H / 2, W / 2, s));
   NX_TRY_U(conv_cat(u, 12, u->d1, u->e0, 2 * c, u->s0, H, W, s));
   NX_TRY_U(conv(u, 13, u->s0, u->d0, H, W, s));
-  NX_TRY_U(nx_head_fwd(u->d0, u->head_w, u->head_b, out, (long long)H * W, c, s));
+  if (scal)
+    NX_TRY_U(nx::head_fwd_scaled(u->d0, u->head_w, u->head_b, out, (long long)H * W, c, scal,
+                                 s));
+  else
+    NX_TRY_U(nx_head_fwd(u->d0, u->head_w, u->head_b, out, (long long)H * W, c, s));
 #undef NX_TRY_U
   return NX_OK;
 }
@@ -250,10 +255,8 @@ int coarse_pipeline(nx_unet* u, int iters, float lmin, hipStream_t s) {
   } else {
     k_pack_input<<<1, 1024, 0, s>>>(u->rc, u->mc, u->x0, u->scal, n);
     if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "pack launch failed");
-    rc = forward(u, u->x0, u->out, s);
+    rc = forward(u, u->x0, u->out, s, u->scal);  // the head applies the rescaling
     if (rc != NX_OK) return rc;
-    k_scale_output<<<(n + 255) / 256, 256, 0, s>>>(u->out, u->scal, n);
-    if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "scale launch failed");
   }
   rc = nx::launch_coarse_smooth(s, u->W, u->H, u->out, u->rc, u->mc, iters, lmin, u->cws);
   if (rc != NX_OK) return rc;

@@ -236,17 +236,18 @@ __global__ void __launch_bounds__(256)
 // In-workgroup split-K (Cin % 16 == 0): a workgroup of S = used waves
 // computes one 32 x 32 output tile (32 pixels, 32 output channels); wave z
 // runs K steps [z per, min(z per + per, steps)) of 16 channels of one tap,
-// exactly the steps split z of k_conv3x3 runs, with every fragment load of
-// its (at most KS_MAX) steps issued before its first MFMA.  The S partial
+// exactly the steps split z of k_conv3x3 runs, in phases of KS_MAX steps
+// (at most KS_PHASES) whose fragment loads are all issued before the
+// phase's first MFMA.  The S partial
 // tiles are then summed in LDS in split order from 0, the bias added, ReLU
 // applied and the tile stored in bf16: the operations of the split kernel's
 // partials summed by k_conv_finish, in the same order (one MFMA computes
 // each output element's 16-term dot product the same way whatever else is
 // in its tile), so the result is bitwise the same, with no float32 partial
 // leaving the chip and no second launch.  Dynamic LDS: S x 4 KiB.
-constexpr int KS_MAX = 9, KS_WAVES = 16;
+constexpr int KS_MAX = 9, KS_PHASES = 2, KS_WAVES = 16;
 
-template <bool WT, bool CAT>
+template <bool WT, bool CAT, int PH>
 __global__ void __launch_bounds__(64 * KS_WAVES)
     k_conv3x3_ks(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                  const float* __restrict__ bias, uint16_t* __restrict__ y, int N, int H,
@@ -270,39 +271,48 @@ __global__ void __launch_bounds__(64 * KS_WAVES)
   const long long pimg = img * hw, pimgh = img * (long long)(H >> 1) * (W >> 1);
   const int cps = Cin >> 4;
   const int ks0 = z * per, nst = min(per, steps - ks0);  // >= 1: z < used
-  uint4 a[KS_MAX], b[KS_MAX];
-  uint32_t msk[KS_MAX];
-#pragma unroll
-  for (int t = 0; t < KS_MAX; ++t) {
-    if (t >= nst) continue;  // wave-uniform; every index stays static
-    const int ks = ks0 + t;
-    const int tap = ks / cps, sblk = ks - tap * cps;
-    const int ci = sblk * 16 + kb * 8;
-    const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
-    const bool ok = valid && yy >= 0 && yy < H && xx >= 0 && xx < W;
-    const bool up = CAT && ci < C1;
-    long long off;
-    if (!CAT)
-      off = ok ? (pimg + (long long)yy * W + xx) * Cin + ci : 0;
-    else if (up)
-      off = ok ? (pimgh + (long long)(yy >> 1) * (W >> 1) + (xx >> 1)) * C1 + ci : 0;
-    else
-      off = ok ? (pimg + (long long)yy * W + xx) * (Cin - C1) + (ci - C1) : 0;
-    a[t] = *reinterpret_cast<const uint4*>((CAT && !up ? x2 : x) + off);
-    msk[t] = ok ? 0xFFFFFFFFu : 0u;
-    b[t] = *reinterpret_cast<const uint4*>(
-        WT ? w + ((long long)(tap * cps + sblk) * Cout + n_base + r) * 16 + kb * 8
-           : w + ((long long)(n_base + r) * 9 + tap) * Cin + ci);
-  }
   f32x16 acc;
   for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+  // the steps in PH phases of at most KS_MAX: all loads of a phase, then
+  // its MFMAs in step order (the accumulation order of the split kernel)
 #pragma unroll
-  for (int t = 0; t < KS_MAX; ++t) {
-    if (t >= nst) continue;
-    const bf16x8 af = __builtin_bit_cast(
-        bf16x8, make_uint4(a[t].x & msk[t], a[t].y & msk[t], a[t].z & msk[t], a[t].w & msk[t]));
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, __builtin_bit_cast(bf16x8, b[t]), acc, 0,
-                                                  0, 0);
+  for (int q = 0; q < PH; ++q) {
+    const int ph = q * KS_MAX;
+    if (ph >= nst) continue;  // wave-uniform
+    const int np = min(KS_MAX, nst - ph);
+    uint4 a[KS_MAX], b[KS_MAX];
+    uint32_t msk[KS_MAX];
+#pragma unroll
+    for (int t = 0; t < KS_MAX; ++t) {
+      if (t >= np) continue;  // wave-uniform; every index stays static
+      const int ks = ks0 + ph + t;
+      const int tap = ks / cps, sblk = ks - tap * cps;
+      const int ci = sblk * 16 + kb * 8;
+      const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+      const bool ok = valid && yy >= 0 && yy < H && xx >= 0 && xx < W;
+      const bool up = CAT && ci < C1;
+      long long off;
+      if (!CAT)
+        off = ok ? (pimg + (long long)yy * W + xx) * Cin + ci : 0;
+      else if (up)
+        off = ok ? (pimgh + (long long)(yy >> 1) * (W >> 1) + (xx >> 1)) * C1 + ci : 0;
+      else
+        off = ok ? (pimg + (long long)yy * W + xx) * (Cin - C1) + (ci - C1) : 0;
+      a[t] = *reinterpret_cast<const uint4*>((CAT && !up ? x2 : x) + off);
+      msk[t] = ok ? 0xFFFFFFFFu : 0u;
+      b[t] = *reinterpret_cast<const uint4*>(
+          WT ? w + ((long long)(tap * cps + sblk) * Cout + n_base + r) * 16 + kb * 8
+             : w + ((long long)(n_base + r) * 9 + tap) * Cin + ci);
+    }
+#pragma unroll
+    for (int t = 0; t < KS_MAX; ++t) {
+      if (t >= np) continue;
+      const bf16x8 af = __builtin_bit_cast(
+          bf16x8,
+          make_uint4(a[t].x & msk[t], a[t].y & msk[t], a[t].z & msk[t], a[t].w & msk[t]));
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, __builtin_bit_cast(bf16x8, b[t]), acc,
+                                                    0, 0, 0);
+    }
   }
   // lane l holds column l mod 32 and rows 8 (e / 4) + 4 (l / 32) + e mod 4
   float* mine = ks_part + z * 1024;
@@ -499,14 +509,18 @@ __global__ void k_concat(uint16_t* __restrict__ y, uint16_t* __restrict__ a,
 }
 
 // 1x1 head: out[p] = sum_c x[p][c] w[c] + bias (float32 out)
+// scal != nullptr: the output is then multiplied by scal[0] (the learned
+// correction's rescaling, one rounding after the head's, as a separate pass
+// would do it)
 __global__ void k_head(const uint16_t* __restrict__ x, const float* __restrict__ w,
                        const float* __restrict__ bias, float* __restrict__ out,
-                       long long P, int C) {
+                       long long P, int C, const float* __restrict__ scal = nullptr) {
   const long long p = blockIdx.x * 256ll + threadIdx.x;
   if (p >= P) return;
   float s = 0.0f;
   for (int c = 0; c < C; ++c) s += bf(x[p * C + c]) * w[c];
-  out[p] = s + bias[0];
+  const float v = s + bias[0];
+  out[p] = scal ? v * scal[0] : v;
 }
 
 // head backward: dx[p][c] = dout[p] w[c]; per-block partials of
@@ -559,19 +573,31 @@ static int conv_fwd(const void* x, const void* w, const float* bias, void* y, in
   const uint16_t* ws = (const uint16_t*)w;
   uint16_t* ys = (uint16_t*)y;
   const uint16_t* x2s = (const uint16_t*)x2;
-  if (part && used <= KS_WAVES && per <= KS_MAX) {
-    // every split fits one wave of a workgroup: in-workgroup split-K
+  if (part && used <= KS_WAVES && per <= KS_MAX * KS_PHASES) {
+    // every split fits one wave of a workgroup: in-workgroup split-K, one
+    // load phase when the splits take at most KS_MAX steps (fewer
+    // registers), else two
     const dim3 grid((unsigned)((M + 31) / 32), cout / 32);
     const size_t lds = (size_t)used * 1024 * sizeof(float);
-    if (x2s)
-      k_conv3x3_ks<true, true><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd, cin,
-                                                           cout, relu, per, steps, x2s, c1);
-    else if (tiled)
-      k_conv3x3_ks<true, false><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd, cin,
-                                                            cout, relu, per, steps, nullptr, 0);
-    else
-      k_conv3x3_ks<false, false><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd, cin,
-                                                             cout, relu, per, steps, nullptr, 0);
+#define NX_KS_LAUNCH(PH)                                                                    \
+  if (x2s)                                                                                  \
+    k_conv3x3_ks<true, true, PH><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd,   \
+                                                              cin, cout, relu, per, steps,  \
+                                                              x2s, c1);                     \
+  else if (tiled)                                                                           \
+    k_conv3x3_ks<true, false, PH><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd,  \
+                                                               cin, cout, relu, per, steps, \
+                                                               nullptr, 0);                 \
+  else                                                                                      \
+    k_conv3x3_ks<false, false, PH><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd, \
+                                                                cin, cout, relu, per, steps, \
+                                                                nullptr, 0);
+    if (per <= KS_MAX) {
+      NX_KS_LAUNCH(1)
+    } else {
+      NX_KS_LAUNCH(KS_PHASES)
+    }
+#undef NX_KS_LAUNCH
     return 0;
   }
   if (x2s) {
@@ -753,6 +779,15 @@ int conv3x3_tile_weights(const void* w, void* wt, int cin, int cout, hipStream_t
   const long long n = 9LL * cin * cout;
   k_tile_weights<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((const uint16_t*)w,
                                                              (uint16_t*)wt, cin, cout);
+  NX_UNET_RET();
+}
+
+// nx_head_fwd with the output multiplied by scal[0] (device scalar)
+int head_fwd_scaled(const void* x, const float* w, const float* bias, float* out,
+                    long long pixels, int c, const float* scal, hipStream_t s) {
+  if (!x || !w || !bias || !out || !scal || pixels < 1 || c < 1)
+    return set_error(NX_ERR_ARG, "head: null argument or empty shape");
+  k_head<<<blocks_for(pixels), 256, 0, s>>>((const uint16_t*)x, w, bias, out, pixels, c, scal);
   NX_UNET_RET();
 }
 

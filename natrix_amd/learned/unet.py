@@ -45,9 +45,10 @@ def conv_splits(m, cin, cout):
     K steps of 16 channels each (fewer, longer splits: every split writes
     and the finish pass re-reads a float32 partial of the whole output); 1
     (no split) when the image alone fills the chip.  A count whose splits do
-    not fit one workgroup's waves (at most 16 splits of at most 9 K steps:
-    the in-workgroup split-K convolution, no finish pass) is replaced by
-    the smallest one that does, when there is one."""
+    not fit one workgroup's waves (at most 16 splits of at most 9 K steps,
+    one load phase of the in-workgroup split-K convolution, no finish pass)
+    is replaced by the smallest one that does, else by 16 splits when those
+    take at most 18 steps (two load phases)."""
     if cin % 16:
         return 1
     base = -(-m // 256) * (cout // (64 if cout % 64 == 0 else 32))
@@ -63,6 +64,8 @@ def conv_splits(m, cin, cout):
         s2 = -(-steps // 9)
         if s2 <= 16:
             s = s2
+        elif -(-steps // 16) <= 18:
+            s = 16
     return s
 
 
