@@ -148,9 +148,10 @@ int head_fwd_scaled(const void* x, const float* w, const float* bias, float* out
                     long long pixels, int c, const float* scal, hipStream_t s);
 // With x2: the input is concat(upsample2(x) (c1 channels, h/2 x wd/2), x2
 // (cin - c1 channels, h x wd)), read in the fragment loads (decoder layers).
+// With ypool: also the 2 x 2 average pool of y (nx_avgpool2's values).
 int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
                       int splits, int h, int wd, int cin, int cout, int relu, hipStream_t s,
-                      const void* x2 = nullptr, int c1 = 0);
+                      const void* x2 = nullptr, int c1 = 0, void* ypool = nullptr);
 
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
 // sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 12

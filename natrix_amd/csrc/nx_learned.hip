@@ -187,12 +187,16 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-int conv(nx_unet* u, int i, const void* x, void* y, int h, int w, hipStream_t s) {
+// ypool: also the 2 x 2 average pool of the output (fused into the
+// convolution where it can be)
+int conv(nx_unet* u, int i, const void* x, void* y, int h, int w, hipStream_t s,
+         void* ypool = nullptr) {
   const long long m = (long long)h * w;
   const int sp = conv_splits(m, u->cin[i], u->cout[i]);
   if (u->wt[i])
     return nx::conv3x3_fwd_tiled(x, u->wt[i], u->b[i], y, u->work, sp, h, w, u->cin[i],
-                                 u->cout[i], 1, s);
+                                 u->cout[i], 1, s, nullptr, 0, ypool);
+  if (ypool) return fail(NX_ERR_ARG, "pooled convolution needs tiled weights");
   if (sp > 1)
     return nx_conv3x3_fwd_split(x, u->w[i], u->b[i], y, u->work, sp, 1, h, w, u->cin[i],
                                 u->cout[i], 1, s);
@@ -219,14 +223,11 @@ int forward(nx_unet* u, const uint16_t* x, float* out, hipStream_t s,
     if (rc_ != NX_OK) return rc_; \
   } while (0)
   NX_TRY_U(conv(u, 0, x, u->t0, H, W, s));
-  NX_TRY_U(conv(u, 1, u->t0, u->e0, H, W, s));
-  NX_TRY_U(nx_avgpool2(u->e0, u->p0, 1, H, W, c, 0, s));
+  NX_TRY_U(conv(u, 1, u->t0, u->e0, H, W, s, u->p0));
   NX_TRY_U(conv(u, 2, u->p0, u->t1, H / 2, W / 2, s));
-  NX_TRY_U(conv(u, 3, u->t1, u->e1, H / 2, W / 2, s));
-  NX_TRY_U(nx_avgpool2(u->e1, u->p1, 1, H / 2, W / 2, 2 * c, 0, s));
+  NX_TRY_U(conv(u, 3, u->t1, u->e1, H / 2, W / 2, s, u->p1));
   NX_TRY_U(conv(u, 4, u->p1, u->t2, H / 4, W / 4, s));
-  NX_TRY_U(conv(u, 5, u->t2, u->e2, H / 4, W / 4, s));
-  NX_TRY_U(nx_avgpool2(u->e2, u->p2, 1, H / 4, W / 4, 4 * c, 0, s));
+  NX_TRY_U(conv(u, 5, u->t2, u->e2, H / 4, W / 4, s, u->p2));
   NX_TRY_U(conv(u, 6, u->p2, u->t3, H / 8, W / 8, s));
   NX_TRY_U(conv(u, 7, u->t3, u->e3, H / 8, W / 8, s));
   NX_TRY_U(conv_cat(u, 8, u->e3, u->e2, 8 * c, u->s2, H / 4, W / 4, s));

@@ -247,13 +247,17 @@ __global__ void __launch_bounds__(256)
 // leaving the chip and no second launch.  Dynamic LDS: S x 4 KiB.
 constexpr int KS_MAX = 9, KS_PHASES = 2, KS_WAVES = 16;
 
-template <bool WT, bool CAT, int PH>
+// POOL (W % 16 == 0, H % 2 == 0): the tile's 32 pixels are 16 columns of 2
+// rows instead of 32 consecutive pixels, and the epilogue also writes the
+// 2 x 2 average pool of its bf16 outputs to yp, summed ((x00 + x01) + x10)
+// + x11 and scaled by 0.25 as k_avgpool2 does.
+template <bool WT, bool CAT, int PH, bool POOL>
 __global__ void __launch_bounds__(64 * KS_WAVES)
     k_conv3x3_ks(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                  const float* __restrict__ bias, uint16_t* __restrict__ y, int N, int H,
                  int W, int Cin, int Cout, int relu, int per, int steps,
-                 const uint16_t* __restrict__ x2, int C1) {
-  extern __shared__ float ks_part[];  // [S][32 rows][32 columns]
+                 const uint16_t* __restrict__ x2, int C1, uint16_t* __restrict__ yp) {
+  extern __shared__ float ks_part[];  // [S][32 rows][32 columns] (+ POOL: [32][32])
   const int lane = threadIdx.x & 63;
   const int z = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int S = blockDim.x >> 6;
@@ -261,9 +265,18 @@ __global__ void __launch_bounds__(64 * KS_WAVES)
   const long long M = (long long)N * H * W;
   const long long m_base = (long long)blockIdx.x * 32;
   const int n_base = blockIdx.y * 32;
-  const long long m = m_base + r;
-  const bool valid = m < M;
   const int hw = H * W;
+  // POOL tile: image ti, rows 2 ty, 2 ty + 1, columns 16 tx .. 16 tx + 15
+  const int tw = W >> 4;
+  const long long ti = POOL ? (long long)blockIdx.x / (tw * (H >> 1)) : 0;
+  const int trem = POOL ? (int)(blockIdx.x - ti * (tw * (H >> 1))) : 0;
+  const int ty = POOL ? trem / tw : 0, tx = POOL ? trem - ty * tw : 0;
+  auto pixel_of = [&](int row) -> long long {  // tile row (MFMA row) -> pixel index
+    return POOL ? ti * hw + (long long)(2 * ty + (row >> 4)) * W + 16 * tx + (row & 15)
+                : m_base + row;
+  };
+  const long long m = pixel_of(r);
+  const bool valid = m < M;
   const long long mm = valid ? m : 0;
   const long long img = mm / hw;
   const int rem = (int)(mm - img * hw);
@@ -319,14 +332,30 @@ __global__ void __launch_bounds__(64 * KS_WAVES)
 #pragma unroll
   for (int e = 0; e < 16; ++e) mine[(8 * (e >> 2) + 4 * kb + (e & 3)) * 32 + r] = acc[e];
   __syncthreads();
+  float* outv = ks_part + S * 1024;  // POOL: the tile's bf16 outputs
   for (int o = threadIdx.x; o < 1024; o += 64 * S) {
     const int row = o >> 5, col = o & 31;
     float v = 0.0f;
     for (int q = 0; q < S; ++q) v += ks_part[q * 1024 + o];
     v += bias ? bias[n_base + col] : 0.0f;
     if (relu) v = fmaxf(v, 0.0f);
-    const long long mo = m_base + row;
-    if (mo < M) y[mo * Cout + n_base + col] = tobf(v);
+    const long long mo = pixel_of(row);
+    const uint16_t ob = tobf(v);
+    if (mo < M) y[mo * Cout + n_base + col] = ob;
+    if (POOL) outv[o] = bf(ob);
+  }
+  if (POOL) {
+    __syncthreads();
+    // pooled pixel (ty, 8 tx + q) from tile rows 2q, 2q + 1 (top) and
+    // 16 + 2q, 17 + 2q (bottom)
+    for (int o = threadIdx.x; o < 256; o += 64 * S) {
+      const int q = o >> 5, col = o & 31;
+      const float sum = ((outv[(2 * q) * 32 + col] + outv[(2 * q + 1) * 32 + col]) +
+                         outv[(16 + 2 * q) * 32 + col]) +
+                        outv[(17 + 2 * q) * 32 + col];
+      const long long po = (ti * (H >> 1) + ty) * (long long)(W >> 1) + 8 * tx + q;
+      yp[po * Cout + n_base + col] = tobf(sum * 0.25f);
+    }
   }
 }
 
@@ -561,10 +590,13 @@ int set_error(int code, const std::string& msg);
 
 extern "C" {
 
+// ypool != nullptr: also the 2 x 2 average pool of y (k_avgpool2's
+// values), fused into the in-workgroup split-K convolution's epilogue when
+// that runs and the size allows, else by k_avgpool2 after the convolution.
 static int conv_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                     int h, int wd, int cin, int cout, int relu, float* part, int splits,
                     hipStream_t s, bool tiled = false, const void* x2 = nullptr,
-                    int c1 = 0) {
+                    int c1 = 0, void* ypool = nullptr) {
   const long long M = (long long)n * h * wd;
   const int steps = 9 * cin / 16;
   const int per = part ? (steps + splits - 1) / splits : 0;
@@ -578,26 +610,32 @@ static int conv_fwd(const void* x, const void* w, const float* bias, void* y, in
     // load phase when the splits take at most KS_MAX steps (fewer
     // registers), else two
     const dim3 grid((unsigned)((M + 31) / 32), cout / 32);
-    const size_t lds = (size_t)used * 1024 * sizeof(float);
-#define NX_KS_LAUNCH(PH)                                                                    \
+    const bool pool = ypool && wd % 16 == 0 && h % 2 == 0 && used < KS_WAVES;
+    const size_t lds = (size_t)(used + (pool ? 1 : 0)) * 1024 * sizeof(float);
+    uint16_t* yp = (uint16_t*)ypool;
+#define NX_KS_LAUNCH(PH, PL)                                                                \
   if (x2s)                                                                                  \
-    k_conv3x3_ks<true, true, PH><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd,   \
-                                                              cin, cout, relu, per, steps,  \
-                                                              x2s, c1);                     \
+    k_conv3x3_ks<true, true, PH, PL><<<grid, 64 * used, lds, s>>>(                          \
+        xs, ws, bias, ys, n, h, wd, cin, cout, relu, per, steps, x2s, c1, yp);              \
   else if (tiled)                                                                           \
-    k_conv3x3_ks<true, false, PH><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd,  \
-                                                               cin, cout, relu, per, steps, \
-                                                               nullptr, 0);                 \
+    k_conv3x3_ks<true, false, PH, PL><<<grid, 64 * used, lds, s>>>(                         \
+        xs, ws, bias, ys, n, h, wd, cin, cout, relu, per, steps, nullptr, 0, yp);           \
   else                                                                                      \
-    k_conv3x3_ks<false, false, PH><<<grid, 64 * used, lds, s>>>(xs, ws, bias, ys, n, h, wd, \
-                                                                cin, cout, relu, per, steps, \
-                                                                nullptr, 0);
+    k_conv3x3_ks<false, false, PH, PL><<<grid, 64 * used, lds, s>>>(                        \
+        xs, ws, bias, ys, n, h, wd, cin, cout, relu, per, steps, nullptr, 0, yp);
     if (per <= KS_MAX) {
-      NX_KS_LAUNCH(1)
+      if (pool) {
+        NX_KS_LAUNCH(1, true)
+      } else {
+        NX_KS_LAUNCH(1, false)
+      }
     } else {
-      NX_KS_LAUNCH(KS_PHASES)
+      NX_KS_LAUNCH(KS_PHASES, false)
     }
 #undef NX_KS_LAUNCH
+    if (ypool && !(pool && per <= KS_MAX))
+      k_avgpool2<<<blocks_for((long long)n * (h / 2) * (wd / 2) * cout), 256, 0, s>>>(
+          ys, yp, n, h, wd, cout);
     return 0;
   }
   if (x2s) {
@@ -633,6 +671,9 @@ static int conv_fwd(const void* x, const void* w, const float* bias, void* y, in
     k_conv_finish<<<(unsigned)((mc + 255) / 256), 256, 0, s>>>(part, bias, (uint16_t*)y, mc,
                                                                cout, used, relu);
   }
+  if (ypool)
+    k_avgpool2<<<blocks_for((long long)n * (h / 2) * (wd / 2) * cout), 256, 0, s>>>(
+        ys, (uint16_t*)ypool, n, h, wd, cout);
   return 0;
 }
 
@@ -793,14 +834,16 @@ int head_fwd_scaled(const void* x, const float* w, const float* bias, float* out
 
 int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y, float* work,
                       int splits, int h, int wd, int cin, int cout, int relu, hipStream_t s,
-                      const void* x2, int c1) {
+                      const void* x2, int c1, void* ypool) {
   if (!x || !wt || !y || h < 1 || wd < 1 || cin % 16 || cout % 32 || splits < 1 ||
       splits > 9 * cin / 16 || (splits > 1 && !work))
     return set_error(NX_ERR_ARG, "tiled conv3x3: cin % 16 == 0, cout % 32 == 0, valid splits");
   if (x2 && (c1 < 16 || c1 % 16 || c1 >= cin || h % 2 || wd % 2))
     return set_error(NX_ERR_ARG, "concatenating conv3x3: 16 <= c1 < cin, c1 % 16 == 0, even size");
+  if (ypool && (h % 2 || wd % 2))
+    return set_error(NX_ERR_ARG, "pooled conv3x3: even size");
   conv_fwd(x, wt, bias, y, 1, h, wd, cin, cout, relu, splits > 1 ? work : nullptr, splits, s,
-           true, x2, c1);
+           true, x2, c1, ypool);
   NX_UNET_RET();
 }
 }  // namespace nx
