@@ -345,10 +345,13 @@ int nx_batch_collect_timing(nx_batch* b, double* ms, int* launches);
 int nx_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int n,
                    int h, int wd, int cin, int cout, int relu, void* stream);
 /* the same convolution with the 9 cin / 16 K steps split over `splits`
- * groups of launches that write float32 partial sums to
- * work[splits * n*h*w * cout] (device), then summed in split order with
- * bias and ReLU: more workgroups for small images with many channels
- * (cin % 16 == 0) */
+ * groups, their float32 partial sums summed in split order with bias and
+ * ReLU: more workgroups for small images with many channels (cin % 16 ==
+ * 0).  When every split fits one wave of a workgroup (at most 16 splits of
+ * at most 18 K steps) the splits of a 32 x 32 output tile run in one
+ * workgroup and are summed in its LDS; otherwise they run as separate
+ * workgroups that write work[splits * n*h*w * cout] (device, float32),
+ * summed by a second launch.  Both give the same values. */
 int nx_conv3x3_fwd_split(const void* x, const void* w, const float* bias, void* y,
                          float* work, int splits, int n, int h, int wd, int cin, int cout,
                          int relu, void* stream);
