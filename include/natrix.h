@@ -159,9 +159,8 @@ int nx_step_begin(nx_sim* sim, float dt);
  * <= tol or max_sweeps sweeps were applied; `out` (nullable) gets the stats
  * of the whole step so far */
 int nx_solve(nx_sim* sim, double tol, int max_sweeps, nx_stats* out);
-/* coarse correction problem: coarse_rhs[Hc*Wc] (device, float32) = the sum
- * of the fine residual over the fluid cells of each factor x factor block
- * (factor^2 x the block mean, solid and out-of-grid cells counting as 0),
+/* coarse correction problem: coarse_rhs[Hc*Wc] (device, float32) = factor^2
+ * x the fluid mean of the fine residual per factor x factor block,
  * coarse_mask[Hc*Wc] (device, uint8) = 1 for blocks with no fluid cell;
  * Wc = ceil(W/factor), Hc = ceil(H/factor) */
 int nx_restrict_residual(nx_sim* sim, int factor, float* coarse_rhs,

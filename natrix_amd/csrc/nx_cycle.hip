@@ -20,15 +20,10 @@ using namespace nx;
 
 // ---- coarse-grid transfer for the learned guess --------------------------
 // Restriction: coarse cell (I, J) of factor f covers fine cells
-// [fI, fI+f) x [fJ, fJ+f).  rhs_c = the sum of the float32 residual
-// r = b - A p over its fluid fine cells, i.e. f^2 times the mean of r over
-// all f x f cells with solid and out-of-grid cells counting as zero (the
-// correction equation in coarse grid units, and the transpose of
-// piecewise-constant interpolation: a block that is mostly solid passes on
-// only its fluid cells' share instead of having it scaled up to a full
-// block, which keeps the coarse correction from overshooting next to
-// obstacles); mask_c = 1 only when every one of its fine cells is solid, so
-// the coarse fluid region covers the fine one.  Two passes: per fine row y and
+// [fI, fI+f) x [fJ, fJ+f).  rhs_c = f^2 * mean over its fluid fine cells of
+// the float32 residual r = b - A p (the correction equation in coarse grid
+// units); mask_c = 1 only when every one of its fine cells is solid, so the
+// coarse fluid region covers the fine one.  Two passes: per fine row y and
 // coarse column I, the sum and fluid count of r over the row's f cells
 // (row-major lanes, one H x Wc record each), then per coarse cell the sum of
 // its f row records.
@@ -167,7 +162,7 @@ __global__ void __launch_bounds__(256)
     sum += rsum[y * Wv + X];
     cnt += rcnt[y * Wv + X];
   }
-  rc[Y * Wp + X] = cnt == 0.0f ? 0.0f : sum;
+  rc[Y * Wp + X] = cnt == 0.0f ? 0.0f : (float)(f * f) * (sum / cnt);
   mc[Y * Wp + X] = cnt == 0.0f ? 1 : 0;
 }
 

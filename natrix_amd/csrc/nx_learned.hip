@@ -3,9 +3,8 @@
 // One call of nx_learned_correct enqueues, on the simulator's stream and
 // with no host synchronisation, the whole correction of one learned cycle:
 //   1. restriction of the current residual to the coarse grid
-//      (rc = block sum of the fluid residual, coarse mask; the network's
-//      grid may extend beyond ceil(W / f) x ceil(H / f), the extra cells
-//      solid);
+//      (rc = f^2 x block mean, coarse mask; the network's grid may extend
+//      beyond ceil(W / f) x ceil(H / f), the extra cells solid);
 //   2. k_pack_input (one block): s = RMS of rc over fluid coarse cells,
 //      network input x[p] = (bf16(rc/s), bf16(solid), 0 x 6);
 //   3. the U-Net forward (4 levels, 3x3 conv + ReLU pairs, 2x2 average pool,

@@ -8,9 +8,8 @@ part of the pressure, which is what plain Jacobi converges slowly on (its
 slowest mode decays like 1 - pi^2 / (4 N^2) per sweep).
 
 One learned cycle:
-  1. restrict the current residual to the coarse grid (block sum: f^2 x
-     block mean with solid cells counting as zero, so the coarse problem
-     is in coarse grid units) and scale it to unit RMS;
+  1. restrict the current residual to the coarse grid (f^2 x block mean, so
+     the coarse problem is in coarse grid units) and scale it to unit RMS;
   2. run the U-Net and scale back: the coarse correction's first guess;
   3. Chebyshev-accelerated Jacobi on the coarse problem from that guess
      (tiled over 88 workgroups, 12 iterations per launch): removes the

@@ -44,10 +44,8 @@ def cheb_ref(x, rhs, solid, iters, lmin):
 
 
 def restrict_ref(p, b, solid, f):
-    """(rc, mc): rc = the sum of r = b - L p over the fluid cells of each
-    f x f block (f^2 x the block mean with solid and out-of-grid cells
-    counting as zero; 0 where the block has no fluid cell), mc = 1 for
-    blocks without a fluid cell."""
+    """(rc, mc): rc = f^2 x the mean of r = b - L p over the fluid cells of
+    each f x f block (0 where the block has none), mc = 1 for such blocks."""
     H, W = p.shape
     Hc, Wc = -(-H // f), -(-W // f)
     r = np.where(solid, 0.0, b - lap(p, solid))
@@ -58,7 +56,7 @@ def restrict_ref(p, b, solid, f):
     pad_c[:H, :W] = fl
     s = pad_r.reshape(Hc, f, Wc, f).sum(axis=(1, 3))
     c = pad_c.reshape(Hc, f, Wc, f).sum(axis=(1, 3))
-    rc = np.where(c > 0, s, 0.0)
+    rc = np.where(c > 0, f * f * s / np.maximum(c, 1), 0.0)
     return rc, (c == 0)
 
 
