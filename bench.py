@@ -495,9 +495,9 @@ def bench_learned(args, copy_bandwidth):
         sim.synchronize()
         return sim
 
-    def timed_learned(sim, steps, g=guess):
+    def timed_learned(sim, steps, g=guess, smooth=CHEB_SMOOTH):
         for _ in range(2):  # first-call costs (allocator, code objects) off the clock
-            solve_step(sim, g, dt, tol=args.tol, smooth=CHEB_SMOOTH)
+            solve_step(sim, g, dt, tol=args.tol, smooth=smooth)
         sim.synchronize()
         torch.cuda.synchronize()
         # no per-launch events in this timed region (they cost ~10 % of a
@@ -505,7 +505,7 @@ def bench_learned(args, copy_bandwidth):
         sweeps, outer, conv = [], [], []
         t0 = time.perf_counter()
         for _ in range(steps):
-            st = solve_step(sim, g, dt, tol=args.tol, smooth=CHEB_SMOOTH)
+            st = solve_step(sim, g, dt, tol=args.tol, smooth=smooth)
             sweeps.append(st["sweeps"])
             outer.append(st["outer"])
             conv.append(st["converged"] and st["residual"] <= args.tol)
@@ -541,13 +541,15 @@ def bench_learned(args, copy_bandwidth):
     if args.classical_coarse > 0:
         # the same two-grid cycle with the network off: the coarse problem
         # is smoothed from zero with more Chebyshev iterations instead (the
-        # method without the learned guess, at its best measured setting)
+        # method without the learned guess, at its best measured setting:
+        # 350 coarse and 72 fine iterations per cycle)
         g0 = LearnedGuess(net, coarse=meta.get("size", 128),
                           coarse_iters=args.classical_coarse, use_network=False)
         sim = fresh()
-        classical = timed_learned(sim, args.steps, g0)
+        classical = timed_learned(sim, args.steps, g0, smooth=args.classical_smooth)
         sim.destroy()
         classical["coarse_iterations"] = args.classical_coarse
+        classical["fine_iterations_per_cycle"] = args.classical_smooth
         classical["flow"] = learned["flow"]
         log(f"bench two-grid without the network: {classical}")
 
@@ -733,6 +735,8 @@ def main():
     ap.add_argument("--classical-coarse", type=int, default=350,
                     help="learned workload: also time the cycle without the network, "
                          "with this many coarse Chebyshev iterations (0: skip)")
+    ap.add_argument("--classical-smooth", type=int, default=CHEB_SMOOTH,
+                    help="fine Chebyshev iterations per cycle of the cycle without the network")
     ap.add_argument("--developed-steps", type=int, default=10,
                     help="learned workload: timed steps of the developed-flow leg")
     ap.add_argument("--fixed-steps", type=int, default=100,
