@@ -165,6 +165,16 @@ int nx_solve(nx_sim* sim, double tol, int max_sweeps, nx_stats* out);
  * Wc = ceil(W/factor), Hc = ceil(H/factor) */
 int nx_restrict_residual(nx_sim* sim, int factor, float* coarse_rhs,
                          uint8_t* coarse_mask);
+/* the restriction of the face-weighted coarse operator (no reference
+ * counterpart): coarse_rhs = the sum of the fine residual over the fluid
+ * cells of each block (factor^2 x the block mean, solid cells counting as
+ * 0), coarse_mask as nx_restrict_residual, faces[2*Hc*Wc] (device, float32)
+ * = the open fraction of each coarse cell's east face (first Hc*Wc) and
+ * south face: the fine rows (columns) of the block whose cells on both
+ * sides of the block boundary are fluid, over factor; 0 at the grid's
+ * edge and for solid cells */
+int nx_restrict_residual_faces(nx_sim* sim, int factor, float* coarse_rhs,
+                               uint8_t* coarse_mask, float* faces);
 /* pressure += scale * bilinear interpolation of coarse[Hc*Wc] (device) on
  * fluid cells, weights renormalised over the fluid coarse cells of
  * coarse_mask (as written by nx_restrict_residual) */
@@ -186,6 +196,15 @@ int nx_prolong_line_search(nx_sim* sim, int factor, const float* coarse,
  * operator on [lambda_min, 8] (-L has its spectrum in [0, 8]) */
 int nx_coarse_smooth(nx_sim* sim, int Wc, int Hc, float* e, const float* rc,
                      const uint8_t* coarse_mask, int iterations, float lambda_min);
+/* Chebyshev smoothing (0 < lambda_min < 8) of the face-weighted coarse
+ * problem (no reference counterpart): (L e)_I = sum over the faces of I of
+ * w (e_J - e_I) with the weights of nx_restrict_residual_faces (spectrum
+ * of -L in [0, 8]); a neighbour across a face of weight w enters as the
+ * centre (w = 0), itself (w = 1) or fma(w, e_J - e_I, e_I), summed in the
+ * plain kernel's order */
+int nx_coarse_smooth_faces(nx_sim* sim, int Wc, int Hc, float* e, const float* rc,
+                           const uint8_t* coarse_mask, const float* faces, int iterations,
+                           float lambda_min);
 /* `sweeps` temporal-blocked Jacobi sweeps of the current pressure system,
  * then (omega > 0) one damped sweep, then the FP64 relative residual; one
  * synchronisation.  out (host, may be NULL) = {relative residual, the last

@@ -140,6 +140,7 @@ SIGNATURES = {
     "nx_step_begin": [_P, _F],
     "nx_solve": [_P, _D, _I, C.POINTER(Stats)],
     "nx_restrict_residual": [_P, _I, _P, _P],
+    "nx_restrict_residual_faces": [_P, _I, _P, _P, _P],
     "nx_prolong_add": [_P, _I, _P, _P, _F],
     "nx_prolong_line_search": [_P, _I, _P, _P, _P],
     "nx_line_search": [_P, _P, _P],
@@ -149,6 +150,7 @@ SIGNATURES = {
     "nx_set_smoother_tile": [_P, C.POINTER(_I), _I],
     "nx_get_smoother_tile": [_P, C.POINTER(_I)],
     "nx_coarse_smooth": [_P, _I, _I, _P, _P, _P, _I, _F],
+    "nx_coarse_smooth_faces": [_P, _I, _I, _P, _P, _P, _P, _I, _F],
     "nx_refine_begin": [_P],
     "nx_refine_residual": [_P, C.POINTER(_D)],
     "nx_refine_accumulate": [_P],

@@ -145,11 +145,15 @@ __global__ void __launch_bounds__(256)
 }
 
 // Coarse cells of a padded coarse grid (Wp x Hp beyond the Wv x Hv cells
-// that cover the fine grid) are solid with a zero right-hand side.
+// that cover the fine grid) are solid with a zero right-hand side.  With
+// `block_sum` rhs_c is the plain sum S over the block's fluid cells (f^2
+// times the mean over all f x f cells, solid ones counting as zero): the
+// transpose of piecewise-constant interpolation, which pairs with the
+// face-weighted coarse operator of k_coarse_faces.
 __global__ void __launch_bounds__(256)
     k_restrict_cols(const float* __restrict__ rsum, const float* __restrict__ rcnt,
                     float* __restrict__ rc, uint8_t* __restrict__ mc, int H, int f,
-                    int Wv, int Hv, int Wp, int Hp) {
+                    int Wv, int Hv, int Wp, int Hp, int block_sum) {
   const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y * 4 + threadIdx.y;
   if (X >= Wp || Y >= Hp) return;
   if (X >= Wv || Y >= Hv) {
@@ -162,8 +166,42 @@ __global__ void __launch_bounds__(256)
     sum += rsum[y * Wv + X];
     cnt += rcnt[y * Wv + X];
   }
-  rc[Y * Wp + X] = cnt == 0.0f ? 0.0f : (float)(f * f) * (sum / cnt);
+  rc[Y * Wp + X] = cnt == 0.0f ? 0.0f : block_sum ? sum : (float)(f * f) * (sum / cnt);
   mc[Y * Wp + X] = cnt == 0.0f ? 1 : 0;
+}
+
+// Face weights of the coarse operator that goes with the block-sum
+// restriction: the east face of coarse cell (X, Y) weighs n / f, n the
+// fine rows y of the block whose cells (f X + f - 1, y) and (f X + f, y)
+// are both fluid and inside the grid (the open fraction of the block
+// boundary); the south face likewise over the block's columns.  A face at
+// the grid's edge, of a padded cell or of a solid cell weighs 0, an open
+// face between full blocks 1.  The coarse operator is then
+// (L e)_I = sum over faces w (e_J - e_I): a channel narrower than a block
+// conducts in proportion to its width instead of as a full cell, and a
+// sliver of fluid at an obstacle's edge couples only through its open
+// faces.  faces[0, Hp Wp) holds the east weights, faces[Hp Wp, 2 Hp Wp)
+// the south ones.
+__global__ void __launch_bounds__(256)
+    k_coarse_faces(const uint8_t* __restrict__ mask, int W, int H, int f, int Wp, int Hp,
+                   float* __restrict__ faces) {
+  const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y * 4 + threadIdx.y;
+  if (X >= Wp || Y >= Hp) return;
+  int ne = 0, ns = 0;
+  const int xe = f * X + f, ys = f * Y + f;  // first column / row of the next block
+  if (xe < W)
+    for (int k = 0; k < f; ++k) {
+      const int y = f * Y + k;
+      if (y < H) ne += (mask[y * W + xe - 1] == 0 && mask[y * W + xe] == 0) ? 1 : 0;
+    }
+  if (ys < H)
+    for (int k = 0; k < f; ++k) {
+      const int x = f * X + k;
+      if (x < W) ns += (mask[(ys - 1) * W + x] == 0 && mask[ys * W + x] == 0) ? 1 : 0;
+    }
+  const int pos = Y * Wp + X;
+  faces[pos] = (float)ne / (float)f;
+  faces[Wp * Hp + pos] = (float)ns / (float)f;
 }
 
 // Prolongation: p += scale * e interpolated at the fine cell, on fluid
@@ -387,12 +425,19 @@ constexpr int CB_T = 12, CT_ROWS = 2, CT_WAVES = 16, CT_HALO = CB_T + 1;
 constexpr int CT_EY = CT_ROWS * CT_WAVES, CT_VX = 64 - 2 * CT_HALO, CT_VY = CT_EY - 2 * CT_HALO;
 static_assert(CT_VX > 0 && CT_VY > 0, "coarse tile smaller than its halo");
 
+// FACES: the face-weighted operator of k_coarse_faces.  A neighbour across
+// a face of weight w enters the sum as the centre c when w = 0 (as a solid
+// neighbour does), as itself when w = 1, and as fma(w, v - c, c) between,
+// so that the sum minus 4 c is the sum of w (v - c); a cell whose faces
+// are all shut or all open sees the operations of the plain kernel.
+template <bool FACES>
 __global__ void __launch_bounds__(64 * CT_WAVES)
     k_coarse_cheb_tile(const float* __restrict__ xin, const float* __restrict__ rin,
                        const float* __restrict__ din, float* __restrict__ xout,
                        float* __restrict__ rout, float* __restrict__ dout,
                        const float* __restrict__ rhs, const uint8_t* __restrict__ mc,
-                       int Wc, int Hc, int s0, int T, float theta, ChebCoef cf) {
+                       const float* __restrict__ faces, int Wc, int Hc, int s0, int T,
+                       float theta, ChebCoef cf) {
   constexpr int R = CT_ROWS;
   const int lane = threadIdx.x & 63;
   // wave-uniform: a scalar register, so branches on it are scalar branches
@@ -427,6 +472,17 @@ __global__ void __launch_bounds__(64 * CT_WAVES)
   }
   float x[R], r[R], d[R], lv[R];
   int code[R];  // bit 0: left, 1: right, 2: up (y-1), 3: down (y+1) solid
+  // FACES: the weights of the cell's four faces and which are partial
+  float wl[FACES ? R : 1], wr[FACES ? R : 1], wu[FACES ? R : 1], wd[FACES ? R : 1];
+  int part[FACES ? R : 1];
+  float fe[FACES ? R : 1], fs[FACES ? R + 1 : 1];
+  if (FACES) {
+    const int n = Wc * Hc;
+#pragma unroll
+    for (int i = 0; i < R; ++i) fe[i] = faces[min(max(gy0 + i, 0), Hc - 1) * Wc + cx];
+#pragma unroll
+    for (int i = 0; i < R + 1; ++i) fs[i] = faces[n + min(max(gy0 - 1 + i, 0), Hc - 1) * Wc + cx];
+  }
   bool any = false;
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -435,12 +491,28 @@ __global__ void __launch_bounds__(64 * CT_WAVES)
     x[i] = in ? lx[i] : 0.0f;
     r[i] = in ? lr[i] : 0.0f;
     d[i] = (in && !first) ? ld[i] : 0.0f;
-    // the lane shifts run with every lane active; lanes 0 and 63 keep their
-    // own flag, which only affects halo cells
-    const int lnb = jt::ilane_from_left(sf[i + 1]);
-    const int rnb = jt::ilane_from_right(sf[i + 1]);
-    code[i] = lnb | (rnb << 1) | (sf[i] << 2) | (sf[i + 2] << 3);
-    any |= code[i] != 0;
+    if (FACES) {
+      // a cell outside the grid, and a face on the grid's edge, is shut;
+      // the west face is the left lane's east face (lane 0 keeps its own:
+      // a halo cell)
+      wr[i] = (in && gx < Wc - 1) ? fe[i] : 0.0f;
+      wd[i] = (in && gy < Hc - 1) ? fs[i + 1] : 0.0f;
+      wu[i] = (in && gy > 0) ? fs[i] : 0.0f;
+      wl[i] = jt::lane_from_left(wr[i]);
+      if (!in) wl[i] = 0.0f;
+      code[i] = (wl[i] == 0.0f ? 1 : 0) | (wr[i] == 0.0f ? 2 : 0) | (wu[i] == 0.0f ? 4 : 0) |
+                (wd[i] == 0.0f ? 8 : 0);
+      part[i] = (wl[i] > 0.0f && wl[i] < 1.0f ? 1 : 0) | (wr[i] > 0.0f && wr[i] < 1.0f ? 2 : 0) |
+                (wu[i] > 0.0f && wu[i] < 1.0f ? 4 : 0) | (wd[i] > 0.0f && wd[i] < 1.0f ? 8 : 0);
+      any |= (code[i] | part[i]) != 0;
+    } else {
+      // the lane shifts run with every lane active; lanes 0 and 63 keep
+      // their own flag, which only affects halo cells
+      const int lnb = jt::ilane_from_left(sf[i + 1]);
+      const int rnb = jt::ilane_from_right(sf[i + 1]);
+      code[i] = lnb | (rnb << 1) | (sf[i] << 2) | (sf[i + 2] << 3);
+      any |= code[i] != 0;
+    }
   }
   const bool special = __any(any);  // wave-uniform
 
@@ -464,10 +536,17 @@ __global__ void __launch_bounds__(64 * CT_WAVES)
         const float U = i > 0 ? v[i - 1] : up0;
         const float D = i < R - 1 ? v[i + 1] : dn0;
         const int cd = code[i];
-        const float l = (cd & 1) ? c : fl;
-        const float rr = (cd & 2) ? c : fr;
-        const float u = (cd & 4) ? c : U;
-        const float dd = (cd & 8) ? c : D;
+        float l = (cd & 1) ? c : fl;
+        float rr = (cd & 2) ? c : fr;
+        float u = (cd & 4) ? c : U;
+        float dd = (cd & 8) ? c : D;
+        if (FACES) {
+          const int pt = part[i];
+          l = (pt & 1) ? __builtin_fmaf(wl[i], fl - c, c) : l;
+          rr = (pt & 2) ? __builtin_fmaf(wr[i], fr - c, c) : rr;
+          u = (pt & 4) ? __builtin_fmaf(wu[i], U - c, c) : u;
+          dd = (pt & 8) ? __builtin_fmaf(wd[i], D - c, c) : dd;
+        }
         out[i] = __builtin_fmaf(-4.0f, c, ((u + l) + rr) + dd);
       }
     } else {
@@ -515,9 +594,13 @@ __global__ void __launch_bounds__(64 * CT_WAVES)
 
 namespace nx {
 int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* rc,
-                         const uint8_t* mc, int iterations, float lambda_min, float* ws) {
+                         const uint8_t* mc, int iterations, float lambda_min, float* ws,
+                         const float* faces) {
   if (iterations == 0) return NX_OK;
-  if (lambda_min > 0.0f && iterations > CB_T) {
+  if (faces && !(lambda_min > 0.0f))
+    return set_error(NX_ERR_ARG, "the face-weighted coarse operator needs lambda_min > 0");
+  // the face-weighted operator always runs tiled (any number of iterations)
+  if (lambda_min > 0.0f && (iterations > CB_T || faces)) {
     // tiled: x, r, d ping-pong between the two halves of ws (6 Wc Hc
     // floats); the last launch writes x to e (read only by the first)
     if (!ws) return set_error(NX_ERR_ARG, "coarse smoother workspace missing");
@@ -533,10 +616,14 @@ int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* 
       float** out = set[done == 0 ? 0 : 1 - cur];
       ChebCoef cf{};
       rec.next(0, T, cf);
-      k_coarse_cheb_tile<<<grid, 64 * CT_WAVES, 0, st>>>(xin, set[cur][1], set[cur][2],
-                                                         last ? e : out[0], out[1], out[2],
-                                                         rc, mc, Wc, Hc, done, T, rec.theta,
-                                                         cf);
+      if (faces)
+        k_coarse_cheb_tile<true><<<grid, 64 * CT_WAVES, 0, st>>>(
+            xin, set[cur][1], set[cur][2], last ? e : out[0], out[1], out[2], rc, mc, faces,
+            Wc, Hc, done, T, rec.theta, cf);
+      else
+        k_coarse_cheb_tile<false><<<grid, 64 * CT_WAVES, 0, st>>>(
+            xin, set[cur][1], set[cur][2], last ? e : out[0], out[1], out[2], rc, mc, nullptr,
+            Wc, Hc, done, T, rec.theta, cf);
       if (hipGetLastError() != hipSuccess)
         return set_error(NX_ERR_HIP, "coarse smoother launch failed");
       cur = done == 0 ? 0 : 1 - cur;
@@ -891,8 +978,9 @@ int nx_get_smoother_tile(nx_sim* s, int* cfg) {
 }
 
 // restriction onto a Wp x Hp coarse grid, Wp >= ceil(W / f), Hp >= ceil(H / f)
+// faces != nullptr: block sums and the face weights (2 Wp Hp floats)
 static int restrict_impl(nx_sim* s, int factor, int Wp, int Hp, float* coarse_rhs,
-                         uint8_t* coarse_mask) {
+                         uint8_t* coarse_mask, float* faces = nullptr) {
   if (!s || !coarse_rhs || !coarse_mask || factor < 1)
     return fail(NX_ERR_ARG, "bad restriction arguments");
   if (!s->in_step) return fail(NX_ERR_STATE, "restriction outside a step");
@@ -916,8 +1004,13 @@ static int restrict_impl(nx_sim* s, int factor, int Wp, int Hp, float* coarse_rh
         s->p[s->pcur], s->div, s->mask[s->mcur], rsum, rcnt, s->W, s->H, factor, Wc);
   NX_HIP(hipGetLastError());
   k_restrict_cols<<<dim3((Wp + 63) / 64, (Hp + 3) / 4), dim3(64, 4), 0, s->stream>>>(
-      rsum, rcnt, coarse_rhs, coarse_mask, s->H, factor, Wc, Hc, Wp, Hp);
+      rsum, rcnt, coarse_rhs, coarse_mask, s->H, factor, Wc, Hc, Wp, Hp, faces ? 1 : 0);
   NX_HIP(hipGetLastError());
+  if (faces) {
+    k_coarse_faces<<<dim3((Wp + 63) / 64, (Hp + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+        s->mask[s->mcur], s->W, s->H, factor, Wp, Hp, faces);
+    NX_HIP(hipGetLastError());
+  }
   return NX_OK;
 }
 
@@ -926,6 +1019,13 @@ int nx_restrict_residual(nx_sim* s, int factor, float* coarse_rhs,
   if (!s || factor < 1) return fail(NX_ERR_ARG, "bad restriction arguments");
   return restrict_impl(s, factor, (s->W + factor - 1) / factor, (s->H + factor - 1) / factor,
                        coarse_rhs, coarse_mask);
+}
+
+int nx_restrict_residual_faces(nx_sim* s, int factor, float* coarse_rhs,
+                               uint8_t* coarse_mask, float* faces) {
+  if (!s || factor < 1 || !faces) return fail(NX_ERR_ARG, "bad restriction arguments");
+  return restrict_impl(s, factor, (s->W + factor - 1) / factor, (s->H + factor - 1) / factor,
+                       coarse_rhs, coarse_mask, faces);
 }
 
 int nx_prolong_add(nx_sim* s, int factor, const float* coarse,
@@ -941,21 +1041,36 @@ int nx_prolong_add(nx_sim* s, int factor, const float* coarse,
   return NX_OK;
 }
 
-int nx_coarse_smooth(nx_sim* s, int Wc, int Hc, float* e, const float* rc,
-                     const uint8_t* coarse_mask, int iterations, float lambda_min) {
+static int coarse_smooth_impl(nx_sim* s, int Wc, int Hc, float* e, const float* rc,
+                              const uint8_t* coarse_mask, const float* faces, int iterations,
+                              float lambda_min) {
   if (!s || !e || !rc || !coarse_mask) return fail(NX_ERR_ARG, "null argument");
   if (Wc < 1 || Hc < 1 || Wc > CJ_MAX || Hc > CJ_ROWS * CJ_WAVES || iterations < 0)
     return fail(NX_ERR_ARG, "coarse grid must be at most 128 x 128, iterations >= 0");
   if (!(lambda_min < 8.0f)) return fail(NX_ERR_ARG, "lambda_min must be below 8");
+  if (faces && !(lambda_min > 0.0f))
+    return fail(NX_ERR_ARG, "the face-weighted coarse operator needs lambda_min > 0");
   const size_t n = (size_t)Wc * Hc;
-  if (lambda_min > 0.0f && iterations > CB_T && 6 * n > s->cws_cap) {
+  if (lambda_min > 0.0f && (iterations > CB_T || faces) && 6 * n > s->cws_cap) {
     if (s->cws) NX_HIP(hipFree(s->cws));
     s->cws = nullptr;
     NX_HIP(hipMalloc(&s->cws, 6 * n * sizeof(float)));
     s->cws_cap = 6 * n;
   }
   return nx::launch_coarse_smooth(s->stream, Wc, Hc, e, rc, coarse_mask, iterations,
-                                  lambda_min, s->cws);
+                                  lambda_min, s->cws, faces);
+}
+
+int nx_coarse_smooth(nx_sim* s, int Wc, int Hc, float* e, const float* rc,
+                     const uint8_t* coarse_mask, int iterations, float lambda_min) {
+  return coarse_smooth_impl(s, Wc, Hc, e, rc, coarse_mask, nullptr, iterations, lambda_min);
+}
+
+int nx_coarse_smooth_faces(nx_sim* s, int Wc, int Hc, float* e, const float* rc,
+                           const uint8_t* coarse_mask, const float* faces, int iterations,
+                           float lambda_min) {
+  if (!faces) return fail(NX_ERR_ARG, "null argument");
+  return coarse_smooth_impl(s, Wc, Hc, e, rc, coarse_mask, faces, iterations, lambda_min);
 }
 
 int nx_damped_sweeps(nx_sim* s, int count, float omega) {
@@ -1094,8 +1209,9 @@ int nx_refine_end(nx_sim* s) {
 }  // extern "C"
 
 namespace nx {
-int restrict_residual_padded(nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc) {
-  return restrict_impl(s, factor, Wp, Hp, rc, mc);
+int restrict_residual_padded(nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc,
+                             float* faces) {
+  return restrict_impl(s, factor, Wp, Hp, rc, mc, faces);
 }
 int prolong_line_search_padded(nx_sim* s, int factor, int Wp, int Hp, const float* e,
                                const uint8_t* mc, double* out, bool apply) {

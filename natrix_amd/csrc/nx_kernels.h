@@ -84,8 +84,11 @@ bool jacobi_cfg_supported(JacobiCfg cfg);
 
 // Restriction of the residual onto, and interpolation with the energy line
 // search from, a Wp x Hp coarse grid padded beyond ceil(W / f) x ceil(H / f)
-// (padded coarse cells are solid, with a zero right-hand side).
-int restrict_residual_padded(::nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc);
+// (padded coarse cells are solid, with a zero right-hand side).  With
+// faces (2 Wp Hp floats): block sums and the face weights of the weighted
+// coarse operator.
+int restrict_residual_padded(::nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc,
+                             float* faces = nullptr);
 int prolong_line_search_padded(::nx_sim* s, int factor, int Wp, int Hp, const float* e,
                                const uint8_t* mc, double* out, bool apply = true);
 // nx_smooth_chebyshev taking first the line-search step that a
@@ -156,9 +159,11 @@ int conv3x3_fwd_tiled(const void* x, const void* wt, const float* bias, void* y,
 // Coarse smoothing of L e = rc on a Wc x Hc (<= 128 x 128) grid: Jacobi
 // sweeps (lambda_min <= 0) or Chebyshev on [lambda_min, 8]; more than 12
 // Chebyshev iterations run on 2D tiles over many workgroups and need
-// ws = 6 Wc Hc floats.
+// ws = 6 Wc Hc floats.  faces (2 Wc Hc floats, east then south weights):
+// the face-weighted operator, always tiled, Chebyshev only.
 int launch_coarse_smooth(hipStream_t st, int Wc, int Hc, float* e, const float* rc,
-                         const uint8_t* mc, int iterations, float lambda_min, float* ws);
+                         const uint8_t* mc, int iterations, float lambda_min, float* ws,
+                         const float* faces = nullptr);
 
 // Exact FP64 residual sums of `p` (of a zero guess when p == nullptr) per
 // 64 x RES_ROWS block of cells (256 threads, RES_ROWS / 4 rows each):

@@ -3,7 +3,10 @@
 // One call of nx_learned_correct enqueues, on the simulator's stream and
 // with no host synchronisation, the whole correction of one learned cycle:
 //   1. restriction of the current residual to the coarse grid
-//      (rc = f^2 x block mean, coarse mask; the network's grid may extend
+//      (with Chebyshev coarse smoothing: rc = the block sum of the
+//      residual and the coarse faces' open fractions, the pair of the
+//      face-weighted coarse operator; with Jacobi: rc = f^2 x fluid mean),
+//      coarse mask; the network's grid may extend
 //      beyond ceil(W / f) x ceil(H / f), the extra cells solid);
 //   2. k_pack_input (one block): s = RMS of rc over fluid coarse cells,
 //      network input x[p] = (bf16(rc/s), bf16(solid), 0 x 6);
@@ -14,7 +17,8 @@
 //      (the decoder convolutions read the upsampled, concatenated input in
 //      their fragment loads: the same values as the Python path's copies);
 //   4. e = s x head output (applied by the head kernel);
-//   5. Chebyshev smoothing of the coarse problem from e (nx_coarse_smooth);
+//   5. Chebyshev smoothing of the coarse problem from e (the face-weighted
+//      operator, nx_coarse_smooth_faces);
 //   6. k_center_output (one block): e -= fluid mean of e, 0 on solid cells;
 //   7. interpolation and the energy line search (nx_prolong_line_search).
 // Steps 2-6 touch only the runner's buffers; from the second call with the
@@ -61,6 +65,7 @@ struct nx_unet {
   size_t work_elems = 0;
   float* scal = nullptr;   // [0] = s (RMS of rc over fluid cells)
   float* cws = nullptr;    // Chebyshev workspace, 6 H W floats
+  float* faces = nullptr;  // coarse face weights, 2 H W floats (east, south)
   // steps 2-6 captured as one HIP graph (every pointer in them is the
   // runner's own), keyed by the stream and the smoothing parameters; the
   // capture happens on the second call with the same key, when everything
@@ -259,7 +264,10 @@ int coarse_pipeline(nx_unet* u, int iters, float lmin, hipStream_t s) {
     rc = forward(u, u->x0, u->out, s, u->scal);  // the head applies the rescaling
     if (rc != NX_OK) return rc;
   }
-  rc = nx::launch_coarse_smooth(s, u->W, u->H, u->out, u->rc, u->mc, iters, lmin, u->cws);
+  // Chebyshev coarse smoothing takes the face-weighted operator (with the
+  // block sums its restriction wrote); Jacobi the plain one
+  rc = nx::launch_coarse_smooth(s, u->W, u->H, u->out, u->rc, u->mc, iters, lmin, u->cws,
+                                lmin > 0.0f ? u->faces : nullptr);
   if (rc != NX_OK) return rc;
   k_center_output<<<1, 1024, 0, s>>>(u->out, u->mc, n);
   if (hipGetLastError() != hipSuccess) return fail(NX_ERR_HIP, "center launch failed");
@@ -378,6 +386,7 @@ int nx_unet_create(int H, int W, int base, const void* const* conv_w,
   u->work = (float*)alloc((u->work_elems ? u->work_elems : 1) * sizeof(float));
   u->scal = (float*)alloc(4 * sizeof(float));
   u->cws = (float*)alloc(6 * P0 * sizeof(float));
+  u->faces = (float*)alloc(2 * P0 * sizeof(float));
   void* wcopy[14] = {};
   for (int i = 0; i < 14 && ok; ++i) {
     if (ci[i] % 16 == 0)
@@ -445,7 +454,8 @@ static int learned_correct_impl(nx_sim* sim, nx_unet* u, int factor, int coarse_
   if (rc != NX_OK) return rc;
   if ((W + factor - 1) / factor > u->W || (H + factor - 1) / factor > u->H)
     return fail(NX_ERR_ARG, "the network's grid is smaller than the fine grid / factor");
-  rc = nx::restrict_residual_padded(sim, factor, u->W, u->H, u->rc, u->mc);
+  rc = nx::restrict_residual_padded(sim, factor, u->W, u->H, u->rc, u->mc,
+                                    lambda_min > 0.0f ? u->faces : nullptr);
   if (rc != NX_OK) return rc;
   rc = run_pipeline(u, coarse_iters, lambda_min, s);
   if (rc != NX_OK) return rc;

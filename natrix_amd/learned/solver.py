@@ -8,8 +8,10 @@ part of the pressure, which is what plain Jacobi converges slowly on (its
 slowest mode decays like 1 - pi^2 / (4 N^2) per sweep).
 
 One learned cycle:
-  1. restrict the current residual to the coarse grid (f^2 x block mean, so
-     the coarse problem is in coarse grid units) and scale it to unit RMS;
+  1. restrict the current residual to the coarse grid (the block sum over
+     fluid cells, so the coarse problem is in coarse grid units, with the
+     open fraction of every coarse face as the coarse operator's weight)
+     and scale it to unit RMS;
   2. run the U-Net and scale back: the coarse correction's first guess;
   3. Chebyshev-accelerated Jacobi on the coarse problem from that guess
      (tiled over 88 workgroups, 12 iterations per launch): removes the
@@ -173,8 +175,16 @@ class LearnedGuess:
                              f"({wc} x {hc} for f = {f}), which only the native runner takes")
         rc = torch.empty(hc * wc, dtype=torch.float32, device="cuda")
         mc = torch.empty(hc * wc, dtype=torch.uint8, device="cuda")
-        _lib.call("nx_restrict_residual", sim._alive(), f, C.c_void_p(rc.data_ptr()),
-                  C.c_void_p(mc.data_ptr()))
+        # Chebyshev coarse smoothing takes the face-weighted operator and its
+        # block-sum restriction, as the native runner does
+        faces = (torch.empty(2 * hc * wc, dtype=torch.float32, device="cuda")
+                 if self.lmin > 0 else None)
+        if faces is not None:
+            _lib.call("nx_restrict_residual_faces", sim._alive(), f, C.c_void_p(rc.data_ptr()),
+                      C.c_void_p(mc.data_ptr()), C.c_void_p(faces.data_ptr()))
+        else:
+            _lib.call("nx_restrict_residual", sim._alive(), f, C.c_void_p(rc.data_ptr()),
+                      C.c_void_p(mc.data_ptr()))
         fluid = mc == 0
         n = fluid.sum().clamp_min(1)
         s = torch.sqrt((rc * rc * fluid).sum() / n).clamp_min(1e-30)
@@ -183,9 +193,14 @@ class LearnedGuess:
             e = (self.net(x).view(-1) * s).contiguous()
         else:
             e = torch.zeros_like(rc)
-        _lib.call("nx_coarse_smooth", sim._alive(), wc, hc, C.c_void_p(e.data_ptr()),
-                  C.c_void_p(rc.data_ptr()), C.c_void_p(mc.data_ptr()),
-                  int(self.coarse_iters), float(self.lmin))
+        if faces is not None:
+            _lib.call("nx_coarse_smooth_faces", sim._alive(), wc, hc, C.c_void_p(e.data_ptr()),
+                      C.c_void_p(rc.data_ptr()), C.c_void_p(mc.data_ptr()),
+                      C.c_void_p(faces.data_ptr()), int(self.coarse_iters), float(self.lmin))
+        else:
+            _lib.call("nx_coarse_smooth", sim._alive(), wc, hc, C.c_void_p(e.data_ptr()),
+                      C.c_void_p(rc.data_ptr()), C.c_void_p(mc.data_ptr()),
+                      int(self.coarse_iters), float(self.lmin))
         e = (e - (e * fluid).sum() / n) * fluid
         if not line_search:
             _lib.call("nx_prolong_add", sim._alive(), f, C.c_void_p(e.data_ptr()),

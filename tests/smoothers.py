@@ -60,6 +60,37 @@ def restrict_ref(p, b, solid, f):
     return rc, (c == 0)
 
 
+def restrict_faces_ref(p, b, solid, f):
+    """(rc, mc, fe, fs) of the face-weighted coarse operator: rc = the sum of
+    r = b - L p over the fluid cells of each f x f block (0 where the block
+    has none), mc as restrict_ref, and the float32 open fractions of each
+    coarse cell's east face fe and south face fs: the number of the block's
+    fine rows (columns) whose cells on both sides of the block boundary are
+    fluid and inside the grid, over f; 0 on the grid's edge."""
+    H, W = p.shape
+    Hc, Wc = -(-H // f), -(-W // f)
+    r = np.where(solid, 0.0, b - lap(p, solid))
+    fl = ~solid
+    pad_r = np.zeros((Hc * f, Wc * f))
+    pad_c = np.zeros((Hc * f, Wc * f))
+    pad_r[:H, :W] = r
+    pad_c[:H, :W] = fl
+    s = pad_r.reshape(Hc, f, Wc, f).sum(axis=(1, 3))
+    c = pad_c.reshape(Hc, f, Wc, f).sum(axis=(1, 3))
+    fe = np.zeros((Hc, Wc), np.float32)
+    fs = np.zeros((Hc, Wc), np.float32)
+    for Y in range(Hc):
+        for X in range(Wc):
+            xe, ys = f * X + f, f * Y + f
+            if xe < W:
+                n = sum(1 for y in range(f * Y, min(f * Y + f, H)) if fl[y, xe - 1] and fl[y, xe])
+                fe[Y, X] = np.float32(n) / np.float32(f)
+            if ys < H:
+                n = sum(1 for x in range(f * X, min(f * X + f, W)) if fl[ys - 1, x] and fl[ys, x])
+                fs[Y, X] = np.float32(n) / np.float32(f)
+    return np.where(c > 0, s, 0.0), (c == 0), fe, fs
+
+
 def prolong_ref(ec, mc, f, H, W, solid):
     """Bilinear interpolation of the coarse field at every fluid fine cell:
     coarse centres at fine coordinate f I + (f - 1) / 2, clamped to the
@@ -142,7 +173,34 @@ def lapc32(v, solid):
     return (((nb[2] + nb[0]) + nb[1]) + nb[3]) - f32(4.0) * v
 
 
-def cheb32(x, rhs, solid, iters, lmin):
+def lapcw32(v, fe, fs):
+    """float32 L v of the face-weighted coarse operator, sum over faces of
+    w (v_J - v), in the Chebyshev kernels' order: a neighbour across a face
+    of weight w enters as v (w = 0), as itself (w = 1) or as
+    fma(w, v_J - v, v), then (((u + l) + r) + d) - 4 v.  Faces on the grid's
+    edge count as shut whatever fe, fs hold there."""
+    f32 = np.float32
+    H, W = v.shape
+    wr = fe.astype(f32).copy()
+    wr[:, -1] = 0
+    wd = fs.astype(f32).copy()
+    wd[-1, :] = 0
+    wl = np.zeros_like(wr)
+    wl[:, 1:] = wr[:, :-1]
+    wu = np.zeros_like(wd)
+    wu[1:, :] = wd[:-1, :]
+    P = np.pad(v, 1, mode="edge")
+
+    def term(w, dy, dx):
+        n = P[1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+        t = np.where(w == 0, v, n)
+        part = (w > 0) & (w < 1)
+        return np.where(part, fma32(w, (n - v).astype(f32), v), t)
+    l, r, u, d = term(wl, 0, -1), term(wr, 0, 1), term(wu, -1, 0), term(wd, 1, 0)
+    return (((u + l) + r) + d) - f32(4.0) * v
+
+
+def cheb32(x, rhs, solid, iters, lmin, lapf=None):
     """The fine and coarse Chebyshev iterations' float32 operations, in the
     kernels' order: L v as lapc32 (the -4 v is one rounding, 4 v being
     exact) and d <- fma(cd, d, cr r)."""
@@ -152,10 +210,11 @@ def cheb32(x, rhs, solid, iters, lmin):
     delta = f32(0.5) * (f32(8.0) - lmin)
     sigma = theta / delta
     rho = f32(1.0) / sigma
-    r = lapc32(x, solid) - rhs
+    lapf = lapf or (lambda v: lapc32(v, solid))
+    r = lapf(x) - rhs
     d = r / theta
     for _ in range(iters):
-        ld = lapc32(d, solid)
+        ld = lapf(d)
         rho1 = f32(1.0) / (f32(2.0) * sigma - rho)
         cd = rho1 * rho
         cr = f32(2.0) * rho1 / delta

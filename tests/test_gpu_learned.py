@@ -16,8 +16,8 @@ import torch
 
 from natrix_amd import FluidSimulator, _lib
 
-from smoothers import (cheb32, cheb_ref, jacobi32, jacobi_ref, lap, prolong_ref,
-                       restrict_ref)
+from smoothers import (cheb32, cheb_ref, jacobi32, jacobi_ref, lap, lapcw32, prolong_ref,
+                       restrict_faces_ref, restrict_ref)
 
 pytestmark = pytest.mark.gpu
 
@@ -77,6 +77,47 @@ def test_coarse_smoother_matches_float64_statement(gpu, rng, Wc, Hc, cheb, iters
     sim.destroy()
 
 
+@pytest.mark.parametrize("Wc,Hc", [(128, 128), (100, 72), (7, 128), (128, 13)])
+@pytest.mark.parametrize("iters", [5, 12, 13, 40, 100])
+def test_weighted_coarse_smoother_matches_float32_statement(gpu, rng, Wc, Hc, iters):
+    """The face-weighted coarse operator (always tiled, 12 iterations per
+    launch) against the float32 statement of its operations, bitwise, with
+    faces of every kind: shut (0, as toward a solid cell), open (1) and
+    partial."""
+    from natrix_amd.learned.solver import chebyshev_lmin
+    sim = in_step_sim(rng, 64, 64)
+    solid = rng.random((Hc, Wc)) < 0.1
+    fe = rng.choice([0.0, 0.25, 0.5, 0.75, 1.0, 1.0, 1.0], size=(Hc, Wc)).astype(np.float32)
+    fs = rng.choice([0.0, 0.25, 0.5, 0.75, 1.0, 1.0, 1.0], size=(Hc, Wc)).astype(np.float32)
+    # faces of and toward solid cells are shut, as k_coarse_faces writes them
+    fe[solid] = 0
+    fe[:, :-1][solid[:, 1:]] = 0
+    fs[solid] = 0
+    fs[:-1, :][solid[1:, :]] = 0
+    e0 = rng.normal(size=(Hc, Wc)).astype(np.float32)
+    rc = rng.normal(size=(Hc, Wc))
+    rc[~solid] -= rc[~solid].mean()
+    rc = rc.astype(np.float32)
+    e = torch.from_numpy(e0).cuda().contiguous()
+    r = torch.from_numpy(rc).cuda()
+    m = torch.from_numpy(solid.astype(np.uint8)).cuda()
+    fw = torch.from_numpy(np.concatenate([fe.ravel(), fs.ravel()])).cuda()
+    lmin = chebyshev_lmin(iters, 1e-3)
+    torch.cuda.synchronize()
+    _lib.call("nx_coarse_smooth_faces", sim._alive(), Wc, Hc, C.c_void_p(e.data_ptr()),
+              C.c_void_p(r.data_ptr()), C.c_void_p(m.data_ptr()), C.c_void_p(fw.data_ptr()),
+              iters, lmin)
+    sim.synchronize()
+    want = cheb32(e0, rc, solid, iters, lmin, lapf=lambda v: lapcw32(v, fe, fs))
+    assert np.array_equal(e.cpu().numpy(), want)
+    with pytest.raises(ValueError):  # Chebyshev only
+        _lib.call("nx_coarse_smooth_faces", sim._alive(), Wc, Hc, C.c_void_p(e.data_ptr()),
+                  C.c_void_p(r.data_ptr()), C.c_void_p(m.data_ptr()),
+                  C.c_void_p(fw.data_ptr()), iters, 0.0)
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
 def test_coarse_smoother_rejects_oversized_grids(gpu, rng):
     sim = in_step_sim(rng, 32, 32)
     t = torch.zeros(129 * 4, device="cuda")
@@ -110,6 +151,38 @@ def test_restriction_matches_float64_statement(gpu, rng, N, f):
     got = rc.cpu().numpy().reshape(Hc, Wc)
     assert np.array_equal(mc.cpu().numpy().reshape(Hc, Wc) != 0, wmask)
     assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+    _lib.call("nx_step_finish", sim._alive())
+    sim.destroy()
+
+
+@pytest.mark.parametrize("N,f", [(64, 2), (96, 3), (128, 4), (128, 16), (80, 32), (100, 16)])
+def test_restriction_with_faces_matches_statement(gpu, rng, N, f):
+    """Block sums and the coarse faces' open fractions (the pair of the
+    face-weighted coarse operator) on a grid with a circular obstacle, which
+    cuts blocks and faces; 100 / 16 leaves a ragged last block."""
+    sim = in_step_sim(rng, N, N)
+    solid = sim.download("mask") != 0
+    b = rng.normal(size=(N, N)).astype(np.float32)
+    p = rng.normal(size=(N, N)).astype(np.float32)
+    dev(sim, "divergence").copy_(torch.from_numpy(b))
+    dev(sim, "pressure").copy_(torch.from_numpy(p))
+    Wc = Hc = -(-N // f)
+    rc = torch.empty(Hc * Wc, device="cuda")
+    mc = torch.empty(Hc * Wc, dtype=torch.uint8, device="cuda")
+    faces = torch.full((2 * Hc * Wc,), -1.0, device="cuda")
+    torch.cuda.synchronize()
+    _lib.call("nx_restrict_residual_faces", sim._alive(), f, C.c_void_p(rc.data_ptr()),
+              C.c_void_p(mc.data_ptr()), C.c_void_p(faces.data_ptr()))
+    sim.synchronize()
+    want, wmask, fe, fs = restrict_faces_ref(p.astype(np.float64), b.astype(np.float64),
+                                             solid, f)
+    got = rc.cpu().numpy().reshape(Hc, Wc)
+    assert np.array_equal(mc.cpu().numpy().reshape(Hc, Wc) != 0, wmask)
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+    fw = faces.cpu().numpy()
+    assert np.array_equal(fw[:Hc * Wc].reshape(Hc, Wc), fe)
+    assert np.array_equal(fw[Hc * Wc:].reshape(Hc, Wc), fs)
+    assert ((fe > 0) & (fe < 1)).any()  # the obstacle cuts faces
     _lib.call("nx_step_finish", sim._alive())
     sim.destroy()
 

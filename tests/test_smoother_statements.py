@@ -141,3 +141,56 @@ def test_fma32_is_the_correctly_rounded_fused_multiply_add(rng):
     # the tie cases are ones where rounding the float64 sum directly errs
     naive = (a[-4:-2].astype(np.float64) * b[-4:-2] + c[-4:-2]).astype(np.float32)
     assert not np.array_equal(naive, want[-4:-2])
+
+
+def test_faces_of_a_block_boundary_with_a_gap():
+    """f = 4 on an 8 x 12 grid (2 x 3 coarse cells) with a wall in fine
+    column 4 (the first column of block X = 1) open at rows 1 and 2."""
+    from smoothers import restrict_faces_ref
+    f = 4
+    solid = np.zeros((8, 12), bool)
+    solid[:, 4] = True
+    solid[1:3, 4] = False
+    b = np.arange(96, dtype=np.float64).reshape(8, 12)
+    rc, mc, fe, fs = restrict_faces_ref(np.zeros_like(b), b, solid, f)
+    # east faces: X = 0 | 1 crosses the wall (rows 1, 2 open in block Y = 0,
+    # none in Y = 1); X = 1 | 2 is open; X = 2 is the grid's edge
+    assert np.array_equal(fe, np.array([[0.5, 1.0, 0.0], [0.0, 1.0, 0.0]], np.float32))
+    # south faces: rows 3 | 4, the wall's column shut in block X = 1
+    assert np.array_equal(fs, np.array([[1.0, 0.75, 1.0], [0.0, 0.0, 0.0]], np.float32))
+    # block sums over fluid cells (p = 0: r = b on them)
+    want = np.where(solid, 0.0, b).reshape(2, 4, 3, 4).sum(axis=(1, 3))
+    assert np.allclose(rc, want) and not mc.any()
+
+
+def test_weighted_operator_reduces_to_the_plain_one(rng):
+    """Faces from a fine mask made of whole solid blocks are 0 toward solid
+    cells and 1 elsewhere: on the fluid cells the weighted float32 operator
+    is then the plain one, bitwise."""
+    from smoothers import lapc32, lapcw32, restrict_faces_ref
+    f, Hc, Wc = 4, 9, 11
+    mc = rng.random((Hc, Wc)) < 0.2
+    solid = np.kron(mc, np.ones((f, f), bool))
+    _, mc2, fe, fs = restrict_faces_ref(np.zeros(solid.shape), np.zeros(solid.shape), solid, f)
+    assert np.array_equal(mc2, mc)
+    assert set(np.unique(np.concatenate([fe.ravel(), fs.ravel()]))) <= {0.0, 1.0}
+    v = rng.normal(size=(Hc, Wc)).astype(np.float32)
+    got, want = lapcw32(v, fe, fs), lapc32(v, mc)
+    assert np.array_equal(got[~mc], want[~mc])
+
+
+def test_weighted_operator_is_symmetric(rng):
+    """sum over faces of w (v_J - v_I) is a symmetric operator: u . L v =
+    v . L u for partial weights too (to float32 round-off), which a face
+    read from the wrong cell would break by order one."""
+    from smoothers import lapcw32
+    Hc, Wc = 13, 17
+    fe = rng.choice([0.0, 0.25, 0.5, 1.0], size=(Hc, Wc)).astype(np.float32)
+    fs = rng.choice([0.0, 0.25, 0.5, 1.0], size=(Hc, Wc)).astype(np.float32)
+    u = rng.normal(size=(Hc, Wc)).astype(np.float32)
+    v = rng.normal(size=(Hc, Wc)).astype(np.float32)
+    a = float(np.dot(u.ravel().astype(np.float64), lapcw32(v, fe, fs).ravel()))
+    b = float(np.dot(v.ravel().astype(np.float64), lapcw32(u, fe, fs).ravel()))
+    assert abs(a - b) <= 1e-5 * np.linalg.norm(u) * np.linalg.norm(v) * 8
+    # and the constant is in its null space
+    assert np.abs(lapcw32(np.full((Hc, Wc), 3.0, np.float32), fe, fs)).max() == 0.0
