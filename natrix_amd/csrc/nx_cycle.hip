@@ -189,16 +189,21 @@ __global__ void __launch_bounds__(256)
   if (X >= Wp || Y >= Hp) return;
   int ne = 0, ns = 0;
   const int xe = f * X + f, ys = f * Y + f;  // first column / row of the next block
-  if (xe < W)
-    for (int k = 0; k < f; ++k) {
-      const int y = f * Y + k;
-      if (y < H) ne += (mask[y * W + xe - 1] == 0 && mask[y * W + xe] == 0) ? 1 : 0;
-    }
-  if (ys < H)
-    for (int k = 0; k < f; ++k) {
-      const int x = f * X + k;
-      if (x < W) ns += (mask[(ys - 1) * W + x] == 0 && mask[ys * W + x] == 0) ? 1 : 0;
-    }
+  // every load is unconditional from a clamped in-grid address and the
+  // conditions apply afterwards, so an unrolled batch of loads is in
+  // flight together
+  const int xa = min(xe - 1, W - 1), xb = min(xe, W - 1);
+  const int ya = min(ys - 1, H - 1), yb = min(ys, H - 1);
+  const bool east = xe < W, south = ys < H;
+#pragma unroll 8
+  for (int k = 0; k < f; ++k) {
+    const int y = f * Y + k, x = f * X + k;
+    const int yc = min(y, H - 1), xc = min(x, W - 1);
+    const uint8_t e0 = mask[yc * W + xa], e1 = mask[yc * W + xb];
+    const uint8_t s0 = mask[ya * W + xc], s1 = mask[yb * W + xc];
+    ne += (east && y < H && e0 == 0 && e1 == 0) ? 1 : 0;
+    ns += (south && x < W && s0 == 0 && s1 == 0) ? 1 : 0;
+  }
   const int pos = Y * Wp + X;
   faces[pos] = (float)ne / (float)f;
   faces[Wp * Hp + pos] = (float)ns / (float)f;
