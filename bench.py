@@ -8,7 +8,9 @@ fixed-iteration steps (80 Jacobi sweeps each).  Inputs are resident in HBM.
 
 Default workload (`--workload learned`): steps whose pressure is solved to
 FP64 relative residual <= 1e-4.  `value` is the steps/s of the learned
-two-grid cycle (U-Net coarse guess, Chebyshev fine smoothing); the same
+three-level cycle (fine grid, a middle grid 4x coarser, the U-Net's coarse
+grid; Chebyshev smoothing on the two finer levels; `--mid 0` for the
+two-level cycle); the same
 criterion with plain Jacobi correction solves is timed on one step
 (capped; its rate is then an upper bound).  The JSON line also carries the
 roofline of the learned step's dominant kernel, the fine Chebyshev
@@ -38,7 +40,8 @@ METRIC = ("sim steps/sec at 2048² @ rel. residual ≤1e-4 "
 HBM_PEAK_GBS = 8000.0
 JACOBI_BYTES_PER_CELL_SWEEP = 13   # read p 4 + b 4 + mask 1, write p 4
 CHEB_BYTES_PER_CELL_ITER = 25      # read x 4 + r 4 + d 4 + mask 1, write x, r, d 12
-CHEB_SMOOTH = 72                   # fine Chebyshev iterations per learned cycle
+CHEB_SMOOTH = 72                   # fine Chebyshev iterations per two-level cycle
+CHEB_SMOOTH_3 = 18                 # the same for the three-level cycle (--mid)
 CHEB_TILE = (8, 4, 8, 2)           # nx_smooth_chebyshev's default tile
 STEP_BYTES_PER_CELL = 104          # the other stages, FP32, dye on, nu = 0
 
@@ -473,7 +476,9 @@ def bench_learned(args, copy_bandwidth):
     from natrix_amd.learned.train import DEFAULT_WEIGHTS, load
     N = args.size
     net, meta = load(args.weights or DEFAULT_WEIGHTS)
-    guess = LearnedGuess(net, coarse=meta.get("size", 128))
+    guess = LearnedGuess(net, coarse=meta.get("size", 128), mid=args.mid,
+                         mid_iters=args.mid_iters)
+    smooth_default = args.smooth or (CHEB_SMOOTH_3 if args.mid > 1 else CHEB_SMOOTH)
     solid, centre = make_scene(N)
     dt = 1.0 / 60.0
     cells = N * N
@@ -495,7 +500,8 @@ def bench_learned(args, copy_bandwidth):
         sim.synchronize()
         return sim
 
-    def timed_learned(sim, steps, g=guess, smooth=CHEB_SMOOTH):
+    def timed_learned(sim, steps, g=guess, smooth=None):
+        smooth = smooth or smooth_default
         for _ in range(2):  # first-call costs (allocator, code objects) off the clock
             solve_step(sim, g, dt, tol=args.tol, smooth=smooth)
         sim.synchronize()
@@ -541,15 +547,16 @@ def bench_learned(args, copy_bandwidth):
     if args.classical_coarse > 0:
         # the same two-grid cycle with the network off: the coarse problem
         # is smoothed from zero with more Chebyshev iterations instead (the
-        # method without the learned guess, at its best measured setting:
-        # 350 coarse and 72 fine iterations per cycle)
+        # method without the learned guess: 350 coarse iterations, the
+        # learned cycle's shape and fine iterations otherwise)
         g0 = LearnedGuess(net, coarse=meta.get("size", 128),
-                          coarse_iters=args.classical_coarse, use_network=False)
+                          coarse_iters=args.classical_coarse, use_network=False, mid=args.mid,
+                          mid_iters=args.mid_iters)
         sim = fresh()
         classical = timed_learned(sim, args.steps, g0, smooth=args.classical_smooth)
         sim.destroy()
         classical["coarse_iterations"] = args.classical_coarse
-        classical["fine_iterations_per_cycle"] = args.classical_smooth
+        classical["fine_iterations_per_cycle"] = args.classical_smooth or smooth_default
         classical["flow"] = learned["flow"]
         log(f"bench two-grid without the network: {classical}")
 
@@ -579,7 +586,7 @@ def bench_learned(args, copy_bandwidth):
     sim._push_params()
     _lib.call("nx_step_begin", sim._alive(), dt)
     lmin = guess.fine_lmin(N, N)
-    n_iter, calls = CHEB_SMOOTH, 20
+    n_iter, calls = smooth_default, 20
     tile = (C.c_int * 5)()
     _lib.call("nx_get_smoother_tile", sim._alive(), tile)
     tile = tuple(tile) if tile[4] > 0 else tuple(tile)[:4]
@@ -603,7 +610,7 @@ def bench_learned(args, copy_bandwidth):
         valu=valu_entry(f"{kname}_{N}_iters{n_iter}", 1e3 * ch_ms / ch_n, grid=N,
                         iterations=n_iter, tile=list(tile)),
         iterations_per_launch=round(n_iter / len(plan), 3),
-        algorithmic_bytes_note="per call of 72 iterations: the first launch reads x, b, mask "
+        algorithmic_bytes_note=f"per call of {n_iter} iterations: the first launch reads x, b, mask "
                                "and writes x, r, d (21 B/cell), the middle ones read x, r, d, "
                                "mask and write x, r, d (25 B/cell), the last reads x, r, d, b, "
                                "mask and writes x (21 B/cell); averaged per launch",
@@ -677,8 +684,15 @@ def bench_learned(args, copy_bandwidth):
            "config": {"workload": f"fluid step {N}x{N} f32, cylinder r=0.1N at "
                                   f"({centre[0]:.3f},{centre[1]:.3f}), left inflow 1.0, vorticity 0.3, "
                                   f"pressure to FP64 rel. residual <= {args.tol:g}",
-                      "grid": N, "solver": "learned two-grid cycle (U-Net bf16 MFMA guess, "
-                                           "Chebyshev coarse and fine smoothing), FP64 refinement",
+                      "grid": N,
+                      "solver": (f"learned three-level cycle ({N}² -> {N // args.mid}² -> "
+                                 f"network grid; U-Net bf16 MFMA guess, face-weighted coarse "
+                                 f"operators, Chebyshev smoothing: {args.mid_iters} middle and "
+                                 f"{smooth_default} fine iterations per cycle), FP64 refinement"
+                                 if args.mid > 1 else
+                                 "learned two-grid cycle (U-Net bf16 MFMA guess, face-weighted "
+                                 "coarse operator, Chebyshev coarse and fine smoothing), FP64 "
+                                 "refinement"),
                       "weights": os.path.basename(args.weights or DEFAULT_WEIGHTS)},
            "learned": learned, "learned_developed_flow": developed,
            "two_grid_without_network": classical, "plain": plain,
@@ -735,8 +749,17 @@ def main():
     ap.add_argument("--classical-coarse", type=int, default=350,
                     help="learned workload: also time the cycle without the network, "
                          "with this many coarse Chebyshev iterations (0: skip)")
-    ap.add_argument("--classical-smooth", type=int, default=CHEB_SMOOTH,
-                    help="fine Chebyshev iterations per cycle of the cycle without the network")
+    ap.add_argument("--classical-smooth", type=int, default=0,
+                    help="fine Chebyshev iterations per cycle of the cycle without the network "
+                         "(0: as the learned cycle)")
+    ap.add_argument("--mid", type=int, default=4,
+                    help="learned workload: middle grid factor of the three-level cycle "
+                         "(0: the two-level cycle)")
+    ap.add_argument("--mid-iters", type=int, default=12,
+                    help="Chebyshev iterations on the middle grid per cycle")
+    ap.add_argument("--smooth", type=int, default=0,
+                    help=f"fine Chebyshev iterations per cycle (0: {CHEB_SMOOTH_3} with --mid, "
+                         f"{CHEB_SMOOTH} without)")
     ap.add_argument("--developed-steps", type=int, default=10,
                     help="learned workload: timed steps of the developed-flow leg")
     ap.add_argument("--fixed-steps", type=int, default=100,

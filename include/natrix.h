@@ -455,6 +455,15 @@ int nx_unet_coarse(nx_unet* net, float** rc, uint8_t** mc, float** e);
  * network, to measure what the network contributes); 0 restores it.  No
  * reference counterpart. */
 int nx_unet_set_bypass(nx_unet* net, int bypass);
+/* three-level cycle (no reference counterpart): with mid_factor >= 2 each
+ * correction with Chebyshev coarse smoothing restricts to a middle grid
+ * mid_factor times coarser than the fine one (block sums and face
+ * weights), from there to the network's grid (the correction's factor must
+ * be a multiple of mid_factor), and smooths the middle problem with
+ * mid_iters Chebyshev iterations of its face-weighted operator on
+ * [mid_lambda_min, 8] before interpolating to the fine grid with the line
+ * search; mid_factor = 0 restores the two-grid cycle */
+int nx_unet_set_levels(nx_unet* net, int mid_factor, int mid_iters, float mid_lambda_min);
 
 /* ---- FP64 oracle solve on the GPU (conjugate gradients on the same
  *      operator, fluid cells, right-hand side centred on its fluid mean) ---- */

@@ -168,6 +168,7 @@ SIGNATURES = {
                           C.POINTER(_D), _I, C.POINTER(_I)],
     "nx_unet_coarse": [_P, _P, _P, _P],
     "nx_unet_set_bypass": [_P, _I],
+    "nx_unet_set_levels": [_P, _I, _I, _F],
     "nx_get_size": [_P, _P, _P],
     "nx_conv3x3_dgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
     "nx_conv3x3_wgrad": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P],

@@ -209,6 +209,33 @@ __global__ void __launch_bounds__(256)
   faces[Wp * Hp + pos] = (float)ns / (float)f;
 }
 
+// Block sums of a restricted field onto a grid f times coarser (the middle
+// level of a three-level cycle onto the network's grid): rc = the sum of
+// rm over the fluid cells of each f x f block of the Wm x Hm grid (rm is 0
+// on its solid cells), mc = 1 when the block has none; cells of a padded
+// Wc x Hc grid beyond the blocks that cover Wm x Hm are solid.
+__global__ void __launch_bounds__(256)
+    k_block_sum(const float* __restrict__ rm, const uint8_t* __restrict__ mm, int Wm, int Hm,
+                int f, float* __restrict__ rc, uint8_t* __restrict__ mc, int Wc, int Hc) {
+  const int X = blockIdx.x * 64 + threadIdx.x, Y = blockIdx.y * 4 + threadIdx.y;
+  if (X >= Wc || Y >= Hc) return;
+  float s = 0.0f;
+  int n = 0;
+  for (int k = 0; k < f; ++k) {
+    const int y = f * Y + k;
+    if (y >= Hm) break;
+    for (int j = 0; j < f; ++j) {
+      const int x = f * X + j;
+      if (x < Wm && mm[y * Wm + x] == 0) {
+        s += rm[y * Wm + x];
+        ++n;
+      }
+    }
+  }
+  rc[Y * Wc + X] = n ? s : 0.0f;
+  mc[Y * Wc + X] = n ? 0 : 1;
+}
+
 // Prolongation: p += scale * e interpolated at the fine cell, on fluid
 // cells.  Coarse cell centres sit at fine coordinate f*I + (f-1)/2
 // (clamped at the edges); the four bilinear weights are renormalised over
@@ -1217,6 +1244,26 @@ namespace nx {
 int restrict_residual_padded(nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc,
                              float* faces) {
   return restrict_impl(s, factor, Wp, Hp, rc, mc, faces);
+}
+
+int block_sum_restrict(hipStream_t st, const float* rm, const uint8_t* mm, int Wm, int Hm, int f,
+                       float* rc, uint8_t* mc, int Wc, int Hc) {
+  k_block_sum<<<dim3((Wc + 63) / 64, (Hc + 3) / 4), dim3(64, 4), 0, st>>>(rm, mm, Wm, Hm, f, rc,
+                                                                         mc, Wc, Hc);
+  return hipGetLastError() == hipSuccess ? NX_OK : set_error(NX_ERR_HIP, "block sum failed");
+}
+
+int coarse_faces_padded(nx_sim* s, int f, int Wp, int Hp, float* faces) {
+  k_coarse_faces<<<dim3((Wp + 63) / 64, (Hp + 3) / 4), dim3(64, 4), 0, s->stream>>>(
+      s->mask[s->mcur], s->W, s->H, f, Wp, Hp, faces);
+  return hipGetLastError() == hipSuccess ? NX_OK : set_error(NX_ERR_HIP, "faces failed");
+}
+
+int prolong_to(hipStream_t st, float* e, const uint8_t* m, int W, int H, int f, const float* ec,
+               const uint8_t* mc, int Wc, int Hc) {
+  k_prolong<false><<<dim3((W + 63) / 64, (H + 3) / 4), dim3(64, 4), 0, st>>>(
+      e, m, ec, mc, W, H, f, Wc, Hc, 1.0f);
+  return hipGetLastError() == hipSuccess ? NX_OK : set_error(NX_ERR_HIP, "prolongation failed");
 }
 int prolong_line_search_padded(nx_sim* s, int factor, int Wp, int Hp, const float* e,
                                const uint8_t* mc, double* out, bool apply) {

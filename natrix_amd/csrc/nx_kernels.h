@@ -89,6 +89,15 @@ bool jacobi_cfg_supported(JacobiCfg cfg);
 // coarse operator.
 int restrict_residual_padded(::nx_sim* s, int factor, int Wp, int Hp, float* rc, uint8_t* mc,
                              float* faces = nullptr);
+// Middle level of a three-level cycle: block sums of a restricted field
+// onto a grid f times coarser (padded cells solid), the coarse faces of
+// the fine mask at factor f on a Wp x Hp grid, and bilinear interpolation
+// e = ec (no line search) onto a W x H grid with mask m.
+int block_sum_restrict(hipStream_t st, const float* rm, const uint8_t* mm, int Wm, int Hm, int f,
+                       float* rc, uint8_t* mc, int Wc, int Hc);
+int coarse_faces_padded(::nx_sim* s, int f, int Wp, int Hp, float* faces);
+int prolong_to(hipStream_t st, float* e, const uint8_t* m, int W, int H, int f, const float* ec,
+               const uint8_t* mc, int Wc, int Hc);
 int prolong_line_search_padded(::nx_sim* s, int factor, int Wp, int Hp, const float* e,
                                const uint8_t* mc, double* out, bool apply = true);
 // nx_smooth_chebyshev taking first the line-search step that a

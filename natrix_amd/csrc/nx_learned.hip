@@ -66,6 +66,14 @@ struct nx_unet {
   float* scal = nullptr;   // [0] = s (RMS of rc over fluid cells)
   float* cws = nullptr;    // Chebyshev workspace, 6 H W floats
   float* faces = nullptr;  // coarse face weights, 2 H W floats (east, south)
+  // three-level cycle (mid_f > 1): a middle grid mid_f times coarser than
+  // the fine one, smoothed with mid_iters Chebyshev iterations on
+  // [mid_lmin, 8] of its face-weighted operator; buffers for Wm x Hm
+  int mid_f = 0, mid_iters = 0;
+  float mid_lmin = 0.0f;
+  int Wm = 0, Hm = 0;
+  float *rm = nullptr, *em = nullptr, *mfaces = nullptr, *mws = nullptr;
+  uint8_t* mm = nullptr;
   // steps 2-6 captured as one HIP graph (every pointer in them is the
   // runner's own), keyed by the stream and the smoothing parameters; the
   // capture happens on the second call with the same key, when everything
@@ -428,6 +436,8 @@ int nx_unet_destroy(nx_unet* u) {
   if (!u) return NX_OK;
   if (u->graph) (void)hipGraphExecDestroy(u->graph);
   for (void* p : u->bufs) (void)hipFree(p);
+  for (void* p : {(void*)u->rm, (void*)u->em, (void*)u->mfaces, (void*)u->mws, (void*)u->mm})
+    if (p) (void)hipFree(p);
   delete u;
   return NX_OK;
 }
@@ -454,6 +464,46 @@ static int learned_correct_impl(nx_sim* sim, nx_unet* u, int factor, int coarse_
   if (rc != NX_OK) return rc;
   if ((W + factor - 1) / factor > u->W || (H + factor - 1) / factor > u->H)
     return fail(NX_ERR_ARG, "the network's grid is smaller than the fine grid / factor");
+  if (u->mid_f > 1 && lambda_min > 0.0f) {
+    // three levels: fine -> middle (block sums and faces), middle -> the
+    // network's grid (block sums; faces from the fine mask at the full
+    // factor), the coarse pipeline, interpolation to the middle grid, its
+    // Chebyshev smoothing, interpolation to the fine grid with the line
+    // search
+    const int fm = u->mid_f, f2 = factor / fm;
+    if (f2 < 2 || f2 * fm != factor) return fail(NX_ERR_ARG, "factor must be a multiple of the middle factor");
+    const int Wm = (W + fm - 1) / fm, Hm = (H + fm - 1) / fm;
+    if ((Wm + f2 - 1) / f2 > u->W || (Hm + f2 - 1) / f2 > u->H)
+      return fail(NX_ERR_ARG, "the network's grid is smaller than the middle grid / factor");
+    if (Wm != u->Wm || Hm != u->Hm) {
+      for (void* p : {(void*)u->rm, (void*)u->em, (void*)u->mfaces, (void*)u->mws, (void*)u->mm})
+        if (p) (void)hipFree(p);
+      u->rm = u->em = u->mfaces = u->mws = nullptr;
+      u->mm = nullptr;
+      u->Wm = u->Hm = 0;
+      const size_t n = (size_t)Wm * Hm;
+      if (hipMalloc(&u->rm, n * 4) != hipSuccess || hipMalloc(&u->em, n * 4) != hipSuccess ||
+          hipMalloc(&u->mfaces, 2 * n * 4) != hipSuccess ||
+          hipMalloc(&u->mws, 6 * n * 4) != hipSuccess || hipMalloc(&u->mm, n) != hipSuccess)
+        return fail(NX_ERR_HIP, "middle-level buffers");
+      u->Wm = Wm;
+      u->Hm = Hm;
+    }
+    rc = nx::restrict_residual_padded(sim, fm, Wm, Hm, u->rm, u->mm, u->mfaces);
+    if (rc != NX_OK) return rc;
+    rc = nx::block_sum_restrict(s, u->rm, u->mm, Wm, Hm, f2, u->rc, u->mc, u->W, u->H);
+    if (rc != NX_OK) return rc;
+    rc = nx::coarse_faces_padded(sim, factor, u->W, u->H, u->faces);
+    if (rc != NX_OK) return rc;
+    rc = run_pipeline(u, coarse_iters, lambda_min, s);
+    if (rc != NX_OK) return rc;
+    rc = nx::prolong_to(s, u->em, u->mm, Wm, Hm, f2, u->out, u->mc, u->W, u->H);
+    if (rc != NX_OK) return rc;
+    rc = nx::launch_coarse_smooth(s, Wm, Hm, u->em, u->rm, u->mm, u->mid_iters, u->mid_lmin,
+                                  u->mws, u->mfaces);
+    if (rc != NX_OK) return rc;
+    return nx::prolong_line_search_padded(sim, fm, Wm, Hm, u->em, u->mm, out, apply);
+  }
   rc = nx::restrict_residual_padded(sim, factor, u->W, u->H, u->rc, u->mc,
                                     lambda_min > 0.0f ? u->faces : nullptr);
   if (rc != NX_OK) return rc;
@@ -522,6 +572,20 @@ int nx_learned_cycles(nx_sim* sim, nx_unet* u, int factor, int coarse_iters,
   result[1] = kept;
   result[2] = converged;
   result[3] = abandoned;
+  return NX_OK;
+}
+
+int nx_unet_set_levels(nx_unet* u, int mid_factor, int mid_iters, float mid_lambda_min) {
+  if (!u) return fail(NX_ERR_ARG, "null argument");
+  if (mid_factor == 0) {
+    u->mid_f = 0;
+    return NX_OK;
+  }
+  if (mid_factor < 2 || mid_iters < 1 || !(mid_lambda_min > 0.0f && mid_lambda_min < 8.0f))
+    return fail(NX_ERR_ARG, "middle level: factor >= 2, iterations >= 1, 0 < lambda_min < 8");
+  u->mid_f = mid_factor;
+  u->mid_iters = mid_iters;
+  u->mid_lmin = mid_lambda_min;
   return NX_OK;
 }
 

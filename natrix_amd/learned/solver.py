@@ -104,10 +104,16 @@ class LearnedGuess:
     run through PyTorch."""
 
     def __init__(self, net, coarse=128, coarse_iters=100, damping=1e-3, native=True,
-                 use_network=True):
+                 use_network=True, mid=0, mid_iters=12):
         """use_network=False keeps the native runner but starts every coarse
         smoothing from zero (nx_unet_set_bypass): the same cycle, launches
-        and timing path without the network's guess."""
+        and timing path without the network's guess.  mid >= 2 (native
+        runner only): a three-level cycle through a middle grid mid times
+        coarser than the fine one, smoothed with mid_iters Chebyshev
+        iterations of its face-weighted operator on [1.536 / (f / mid)^2,
+        8] (nx_unet_set_levels); the fine interval's lower end is then
+        1.536 / mid^2."""
+        self.mid, self.mid_iters = int(mid), int(mid_iters)
         self.net = net.eval() if net is not None else None
         self.use_network = bool(use_network)
         if coarse < 8 or coarse % 8:
@@ -133,7 +139,7 @@ class LearnedGuess:
         """Lower end of the fine Chebyshev smoother's interval, 1.536 / f^2
         (0.006 at f = 16): the modes of -L below it (wavelengths beyond
         about 4 f cells) are left to the coarse correction."""
-        f = self.coarse_grid(W, H)[0]
+        f = self.mid if self.mid > 1 else self.coarse_grid(W, H)[0]
         return 1.536 / (f * f)
 
     def _runner(self, wc, hc):
@@ -151,7 +157,22 @@ class LearnedGuess:
         """(f, runner) when corrections of `sim` run natively, else None."""
         f, wc, hc = self.coarse_grid(sim.width, sim.height)
         runner = self._runner(wc, hc)
+        if runner is not None:
+            self._levels(runner, f)
         return None if runner is None else (f, runner)
+
+    def _levels(self, runner, f):
+        """The runner's cycle shape for factor f: three levels with the
+        middle interval's lower end 1.536 / (f / mid)^2, or two."""
+        if self.mid > 1:
+            if f % self.mid or f // self.mid < 2:
+                raise ValueError(f"factor {f} is not a multiple >= 2 x of the middle factor "
+                                 f"{self.mid}")
+            f2 = f // self.mid
+            _lib.call("nx_unet_set_levels", runner.handle, self.mid, self.mid_iters,
+                      1.536 / (f2 * f2))
+        else:
+            _lib.call("nx_unet_set_levels", runner.handle, 0, 0, 0.0)
 
     def correct(self, sim, report=False, line_search=True):
         """Add one learned correction to the simulator's pressure (on the
@@ -160,6 +181,10 @@ class LearnedGuess:
         (synchronises); otherwise None."""
         f, wc, hc = self.coarse_grid(sim.width, sim.height)
         runner = self._runner(wc, hc)
+        if runner is not None:
+            self._levels(runner, f)
+        elif self.mid > 1:
+            raise ValueError("the three-level cycle runs only in the native runner")
         if runner is not None and line_search:
             # the native runner enqueues everything on the simulator's stream
             out = (C.c_double * 3)() if report else None

@@ -249,6 +249,51 @@ def test_learned_solve_1024_unseen_seeds(gpu, kind, seed):
     sim.destroy()
 
 
+@pytest.mark.parametrize("kind,seed", [("cylinder", 900001), ("rectangles", 900002)])
+def test_three_level_cycle_1024_unseen_seeds(gpu, kind, seed):
+    """The three-level cycle (1024² fine grid, a 256² middle grid smoothed
+    with 12 Chebyshev iterations of its face-weighted operator, the
+    network's 128² grid; 18 fine iterations per cycle on [0.096, 8]) on the
+    unseen scenes of the two-level test: every step meets FP64 relative
+    residual <= 1e-4 without the plain fallback, and the pressure is as
+    close to the FP64 solution as the residual allows (the bound of the
+    two-level test)."""
+    from natrix_amd import fp64
+    from natrix_amd.learned.solver import LearnedGuess, solve_step
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    g = LearnedGuess(net, meta.get("size", 128), mid=4, mid_iters=12)
+    assert g.fine_lmin(1024, 1024) == pytest.approx(0.096)
+    sim, splats = unseen_scene_1024(kind, seed)
+    for _ in range(10):
+        if splats:
+            sim.add_splats(splats)
+        sim.update(1 / 60)
+    for _ in range(3):
+        if splats:
+            sim.add_splats(splats)
+        st = solve_step(sim, g, 1 / 60, tol=1e-4, smooth=18)
+        print(kind, seed, st)
+        assert st["converged"] and st["residual"] <= 1e-4
+        assert not st["fallback"]
+    x, info = fp64.solve(sim, tol=1e-11, max_iter=100000)
+    err = fp64.pressure_error(sim, x)
+    print(f"{kind} {seed}: three levels, FP64 relative error {err:.3e}")
+    assert err <= 200 * st["residual"] and err <= 2e-2
+    sim.destroy()
+
+
+def test_three_level_cycle_needs_a_divisible_factor(gpu):
+    from natrix_amd.learned.solver import LearnedGuess
+    from natrix_amd.learned.train import load
+    net, meta = load()
+    g = LearnedGuess(net, meta.get("size", 128), mid=4)
+    sim = make_sim(256, 256)  # f = 2: no middle grid fits
+    with pytest.raises(ValueError):
+        g.native(sim)
+    sim.destroy()
+
+
 def test_network_share_of_the_learned_step_2048(gpu):
     """What the network contributes at the headline size: the same 2048²
     cylinder scene (10 developed steps) and the same two-grid cycle with
